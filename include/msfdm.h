@@ -1,0 +1,161 @@
+/*
+ * msfdm.h - C ABI of the MI355X fluid-solid time stepper (libmsfdm.so).
+ *
+ * Drop-in replacement for the time loop of the reference's src/main.cu driven
+ * by PmlFs::FluidSolid::update_RK4 (include/pml_fs_kernels.cuh:848-852).  The
+ * reference has no library API: its boundaries are the process
+ * (bin/main_debug <model.json> <out_root>, src/main.cu:58-77) and the
+ * in-process MemManager + update_RK4 pair (include/elements.h:112-173).  Each
+ * entry point below names the reference interface it replaces.
+ *
+ * Conventions: every function returns MSFDM_OK (0) or an error code and never
+ * exits or throws; msfdm_last_error() returns the calling thread's last error
+ * message.  Host buffers are caller-owned; device memory is owned by the
+ * solver handle.  Arrays use the reference layout: float32, row-major
+ * [rows][nx], x fastest (docs/io.md:5-12).
+ */
+#ifndef MSFDM_H
+#define MSFDM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MSFDM_OK = 0,
+    MSFDM_EINVAL = 1,   /* invalid argument or inconsistent model */
+    MSFDM_EIO = 2,      /* file cannot be opened, read or written */
+    MSFDM_EPARSE = 3,   /* malformed JSON, missing key or wrong type */
+    MSFDM_EDEVICE = 4,  /* HIP runtime error / no device */
+    MSFDM_ESTATE = 5    /* call not valid in the current state */
+};
+
+/* Field identifiers for msfdm_get_field / msfdm_set_field.  U_fluid, U_solid and
+ * the ADE memory sets of include/elements.h:6-26 (fluid zeta memory is never
+ * driven by the reference and is not stored). */
+enum {
+    MSFDM_FVX = 0, MSFDM_FVZ, MSFDM_FP,
+    MSFDM_FXVX, MSFDM_FXVZ, MSFDM_FXEX, MSFDM_FXEZ,
+    MSFDM_SVX, MSFDM_SVZ, MSFDM_SSX, MSFDM_SSZ, MSFDM_SSXZ,
+    MSFDM_SXVX, MSFDM_SXVZ, MSFDM_SXEX, MSFDM_SXEZ,
+    MSFDM_SZVX, MSFDM_SZVZ, MSFDM_SZEX, MSFDM_SZEZ,
+    MSFDM_NFIELD
+};
+
+/* Model arrays for msfdm_model_get_array. */
+enum {
+    MSFDM_A_K = 0, MSFDM_A_RHO_F, MSFDM_A_C11, MSFDM_A_C13, MSFDM_A_C33, MSFDM_A_C55,
+    MSFDM_A_RHO_S, MSFDM_A_J, MSFDM_A_XI_X, MSFDM_A_XI_Z, MSFDM_A_ZE_X, MSFDM_A_ZE_Z,
+    MSFDM_NARRAY
+};
+
+/* Scalar part of the model: the JSON keys read by MemManager::MemManager
+ * (src/elements.cu:72-111).  Only calc_type = 1 (FANDS) with
+ * boundary_type = 2 (PANDF) is accepted. */
+typedef struct msfdm_desc {
+    int nx, nz_fluid, nz_solid;
+    int calc_type, boundary_type;
+    float fpeak, dt;
+    int nt, snapshot;
+    int npd;
+    float thickness;
+    int npow;
+    float vp_max, rc, beta0;
+    int nsrc;
+    const int *src_x; /* fluid-array column of each shot (grid.src_pos.posx) */
+    const int *src_z; /* fluid-array row of each shot (grid.src_pos.posz) */
+} msfdm_desc;
+
+typedef struct msfdm_model msfdm_model;
+typedef struct msfdm_solver msfdm_solver;
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *msfdm_last_error(void);
+/* Library version string. */
+const char *msfdm_version(void);
+
+/* --- Host model (no GPU needed) ------------------------------------------ */
+
+/* Parse <model.json> and read its 12 <f4 arrays (paths relative to the CWD).
+ * Replaces JsonConfig + MemManager::MemManager's input half
+ * (include/json_config.hpp:20-368, src/elements.cu:64-176). */
+int msfdm_model_load_json(const char *json_path, msfdm_model **out);
+
+/* Build a model from in-memory arrays (same shapes as the files). */
+int msfdm_model_create(const msfdm_desc *desc, const float *K, const float *rho_f,
+                       const float *C11, const float *C13, const float *C33, const float *C55,
+                       const float *rho_s, const float *J, const float *xi_x, const float *xi_z,
+                       const float *ze_x, const float *ze_z, msfdm_model **out);
+
+/* Copy the scalars; desc->src_x/src_z point into the model (valid while it lives). */
+int msfdm_model_get_desc(const msfdm_model *m, msfdm_desc *out);
+/* Copy one array (MSFDM_A_*) into out ([rows][nx] floats). */
+int msfdm_model_get_array(const msfdm_model *m, int which, float *out);
+void msfdm_model_destroy(msfdm_model *m);
+
+/* --- Device solver ------------------------------------------------------- */
+
+/* Upload the model, build PML profiles, normals and the source table, and
+ * allocate the wavefield on HIP device `device`.  Replaces the device half of
+ * MemManager (src/elements.cu:557-905, 979-987) and grid_cp_h2d
+ * (src/elements.cu:307-333). */
+int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solver **out);
+void msfdm_solver_destroy(msfdm_solver *s);
+
+/* Zero the wavefield, the records and the step counter and select shot isrc.
+ * (The reference keeps the previous shot's wavefield, src/main.cu:94-136.) */
+int msfdm_reset_shot(msfdm_solver *s, int isrc);
+
+/* Advance nsteps RK4 steps (each = PmlFs::FluidSolid::update_RK4 with
+ * rk4dire = it & 1, src/main.cu:136-176, plus the interface filter), sampling
+ * the seafloor receiver rows on the device after steps with it % gap == 0
+ * (gap = (int)(1e-3/dt + 0.5), src/main.cu:92).  Asynchronous. */
+int msfdm_run(msfdm_solver *s, int nsteps);
+/* Wait for all queued work. */
+int msfdm_synchronize(msfdm_solver *s);
+
+/* Steps done since the last reset. */
+int msfdm_step_index(const msfdm_solver *s, int *it);
+/* Records available (= number of it with it % gap == 0 done so far). */
+int msfdm_num_records(const msfdm_solver *s, int *n);
+/* Copy records [first, first+count) of solid-row-0 vx, vz and fluid interface
+ * row p (each count x nx floats, time-major): output_record, src/main.cu:245-295.
+ * Any output pointer may be NULL. */
+int msfdm_get_records(msfdm_solver *s, int first, int count, float *vx, float *vz, float *p);
+
+/* Full-array field access in the reference layout (ADE fields expand to the
+ * full [rows][nx] array, zero outside the PML strips).  U_cp_d2h,
+ * src/elements.cu:335-343. */
+int msfdm_get_field(msfdm_solver *s, int field, float *out);
+int msfdm_set_field(msfdm_solver *s, int field, const float *in);
+
+/* Evaluate one RK stage right-hand side k = BC(L_dir(U) + src) of the current
+ * state (dir 0 = F, 1 = B; amp = Ricker amplitude added at the source) and
+ * keep it for msfdm_get_rhs: linear_operate + fl_so_apply,
+ * src/pml_fs_kernels.cu:533-660. */
+int msfdm_eval_rhs(msfdm_solver *s, int dir, float amp);
+int msfdm_get_rhs(msfdm_solver *s, int field, float *out);
+
+/* Enable (1) / disable (0) hipGraph replay of the time loop (default 1). */
+int msfdm_set_graph(msfdm_solver *s, int enable);
+/* Time `reps` launches of one kernel class on the solver's stream with HIP
+ * events (which: 0 = fluid stage, 1 = solid stage, 2 = interface stage,
+ * 3 = interface stage + filter); returns the mean ms per launch. */
+int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms);
+/* Time nsteps of msfdm_run between HIP events on the solver's stream. */
+int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
+/* Algorithmic HBM bytes of one time step (state + coefficients + PML memory). */
+int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
+
+/* --- Process boundary ---------------------------------------------------- */
+
+/* The whole bin/main_debug run: argument guards, output-dir reset, shot loop,
+ * snapshots and records in the src/main.cu:58-339 layout.  Returns the process
+ * exit code (0 ok, 1 error) with the reason in msfdm_last_error(). */
+int msfdm_run_cli(const char *json_path, const char *out_root);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSFDM_H */

@@ -1,0 +1,137 @@
+// Process boundary: the bin/main_debug run (src/main.cu:58-339) on top of the C ABI.
+//
+// Output layout under <out_root> (cleared first, src/main.cu:82-89):
+//   <shot>/snapshot/solid/{vz,vx}/{vz,vx}_%06d.bin   full [nz_s][nx] arrays
+//   <shot>/snapshot/fluid/{vz,vx}/{vz,vx}_%06d.bin   full [nz_f][nx] arrays
+//   <shot>/record/solid_vx.bin, solid_vz.bin          solid row 0, [nrec][nx]
+//   <shot>/record/fluid_p.bin                         fluid row nz_f-1, [nrec][nx]
+// Snapshots are taken after steps with it % snapshot == 0, records after steps
+// with it % gap == 0 (src/main.cu:194-211).  Each shot starts from a zero
+// wavefield (the reference carries the previous shot's field over).
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <system_error>
+#include <vector>
+
+#include "model.hpp"
+
+namespace fs = std::filesystem;
+using namespace msfdm;
+
+namespace {
+
+struct ModelDel { void operator()(msfdm_model *m) const { msfdm_model_destroy(m); } };
+struct SolverDel { void operator()(msfdm_solver *s) const { msfdm_solver_destroy(s); } };
+
+int fail(const std::string &msg) {
+    set_error(MSFDM_EIO, msg);
+    return 1;
+}
+
+bool make_dirs(const fs::path &p, std::string &err) {
+    std::error_code ec;
+    fs::create_directories(p, ec);
+    if (ec) {
+        err = "cannot create " + p.string() + ": " + ec.message();
+        return false;
+    }
+    return true;
+}
+
+// write_field, src/main.cu:225-243
+bool write_field(const fs::path &dir, const char *field, int it, const std::vector<float> &data, std::string &err) {
+    if (!make_dirs(dir, err)) return false;
+    std::ostringstream name;
+    name << field << "_" << std::setw(6) << std::setfill('0') << it << ".bin";
+    const fs::path file = dir / name.str();
+    std::ofstream fp(file, std::ios::binary);
+    if (!fp) { err = "cannot write " + file.string(); return false; }
+    fp.write(reinterpret_cast<const char *>(data.data()), (std::streamsize)(data.size() * sizeof(float)));
+    if (!fp) { err = "cannot write " + file.string(); return false; }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
+    clear_error();
+    if (!json_path || !out_root_c) return fail("usage: main_debug <model.json> <output_dir>");
+    const std::string out_root_s = out_root_c;
+    if (out_root_s.empty() || out_root_s == "/" || out_root_s == "." || out_root_s == "..")
+        return fail("Invalid output dir: " + out_root_s);
+    const fs::path out_root = out_root_s;
+
+    msfdm_model *mraw = nullptr;
+    if (msfdm_model_load_json(json_path, &mraw) != MSFDM_OK) return 1;
+    std::unique_ptr<msfdm_model, ModelDel> model(mraw);
+    msfdm_desc d{};
+    msfdm_model_get_desc(model.get(), &d);
+
+    msfdm_solver *sraw = nullptr;
+    if (msfdm_solver_create(model.get(), 0, &sraw) != MSFDM_OK) return 1;
+    std::unique_ptr<msfdm_solver, SolverDel> solver(sraw);
+
+    std::error_code ec;
+    fs::remove_all(out_root, ec);
+    if (ec) return fail("cannot clear " + out_root.string() + ": " + ec.message());
+    std::string err;
+    if (!make_dirs(out_root, err)) return fail(err);
+    std::printf("[output] cleared %s\n", out_root_s.c_str());
+
+    const int nx = d.nx;
+    const size_t nf = (size_t)d.nz_fluid * nx, ns = (size_t)d.nz_solid * nx;
+    std::vector<float> svz(ns), svx(ns), fvz(nf), fvx(nf);
+    for (int isrc = 0; isrc < d.nsrc; ++isrc) {
+        const fs::path shot = out_root / std::to_string(isrc);
+        if (!make_dirs(shot / "snapshot", err) || !make_dirs(shot / "record", err)) return fail(err);
+        std::ofstream rvx(shot / "record" / "solid_vx.bin", std::ios::binary | std::ios::trunc);
+        std::ofstream rvz(shot / "record" / "solid_vz.bin", std::ios::binary | std::ios::trunc);
+        std::ofstream rp(shot / "record" / "fluid_p.bin", std::ios::binary | std::ios::trunc);
+        if (!rp) return fail("cannot open fluid record files");
+        if (!rvx || !rvz) return fail("cannot open solid record files");
+        if (msfdm_reset_shot(solver.get(), isrc) != MSFDM_OK) return 1;
+
+        int it = 0;
+        while (it < d.nt) {
+            // next snapshot step at or after it
+            const int snap = ((it + d.snapshot - 1) / d.snapshot) * d.snapshot;
+            const int stop = snap < d.nt ? snap + 1 : d.nt;
+            if (msfdm_run(solver.get(), stop - it) != MSFDM_OK) return 1;
+            it = stop;
+            if (snap < d.nt) {
+                std::printf("finished %0.2f%%\r", 100.0 * snap / d.nt);
+                std::fflush(stdout);
+                if (msfdm_get_field(solver.get(), MSFDM_SVZ, svz.data()) || msfdm_get_field(solver.get(), MSFDM_SVX, svx.data()) ||
+                    msfdm_get_field(solver.get(), MSFDM_FVZ, fvz.data()) || msfdm_get_field(solver.get(), MSFDM_FVX, fvx.data()))
+                    return 1;
+                const fs::path sd = shot / "snapshot";
+                if (!write_field(sd / "solid" / "vz", "vz", snap, svz, err) || !write_field(sd / "solid" / "vx", "vx", snap, svx, err) ||
+                    !write_field(sd / "fluid" / "vz", "vz", snap, fvz, err) || !write_field(sd / "fluid" / "vx", "vx", snap, fvx, err))
+                    return fail(err);
+            }
+        }
+        if (msfdm_synchronize(solver.get()) != MSFDM_OK) return 1;
+        // records: time-major [nrec][nx] (output_record, src/main.cu:245-295)
+        int nrec = 0;
+        msfdm_num_records(solver.get(), &nrec);
+        const int chunk = 256;
+        std::vector<float> bx((size_t)chunk * nx), bz((size_t)chunk * nx), bp((size_t)chunk * nx);
+        for (int r0 = 0; r0 < nrec; r0 += chunk) {
+            const int n = std::min(chunk, nrec - r0);
+            if (msfdm_get_records(solver.get(), r0, n, bx.data(), bz.data(), bp.data()) != MSFDM_OK) return 1;
+            const std::streamsize bytes = (std::streamsize)((size_t)n * nx * sizeof(float));
+            rvx.write(reinterpret_cast<const char *>(bx.data()), bytes);
+            rvz.write(reinterpret_cast<const char *>(bz.data()), bytes);
+            rp.write(reinterpret_cast<const char *>(bp.data()), bytes);
+            if (!rp) return fail("failed to write fluid record data");
+            if (!rvx || !rvz) return fail("failed to write solid record data");
+        }
+    }
+    std::printf("\n");
+    return 0;
+}

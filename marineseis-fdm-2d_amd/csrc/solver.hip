@@ -1,0 +1,631 @@
+// Device solver: HBM layout, PML/normal/source tables, RK4 stage launches,
+// hipGraph replay of the time loop and the msfdm_* device entry points.
+//
+// Replaces MemManager's device half (src/elements.cu:557-905, 979-987),
+// PmlFs::FluidSolid::update_RK4 (src/pml_fs_kernels.cu:765-868) and the
+// record sampling of src/main.cu:245-295.  One stream, explicit ordering (no
+// cross-stream races), records kept on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "model.hpp"
+#include "stencil.hpp"
+
+using namespace msfdm;
+namespace D = msfdm::dev;
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return set_error(MSFDM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+namespace {
+
+constexpr int GRAPH_STEPS = 32;   // steps captured per graph replay (even: F/B parity is baked in)
+
+enum Role { R_U = 0, R_A = 1, R_T1 = 2, R_T2 = 3 };
+
+inline long long align64(long long n) { return (n + 63) / 64 * 64; }
+
+}  // namespace
+
+struct msfdm_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    msfdm_desc d{};
+    std::vector<int> srcx, srcz;
+    D::Geo geo{};
+    D::Coef coef{};
+    D::Ctl ctl{};
+    long long buf_floats = 0;
+    float *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *kbuf = nullptr;       // raw-k output of msfdm_eval_rhs
+    float *kif = nullptr;        // [8][P] interface-row k
+    float *coefmem = nullptr;
+    float *amp = nullptr;
+    int *base = nullptr;
+    float *rec = nullptr;
+    int amp_cap = 0;             // steps covered by the amplitude table
+    int rec_cap = 0;             // record slots
+    int gap = 1;
+    int host_it = 0;
+    int isrc = 0;
+    int nrec_done = 0;
+    int tz_f = 16, tz_s = 16;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double bytes_step = 0.0;
+    std::vector<double> amp_host;   // Ricker amplitudes (double), 4 per step
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Tables
+// ---------------------------------------------------------------------------
+
+// Ricker at a float stage time (src/elements.cu:385-392): T frozen as float
+// 1.2/fpeak, zero once t - T > T; the wavelet is evaluated in double.
+double ricker(float fpeak, float time) {
+    const float T = (float)(1.2 / (double)fpeak);
+    const float t = time - T;
+    if (t > T) return 0.0;
+    double a = M_PI * (double)fpeak * (double)t;
+    a *= a;
+    return (1.0 - 2.0 * a) * std::exp(-a);
+}
+// Stage times: float it*dt, +dt/2, +dt/2, +dt (src/pml_fs_kernels.cu:768-844).
+double stage_amp(float fpeak, float dt, int it, int stage) {
+    const float time = (float)it * dt;
+    const float hf = (float)(dt * 0.5);
+    float t = time;
+    if (stage == 1 || stage == 2) t = time + hf;
+    if (stage == 3) t = time + dt;
+    return ricker(fpeak, t);
+}
+
+// PML profile (src/elements.cu:824-874) in double: returns al + d/be, d/be, 1/be.
+void pml_profile(int n, int npd, int npow, bool lo, bool hi, double damp0, double beta0, double alpha0,
+                 std::vector<float> &pa, std::vector<float> &pb, std::vector<float> &pc) {
+    pa.assign(n, 0.f); pb.assign(n, 0.f); pc.assign(n, 1.f);
+    for (int i = 0; i < n; ++i) {
+        int k = -1;
+        if (lo && i >= 3 && i <= 2 + npd) k = 3 + npd - i;
+        if (hi && i >= n - 3 - npd && i <= n - 4) k = i - (n - 4 - npd);
+        double dp = 0.0, be = 1.0, al = alpha0;
+        if (k > 0) {
+            const double r = double(k) / double(npd);
+            dp = damp0 * std::pow(r, (double)npow);
+            be = 1.0 + (beta0 - 1.0) * r * r;
+            al = alpha0 * (1.0 - r);
+        }
+        pa[i] = (float)(al + dp / be);
+        pb[i] = (float)(dp / be);
+        pc[i] = (float)(1.0 / be);
+    }
+}
+
+int grow_tables(msfdm_solver *s, int steps_needed) {
+    // amplitude table: 4 floats per step
+    if (steps_needed > s->amp_cap) {
+        int cap = std::max(steps_needed, s->amp_cap * 2);
+        cap = std::max(cap, 64);
+        std::vector<float> h(4 * (size_t)cap);
+        for (int it = 0; it < cap; ++it)
+            for (int st = 0; st < 4; ++st) h[4 * (size_t)it + st] = (float)stage_amp(s->d.fpeak, s->d.dt, it, st);
+        float *na = nullptr;
+        HIPCHK(hipMalloc(&na, h.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(na, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (s->amp) { HIPCHK(hipStreamSynchronize(s->stream)); HIPCHK(hipFree(s->amp)); }
+        s->amp = na;
+        s->amp_cap = cap;
+        s->ctl.amp = na;
+        if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+        if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    }
+    const int slots = steps_needed > 0 ? (steps_needed - 1) / s->gap + 1 : 0;
+    if (slots > s->rec_cap) {
+        const int cap = std::max({slots, s->rec_cap * 2, 16});
+        const size_t nx = (size_t)s->d.nx;
+        float *nr = nullptr;
+        HIPCHK(hipMalloc(&nr, 3 * (size_t)cap * nx * sizeof(float)));
+        HIPCHK(hipMemsetAsync(nr, 0, 3 * (size_t)cap * nx * sizeof(float), s->stream));
+        if (s->rec) {
+            for (int p = 0; p < 3; ++p)
+                HIPCHK(hipMemcpyAsync(nr + (size_t)p * cap * nx, s->rec + (size_t)p * s->rec_cap * nx,
+                                      (size_t)s->rec_cap * nx * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+            HIPCHK(hipFree(s->rec));
+        }
+        s->rec = nr;
+        s->rec_cap = cap;
+        s->ctl.rec = nr;
+        s->ctl.rec_cap = cap;
+        if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+        if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    }
+    return MSFDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launches
+// ---------------------------------------------------------------------------
+template <int DIR, int STAGE, int TZ>
+void launch_fluid(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
+                  float amp) {
+    const dim3 grid((s->d.nx - 4) / D::TX + 1, (s->d.nz_fluid - 3 + TZ - 1) / TZ);
+    D::fluid_stage<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, A, O, s->kif, j,
+                                                                      ct, cb, amp);
+}
+template <int DIR, int STAGE, int TZ>
+void launch_solid(msfdm_solver *s, const float *S, const float *U, float *A, float *O, float ct, float cb) {
+    const dim3 grid((s->d.nx - 4) / D::TX + 1, (s->d.nz_solid - 3 + TZ - 1) / TZ);
+    D::solid_stage<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, S, U, A, O, s->kif, ct, cb);
+}
+template <int DIR, int STAGE>
+void launch_stage(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
+                  float amp) {
+    if (s->tz_f == 8) launch_fluid<DIR, STAGE, 8>(s, S, U, A, O, j, ct, cb, amp);
+    else launch_fluid<DIR, STAGE, 16>(s, S, U, A, O, j, ct, cb, amp);
+    if (s->tz_s == 8) launch_solid<DIR, STAGE, 8>(s, S, U, A, O, ct, cb);
+    else launch_solid<DIR, STAGE, 16>(s, S, U, A, O, ct, cb);
+    if constexpr (STAGE == 4) {
+        D::iface_final<DIR><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, A, O,
+                                                                                    s->kif, cb, j);
+    } else {
+        D::iface_stage<DIR, STAGE><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, S, U, A, O, s->kif,
+                                                                                 ct, cb);
+    }
+}
+
+// One time step (update_RK4 FBFB / BFBF, src/pml_fs_kernels.cu:771-844) at
+// offset j of the current batch; it is the absolute step index.
+template <int D1>
+void enqueue_step_dir(msfdm_solver *s, int j) {
+    constexpr int D2 = 1 - D1;
+    const float dt = s->d.dt;
+    float *U = s->buf[R_U], *A = s->buf[R_A], *T1 = s->buf[R_T1], *T2 = s->buf[R_T2];
+    launch_stage<D1, 1>(s, U, U, A, T1, j, dt / 2, dt / 6, 0.f);
+    launch_stage<D2, 2>(s, T1, U, A, T2, j, dt / 2, dt / 3, 0.f);
+    launch_stage<D1, 3>(s, T2, U, A, T1, j, dt, dt / 3, 0.f);
+    launch_stage<D2, 4>(s, T1, U, A, U, j, 0.f, dt / 6, 0.f);
+}
+void enqueue_steps(msfdm_solver *s, int it0, int n) {
+    for (int j = 0; j < n; ++j) {
+        if ((it0 + j) & 1) enqueue_step_dir<1>(s, j);
+        else enqueue_step_dir<0>(s, j);
+    }
+    D::advance_base<<<1, 64, 0, s->stream>>>(s->base, n);
+}
+
+int build_graph(msfdm_solver *s) {
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_steps(s, 0, GRAPH_STEPS);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(s->stream, &g));
+    s->graph = g;
+    HIPCHK(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    return MSFDM_OK;
+}
+
+void drop_graph(msfdm_solver *s) {
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+}
+
+int field_rows(const msfdm_solver *s, int f) {
+    if (f <= MSFDM_FXEZ) return s->d.nz_fluid;
+    return s->d.nz_solid;
+}
+
+// Copy between the reference layout [rows][nx] and the device layout of field f.
+int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float *host_in) {
+    const int nx = s->d.nx, P = s->geo.P, npd = s->d.npd, W2 = s->geo.W2;
+    const int rows = field_rows(s, f);
+    float *dp = dbuf + s->geo.off[f];
+    const bool side = (f >= MSFDM_FXVX && f <= MSFDM_FXEZ) || (f >= MSFDM_SXVX && f <= MSFDM_SXEZ);
+    const bool bottom = f >= MSFDM_SZVX;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (!side && !bottom) {
+        if (host_out) HIPCHK(hipMemcpy2D(host_out, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), rows, hipMemcpyDeviceToHost));
+        else HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in, nx * sizeof(float), nx * sizeof(float), rows, hipMemcpyHostToDevice));
+        return MSFDM_OK;
+    }
+    if (side) {
+        std::vector<float> c((size_t)rows * W2);
+        if (host_out) {
+            HIPCHK(hipMemcpy(c.data(), dp, c.size() * sizeof(float), hipMemcpyDeviceToHost));
+            std::fill(host_out, host_out + (size_t)rows * nx, 0.0f);
+            for (int r = 0; r < rows; ++r)
+                for (int k = 0; k < W2; ++k) {
+                    const int ix = k < npd ? 3 + k : nx - 3 - npd + (k - npd);
+                    host_out[(size_t)r * nx + ix] = c[(size_t)r * W2 + k];
+                }
+        } else {
+            for (int r = 0; r < rows; ++r)
+                for (int k = 0; k < W2; ++k) {
+                    const int ix = k < npd ? 3 + k : nx - 3 - npd + (k - npd);
+                    c[(size_t)r * W2 + k] = host_in[(size_t)r * nx + ix];
+                }
+            HIPCHK(hipMemcpy(dp, c.data(), c.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+        return MSFDM_OK;
+    }
+    // bottom strip: solid rows nzs-3-npd .. nzs-4
+    const int zb0 = s->d.nz_solid - 3 - npd;
+    if (host_out) {
+        std::fill(host_out, host_out + (size_t)rows * nx, 0.0f);
+        HIPCHK(hipMemcpy2D(host_out + (size_t)zb0 * nx, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
+    } else {
+        HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in + (size_t)zb0 * nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
+    }
+    return MSFDM_OK;
+}
+
+void free_all(msfdm_solver *s) {
+    drop_graph(s);
+    for (auto &b : s->buf) if (b) { (void)hipFree(b); b = nullptr; }
+    for (void *p : {(void *)s->kbuf, (void *)s->kif, (void *)s->coefmem, (void *)s->amp, (void *)s->base, (void *)s->rec})
+        if (p) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solver **out) {
+    clear_error();
+    if (!m || !out) return set_error(MSFDM_EINVAL, "null argument");
+    *out = nullptr;
+    if (int rc = validate_desc(m->d)) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(MSFDM_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_error(MSFDM_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+
+    msfdm_solver *s = new msfdm_solver();
+    s->device = device;
+    s->d = m->d;
+    s->srcx = m->src_x;
+    s->srcz = m->src_z;
+    s->d.src_x = s->srcx.data();
+    s->d.src_z = s->srcz.data();
+    s->gap = record_gap(s->d.dt);
+    auto fail = [&](int rc) { free_all(s); delete s; return rc; };
+#define CK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return fail(set_error(MSFDM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); } while (0)
+    CK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+
+    const int nx = s->d.nx, nzf = s->d.nz_fluid, nzs = s->d.nz_solid, npd = s->d.npd;
+    const int nz = nzf + nzs - 1;
+    const long long P = align64(nx);
+    D::Geo &g = s->geo;
+    g.nx = nx; g.nzf = nzf; g.nzs = nzs; g.P = (int)P; g.npd = npd; g.W2 = 2 * npd;
+    long long off = 0;
+    for (int f = 0; f < MSFDM_NFIELD; ++f) {
+        g.off[f] = off;
+        long long n;
+        if (f <= MSFDM_FP) n = nzf * P;
+        else if (f <= MSFDM_FXEZ) n = (long long)nzf * g.W2;
+        else if (f <= MSFDM_SSXZ) n = nzs * P;
+        else if (f <= MSFDM_SXEZ) n = (long long)nzs * g.W2;
+        else n = (long long)npd * P;
+        off += align64(n);
+    }
+    s->buf_floats = off;
+    for (int r = 0; r < 4; ++r) {
+        CK(hipMalloc(&s->buf[r], off * sizeof(float)));
+        CK(hipMemset(s->buf[r], 0, off * sizeof(float)));
+    }
+    CK(hipMalloc(&s->kif, 8 * P * sizeof(float)));
+    CK(hipMemset(s->kif, 0, 8 * P * sizeof(float)));
+    CK(hipMalloc(&s->base, 64 * sizeof(int)));
+    CK(hipMemset(s->base, 0, 64 * sizeof(int)));
+
+    // Coefficients, padded to pitch P (host prep in double where derived).
+    const long long nF = align64(nzf * P), nS = align64(nzs * P), nG = align64(nz * P);
+    const long long nX = align64(P), nZ = align64(nzs);
+    const long long total = 2 * nF + 5 * nS + 5 * nG + 3 * nX + 3 * nZ + 2 * nX;
+    CK(hipMalloc(&s->coefmem, total * sizeof(float)));
+    CK(hipMemset(s->coefmem, 0, total * sizeof(float)));
+    std::vector<float> h;
+    float *cur = s->coefmem;
+    auto upload = [&](const std::vector<float> &v, long long n) -> const float * {
+        float *p = cur;
+        cur += n;
+        if (hipMemcpy(p, v.data(), std::min<long long>((long long)v.size(), n) * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return nullptr;
+        return p;
+    };
+    auto pad = [&](const std::vector<float> &src, int rows, bool invert) {
+        std::vector<float> o((size_t)rows * P, 0.0f);
+        for (int r = 0; r < rows; ++r)
+            for (int x = 0; x < nx; ++x) {
+                const float v = src[(size_t)r * nx + x];
+                o[(size_t)r * P + x] = invert ? (float)(1.0 / (double)v) : v;
+            }
+        return o;
+    };
+    D::Coef &c = s->coef;
+    const float *ptrs[32];
+    int np = 0;
+    ptrs[np++] = c.rhof_inv = upload(pad(m->a[MSFDM_A_RHO_F], nzf, true), nF);
+    ptrs[np++] = c.Kf = upload(pad(m->a[MSFDM_A_K], nzf, false), nF);
+    ptrs[np++] = c.rhos_inv = upload(pad(m->a[MSFDM_A_RHO_S], nzs, true), nS);
+    ptrs[np++] = c.C11 = upload(pad(m->a[MSFDM_A_C11], nzs, false), nS);
+    ptrs[np++] = c.C13 = upload(pad(m->a[MSFDM_A_C13], nzs, false), nS);
+    ptrs[np++] = c.C33 = upload(pad(m->a[MSFDM_A_C33], nzs, false), nS);
+    ptrs[np++] = c.C55 = upload(pad(m->a[MSFDM_A_C55], nzs, false), nS);
+    ptrs[np++] = c.J = upload(pad(m->a[MSFDM_A_J], nz, false), nG);
+    ptrs[np++] = c.xix = upload(pad(m->a[MSFDM_A_XI_X], nz, false), nG);
+    ptrs[np++] = c.xiz = upload(pad(m->a[MSFDM_A_XI_Z], nz, false), nG);
+    ptrs[np++] = c.zex = upload(pad(m->a[MSFDM_A_ZE_X], nz, false), nG);
+    ptrs[np++] = c.zez = upload(pad(m->a[MSFDM_A_ZE_Z], nz, false), nG);
+    // PML (src/elements.cu:117-124, 824-874): alpha0 = 2 pi fpeak, p_beta = 2, p_alpha = 1.
+    const double damp0 = -(s->d.npow + 1) * (double)s->d.vp_max * std::log((double)s->d.rc) / (2.0 * (double)s->d.thickness);
+    const double alpha0 = 2.0 * M_PI * (double)s->d.fpeak;
+    std::vector<float> pa, pb, pc;
+    pml_profile(nx, npd, s->d.npow, true, true, damp0, s->d.beta0, alpha0, pa, pb, pc);
+    ptrs[np++] = c.pxa = upload(pa, nX);
+    ptrs[np++] = c.pxb = upload(pb, nX);
+    ptrs[np++] = c.pxc = upload(pc, nX);
+    pml_profile(nzs, npd, s->d.npow, false, true, damp0, s->d.beta0, alpha0, pa, pb, pc);
+    ptrs[np++] = c.pza = upload(pa, nZ);
+    ptrs[np++] = c.pzb = upload(pb, nZ);
+    ptrs[np++] = c.pzc = upload(pc, nZ);
+    // Interface normal (src/elements.cu:892-905) in double.
+    std::vector<float> nif(2 * (size_t)nx);
+    for (int ix = 0; ix < nx; ++ix) {
+        const size_t gi = (size_t)(nzf - 1) * nx + ix;
+        const double zx = m->a[MSFDM_A_ZE_X][gi], zz = m->a[MSFDM_A_ZE_Z][gi];
+        const double n = std::sqrt(zx * zx + zz * zz);
+        nif[2 * ix] = (float)(zx / n);
+        nif[2 * ix + 1] = (float)(zz / n);
+    }
+    const float *nifp = upload(nif, 2 * nX);
+    c.nif = reinterpret_cast<const float2 *>(nifp);
+    ptrs[np++] = nifp;
+    for (int i = 0; i < np; ++i)
+        if (!ptrs[i]) return fail(set_error(MSFDM_EDEVICE, "coefficient upload failed"));
+
+    s->ctl.base = s->base;
+    s->ctl.gap = s->gap;
+    s->ctl.srcx = s->srcx.empty() ? -100 : s->srcx[0];
+    s->ctl.srcz = s->srcz.empty() ? -100 : s->srcz[0];
+    if (int rc = grow_tables(s, std::max(s->d.nt, 1))) return fail(rc);
+
+    // Tile heights: small grids need more blocks to fill 256 CUs.
+    const long long cells = (long long)nx * nz;
+    s->tz_f = s->tz_s = cells < 1000000 ? 8 : 16;
+
+    // Algorithmic bytes per step: 16 state + 4 coefficient floats per field
+    // (fluid 3 fields / 6 coefficients, solid 5 / 9) per updated cell, plus
+    // 16 floats x 4 memory fields per active PML direction.
+    const double cf = (double)(nzf - 3) * (nx - 6), cs = (double)(nzs - 3) * (nx - 6);
+    const double pmlf = (double)(nzf - 3) * 2 * npd, pmls = (double)(nzs - 3) * 2 * npd + (double)npd * (nx - 6);
+    s->bytes_step = 4.0 * (cf * (16 * 3 + 4 * 6) + cs * (16 * 5 + 4 * 9) + (pmlf + pmls) * 16 * 4);
+    CK(hipStreamSynchronize(s->stream));
+#undef CK
+    *out = s;
+    return MSFDM_OK;
+}
+
+extern "C" void msfdm_solver_destroy(msfdm_solver *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+}
+
+extern "C" int msfdm_reset_shot(msfdm_solver *s, int isrc) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (isrc < 0 || (isrc >= (int)s->srcx.size() && !(s->srcx.empty() && isrc == 0)))
+        return set_error(MSFDM_EINVAL, "shot index out of range");
+    HIPCHK(hipSetDevice(s->device));
+    for (int r = 0; r < 4; ++r) HIPCHK(hipMemsetAsync(s->buf[r], 0, s->buf_floats * sizeof(float), s->stream));
+    HIPCHK(hipMemsetAsync(s->base, 0, sizeof(int), s->stream));
+    HIPCHK(hipMemsetAsync(s->rec, 0, 3 * (size_t)s->rec_cap * s->d.nx * sizeof(float), s->stream));
+    const int nsx = s->srcx.empty() ? -100 : s->srcx[isrc], nsz = s->srcz.empty() ? -100 : s->srcz[isrc];
+    if (nsx != s->ctl.srcx || nsz != s->ctl.srcz) drop_graph(s);
+    s->ctl.srcx = nsx;
+    s->ctl.srcz = nsz;
+    s->isrc = isrc;
+    s->host_it = 0;
+    s->nrec_done = 0;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_run(msfdm_solver *s, int nsteps) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (nsteps < 0) return set_error(MSFDM_EINVAL, "negative step count");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = grow_tables(s, s->host_it + nsteps)) return rc;
+    int left = nsteps;
+    while (left > 0) {
+        if (s->use_graph && left >= GRAPH_STEPS && (s->host_it & 1) == 0) {
+            if (!s->gexec)
+                if (int rc = build_graph(s)) return rc;
+            HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+            s->host_it += GRAPH_STEPS;
+            left -= GRAPH_STEPS;
+        } else {
+            const int n = (s->use_graph && (s->host_it & 1)) ? 1 : std::min(left, GRAPH_STEPS);
+            enqueue_steps(s, s->host_it, n);
+            HIPCHK(hipGetLastError());
+            s->host_it += n;
+            left -= n;
+        }
+    }
+    s->nrec_done = s->host_it > 0 ? (s->host_it - 1) / s->gap + 1 : 0;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_synchronize(msfdm_solver *s) {
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_step_index(const msfdm_solver *s, int *it) {
+    if (!s || !it) return set_error(MSFDM_EINVAL, "null argument");
+    *it = s->host_it;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_num_records(const msfdm_solver *s, int *n) {
+    if (!s || !n) return set_error(MSFDM_EINVAL, "null argument");
+    *n = s->nrec_done;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_get_records(msfdm_solver *s, int first, int count, float *vx, float *vz, float *p) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (first < 0 || count < 0 || first + count > s->nrec_done) return set_error(MSFDM_EINVAL, "record range out of bounds");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const size_t nx = (size_t)s->d.nx, bytes = (size_t)count * nx * sizeof(float);
+    float *outs[3] = {vx, vz, p};
+    for (int k = 0; k < 3; ++k)
+        if (outs[k] && count > 0)
+            HIPCHK(hipMemcpy(outs[k], s->rec + ((size_t)k * s->rec_cap + first) * nx, bytes, hipMemcpyDeviceToHost));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_get_field(msfdm_solver *s, int field, float *out) {
+    clear_error();
+    if (!s || !out) return set_error(MSFDM_EINVAL, "null argument");
+    if (field < 0 || field >= MSFDM_NFIELD) return set_error(MSFDM_EINVAL, "bad field id");
+    HIPCHK(hipSetDevice(s->device));
+    return copy_field(s, s->buf[R_U], field, out, nullptr);
+}
+
+extern "C" int msfdm_set_field(msfdm_solver *s, int field, const float *in) {
+    clear_error();
+    if (!s || !in) return set_error(MSFDM_EINVAL, "null argument");
+    if (field < 0 || field >= MSFDM_NFIELD) return set_error(MSFDM_EINVAL, "bad field id");
+    HIPCHK(hipSetDevice(s->device));
+    // The stage buffers keep the same values in cells the stepper never
+    // updates (frozen bands), as the reference's RK4_reset copies do.
+    for (int r : {R_U, R_T1, R_T2})
+        if (int rc = copy_field(s, s->buf[r], field, nullptr, in)) return rc;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_eval_rhs(msfdm_solver *s, int dir, float amp) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (dir != 0 && dir != 1) return set_error(MSFDM_EINVAL, "dir must be 0 (F) or 1 (B)");
+    HIPCHK(hipSetDevice(s->device));
+    if (!s->kbuf) {
+        HIPCHK(hipMalloc(&s->kbuf, s->buf_floats * sizeof(float)));
+    }
+    HIPCHK(hipMemsetAsync(s->kbuf, 0, s->buf_floats * sizeof(float), s->stream));
+    float *U = s->buf[R_U];
+    if (dir == 0) launch_stage<0, 0>(s, U, U, s->buf[R_A], s->kbuf, 0, 0.f, 0.f, amp);
+    else launch_stage<1, 0>(s, U, U, s->buf[R_A], s->kbuf, 0, 0.f, 0.f, amp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_get_rhs(msfdm_solver *s, int field, float *out) {
+    clear_error();
+    if (!s || !out) return set_error(MSFDM_EINVAL, "null argument");
+    if (!s->kbuf) return set_error(MSFDM_ESTATE, "msfdm_eval_rhs has not been called");
+    if (field < 0 || field >= MSFDM_NFIELD) return set_error(MSFDM_EINVAL, "bad field id");
+    HIPCHK(hipSetDevice(s->device));
+    return copy_field(s, s->kbuf, field, out, nullptr);
+}
+
+extern "C" int msfdm_set_graph(msfdm_solver *s, int enable) {
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    s->use_graph = enable != 0;
+    if (!s->use_graph) drop_graph(s);
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms) {
+    clear_error();
+    if (!s || !ms || reps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = grow_tables(s, s->host_it + 1)) return rc;   // the stage reads amplitude row host_it
+    float *U = s->buf[R_U], *A = s->buf[R_A], *T1 = s->buf[R_T1], *T2 = s->buf[R_T2];
+    const float dt = s->d.dt;
+    auto one = [&]() {
+        switch (which) {
+            case 0:
+                if (s->tz_f == 8) launch_fluid<0, 2, 8>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
+                else launch_fluid<0, 2, 16>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
+                break;
+            case 1:
+                if (s->tz_s == 8) launch_solid<0, 2, 8>(s, T1, U, A, T2, dt / 2, dt / 3);
+                else launch_solid<0, 2, 16>(s, T1, U, A, T2, dt / 2, dt / 3);
+                break;
+            case 2:
+                D::iface_stage<0, 2><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, T1, U, A, T2, s->kif, dt / 2, dt / 3);
+                break;
+            default: {
+                D::Ctl quiet = s->ctl;   // no record slot is written while timing
+                quiet.gap = 0x7fffffff;
+                D::iface_final<0><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, A, T2, s->kif, dt / 6, 1);
+                break;
+            }
+        }
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    one();   // warm-up
+    HIPCHK(hipEventRecord(e0, s->stream));
+    for (int i = 0; i < reps; ++i) one();
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms) {
+    clear_error();
+    if (!s || !ms) return set_error(MSFDM_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = grow_tables(s, s->host_it + nsteps)) return rc;
+    if (s->use_graph && !s->gexec && (s->host_it & 1) == 0 && nsteps >= GRAPH_STEPS)
+        if (int rc = build_graph(s)) return rc;   // instantiate outside the timed region
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s->stream));
+    if (int rc = msfdm_run(s, nsteps)) return rc;
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes) {
+    if (!s || !bytes) return set_error(MSFDM_EINVAL, "null argument");
+    *bytes = s->bytes_step;
+    return MSFDM_OK;
+}

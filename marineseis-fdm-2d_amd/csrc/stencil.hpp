@@ -1,0 +1,604 @@
+// Device code of the fluid-solid RK4 stepper for gfx950.
+//
+// One RK stage = three launches:
+//   fluid_stage<DIR,STAGE,TZ>  fluid RHS (update_velocity_fluid + update_stress_fluid,
+//                              include/pml_fs_kernels.cuh:333-507) + xi-PML memory
+//                              + source (src/pml_fs_kernels.cu:527-531, 618-621)
+//                              + the RK4 combination (src/pml_fs_kernels.cu:663-763)
+//   solid_stage<DIR,STAGE,TZ>  solid RHS (update_stress_solid + update_velocity_solid,
+//                              include/pml_fs_kernels.cuh:509-752) + xi/zeta-PML memory
+//                              + the RK4 combination
+//   iface_stage<DIR,STAGE>     fl_so_boundary (include/pml_fs_kernels.cuh:754-827)
+//                              on the interface rows + their RK4 combination;
+//                              in stage 4 also fl_so_filter (src/pml_fs_kernels.cu:428-525,
+//                              out of place) and the receiver-row record.
+//
+// The stage kernels read the stage state once through an LDS tile with a 3-cell
+// halo on every side, build the interface / free-surface ghost fluxes
+// (src/pml_fs_kernels.cu:385-426) from that tile, and never materialise k: the
+// classical-RK4 accumulator and the next stage state are written directly.
+// Interface-row k values (fluid row nzf-1, solid row 0) go to a small [8][P]
+// buffer because the boundary condition couples both media.
+//
+// STAGE 1..4 are the RK stages; STAGE 0 writes the raw k (used to check the
+// right-hand side against the oracle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "msfdm.h"
+
+namespace msfdm {
+namespace dev {
+
+// DRP/opt MacCormack weights, offsets -1..3 (include/tools_func.cuh:6-10).
+constexpr float DRP_M1 = -0.30874f;
+constexpr float DRP_0 = -0.63260f;
+constexpr float DRP_1 = +1.23300f;
+constexpr float DRP_2 = -0.33340f;
+constexpr float DRP_3 = +0.04168f;
+// 4-2 and 2-2 closures (include/tools_func.cuh:12-17).
+constexpr float M42_0 = -7.0f / 6.0f;
+constexpr float M42_1 = +8.0f / 6.0f;
+constexpr float M42_2 = -1.0f / 6.0f;
+
+constexpr int TX = 64;          // tile width = one wavefront
+constexpr int HALO = 3;
+constexpr int LXW = TX + 2 * HALO;  // 70 loaded columns
+constexpr int LXP = 72;         // LDS row pitch (floats)
+constexpr int NTHR = 256;       // 64 x 4 threads per stage block
+
+struct Geo {
+    int nx, nzf, nzs, P, npd, W2;
+    long long off[MSFDM_NFIELD];  // float offset of each field inside a state buffer
+};
+
+struct Coef {
+    const float *rhof_inv, *Kf;                          // [nzf][P]
+    const float *rhos_inv, *C11, *C13, *C33, *C55;       // [nzs][P]
+    const float *J, *xix, *xiz, *zex, *zez;              // [nz][P] global rows
+    const float *pxa, *pxb, *pxc;                        // xi PML: al+d/be, d/be, 1/be  [nx]
+    const float *pza, *pzb, *pzc;                        // solid zeta PML               [nzs]
+    const float2 *nif;                                   // interface normal            [nx]
+};
+
+struct Ctl {
+    const float *amp;     // Ricker amplitude per (step, stage): [4 * cap]
+    const int *base;      // first step index of the current launch batch
+    float *rec;           // records [3][rec_cap][nx]: solid vx, solid vz, fluid p
+    int rec_cap;
+    int gap;
+    int srcx, srcz;       // fluid-array source position of the current shot
+};
+
+// 5-point one-sided derivative from values at ascending offsets
+// (F: -1..3, B: -3..1), include/tools_func.cuh:51-71.
+template <int DIR>
+__device__ __forceinline__ float drp5(float v0, float v1, float v2, float v3, float v4) {
+    if constexpr (DIR == 0) return DRP_M1 * v0 + DRP_0 * v1 + DRP_1 * v2 + DRP_2 * v3 + DRP_3 * v4;
+    else return -(DRP_M1 * v4 + DRP_0 * v3 + DRP_1 * v2 + DRP_2 * v1 + DRP_3 * v0);
+}
+
+// Offsets read by the DRP stencil of direction DIR.
+template <int DIR> __device__ __forceinline__ constexpr int lo_off() { return DIR == 0 ? -1 : -3; }
+
+// x-derivative of an LDS row at column c (tile coordinates).
+template <int DIR>
+__device__ __forceinline__ float dx_lds(const float *row, int c) {
+    constexpr int o = lo_off<DIR>();
+    return drp5<DIR>(row[c + o], row[c + o + 1], row[c + o + 2], row[c + o + 3], row[c + o + 4]);
+}
+// z-derivative at tile row r, column c.
+template <int DIR>
+__device__ __forceinline__ float dz_lds(const float (*t)[LXP], int r, int c) {
+    constexpr int o = lo_off<DIR>();
+    return drp5<DIR>(t[r + o][c], t[r + o + 1][c], t[r + o + 2][c], t[r + o + 3][c], t[r + o + 4][c]);
+}
+// Distance-dependent zeta derivative Mac_dv_dze (include/tools_func.cuh:131-140).
+template <int DIR>
+__device__ __forceinline__ float mac_dz(const float (*t)[LXP], int r, int c, int dist, bool solid_side) {
+    if (dist >= 3) return dz_lds<DIR>(t, r, c);
+    if (dist == 2) {
+        if constexpr (DIR == 0) return M42_0 * t[r][c] + M42_1 * t[r + 1][c] + M42_2 * t[r + 2][c];
+        else return -(M42_0 * t[r][c] + M42_1 * t[r - 1][c] + M42_2 * t[r - 2][c]);
+    }
+    if (dist == 1) {
+        if constexpr (DIR == 0) return -t[r][c] + t[r + 1][c];
+        else return -(-t[r][c] + t[r - 1][c]);
+    }
+    return solid_side ? (-t[r][c] + t[r + 1][c]) : -(-t[r][c] + t[r - 1][c]);
+}
+
+// RK4 combination of one value (src/pml_fs_kernels.cu:663-763 with
+// MemManager::RK4_reset folded in): stage 1 reads U from the stage state.
+template <int STAGE>
+__device__ __forceinline__ void rk(const float *__restrict__ U, float *__restrict__ A, float *__restrict__ O,
+                                   long long i, float s, float k, float ct, float cb) {
+    if constexpr (STAGE == 0) {
+        O[i] = k;
+    } else if constexpr (STAGE == 1) {
+        A[i] = s + cb * k;
+        O[i] = s + ct * k;
+    } else if constexpr (STAGE == 4) {
+        O[i] = A[i] + cb * k;
+    } else {
+        A[i] = A[i] + cb * k;
+        O[i] = U[i] + ct * k;
+    }
+}
+
+// Compact PML-strip column of ix (left strip first), or -1.
+__device__ __forceinline__ int side_col(int ix, int nx, int npd) {
+    if (ix - 3 < npd) return ix - 3;
+    if (nx - 4 - ix < npd) return npd + (ix - (nx - 3 - npd));
+    return -1;
+}
+
+__constant__ float SRC_W[5][5] = {   // src/elements.cu:34-40
+    {0.0030f, 0.0133f, 0.0219f, 0.0133f, 0.0030f},
+    {0.0133f, 0.0596f, 0.0983f, 0.0596f, 0.0133f},
+    {0.0219f, 0.0983f, 0.1621f, 0.0983f, 0.0219f},
+    {0.0133f, 0.0596f, 0.0983f, 0.0596f, 0.0133f},
+    {0.0030f, 0.0133f, 0.0219f, 0.0133f, 0.0030f}};
+__constant__ float FILT_W[7] = {   // GAUSS_W7_PML, src/elements.cu:30-33
+    1.1e-13f, 1.678e-06f, 0.03419f, 0.93163f, 0.03419f, 1.678e-06f, 1.1e-13f};
+
+// Load NF fields of the stage state (rows z0-3 .. z0+TZ+2, columns x0-3 .. x0+66)
+// into LDS; cells outside [0,nrows) x [0,P) read as 0.
+template <int NF, int TZ>
+__device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const float *__restrict__ S,
+                                          const long long *off, int f0, int z0, int x0, int nrows, int P) {
+    constexpr int N = (TZ + 2 * HALO) * LXW;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const float *src = S + off[f0 + f];
+        for (int e = threadIdx.x; e < N; e += NTHR) {
+            const int r = e / LXW, c = e - r * LXW;
+            const int gz = z0 - HALO + r, gx = x0 - HALO + c;
+            float v = 0.0f;
+            if (gz >= 0 && gz < nrows && gx >= 0 && gx < P) v = src[(long long)gz * P + gx];
+            sh[f][r][c] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fluid stage: rows 3..nzf-1, columns 3..nx-4.
+// ---------------------------------------------------------------------------
+template <int DIR, int STAGE, int TZ>
+__global__ void __launch_bounds__(NTHR)
+fluid_stage(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
+            float *__restrict__ A, float *__restrict__ O, float *__restrict__ kif, int jstep, float ct,
+            float cb, float amp_direct) {
+    constexpr int LZ = TZ + 2 * HALO;
+    __shared__ float sh[3][LZ][LXP];   // vx, vz, p
+    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    const int x0 = blockIdx.x * TX, z0 = 3 + blockIdx.y * TZ;
+    const int P = g.P, nx = g.nx, nzf = g.nzf;
+    load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
+    __syncthreads();
+    const int ix = x0 + tx;
+    if (ix < 3 || ix > nx - 4) return;
+    const float amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
+    const int scol = side_col(ix, nx, g.npd);
+    const int cc = tx + HALO;
+    const float (*shp)[LXP] = sh[2];
+    const float (*shvx)[LXP] = sh[0];
+    const float (*shvz)[LXP] = sh[1];
+    float pxa = 0.f, pxb = 0.f, pxc = 0.f;
+    if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
+
+    for (int j = 0; j < TZ / 4; ++j) {
+        const int lz = ty + 4 * j;
+        const int iz = z0 + lz;
+        if (iz > nzf - 1) break;
+        const int r = lz + HALO;
+        const long long gi = (long long)iz * P + ix;
+        const float xx = c.xix[gi], xz = c.xiz[gi], zx = c.zex[gi], zz = c.zez[gi];
+        const float gxixi = xx * xx + xz * xz;
+        const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
+
+        // ---- velocity: rho dv/dt = -grad p (include/pml_fs_kernels.cuh:333-450)
+        float rvx, rvz, Dx, Dz;
+        if (dist_flso < 3 || dist_fs < 3) {
+            const bool at_if = dist_flso < 3;
+            // zeta flux -J p (ze_x, ze_z) at real fluid row rr of this column
+            auto fze_row = [&](int rr, float &fx, float &fz) {
+                const long long q = (long long)rr * P + ix;
+                const float mJp = -c.J[q] * shp[r + (rr - iz)][cc];
+                fx = mJp * c.zex[q];
+                fz = mJp * c.zez[q];
+            };
+            float ax[5], az[5], bx[5], bz[5];
+            constexpr int o = lo_off<DIR>();
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int m = o + q;
+                const long long gq = gi + m;
+                const float mJp = -c.J[gq] * shp[r][cc + m];
+                ax[q] = mJp * c.xix[gq];
+                az[q] = mJp * c.xiz[gq];
+                const int rr = iz + m;
+                if (at_if) {
+                    if (rr <= nzf - 1) fze_row(rr, bx[q], bz[q]);
+                    else {   // Gf ghost n = rr - (nzf-1): 2 F(interface) - F(interface - n)
+                        float fx0, fz0, fxn, fzn;
+                        fze_row(nzf - 1, fx0, fz0);
+                        fze_row(2 * (nzf - 1) - rr, fxn, fzn);
+                        bx[q] = 2.0f * fx0 - fxn;
+                        bz[q] = 2.0f * fz0 - fzn;
+                    }
+                } else {
+                    if (rr >= 3) fze_row(rr, bx[q], bz[q]);
+                    else {   // Gfs ghost n = 3 - rr: -F(3 + n)
+                        float fxn, fzn;
+                        fze_row(6 - rr, fxn, fzn);
+                        bx[q] = -fxn;
+                        bz[q] = -fzn;
+                    }
+                }
+            }
+            const float sx = drp5<DIR>(ax[0], ax[1], ax[2], ax[3], ax[4]) + drp5<DIR>(bx[0], bx[1], bx[2], bx[3], bx[4]);
+            const float sz = drp5<DIR>(az[0], az[1], az[2], az[3], az[4]) + drp5<DIR>(bz[0], bz[1], bz[2], bz[3], bz[4]);
+            const float Jc = c.J[gi];
+            rvx = sx / Jc;
+            rvz = sz / Jc;
+            const float proj = (xx * sx + xz * sz) / (gxixi * Jc);
+            Dx = xx * proj;
+            Dz = xz * proj;
+        } else {
+            const float dpx = dx_lds<DIR>(shp[r], cc);
+            const float dpz = dz_lds<DIR>(shp, r, cc);
+            rvx = -(xx * dpx + zx * dpz);
+            rvz = -(xz * dpx + zz * dpz);
+            const float gxize = xx * zx + xz * zz;
+            const float t = (gxixi * dpx + gxize * dpz) / gxixi;
+            Dx = -xx * t;
+            Dz = -xz * t;
+        }
+        const float rinv = c.rhof_inv[gi];
+        float kvx = rvx * rinv, kvz = rvz * rinv;
+
+        // ---- pressure: dp/dt = -K div v (include/pml_fs_kernels.cuh:452-507)
+        float kp = 0.0f, kEx = 0.0f, kEz = 0.0f;
+        const float Km = c.Kf[gi];
+        float dvxx = 0.f, dvzx = 0.f, dvxz = 0.f, dvzz = 0.f;
+        if (dist_fs != 0) {
+            dvxx = dx_lds<DIR>(shvx[r], cc);
+            dvzx = dx_lds<DIR>(shvz[r], cc);
+            const int dzn = min(dist_fs, dist_flso);
+            dvxz = mac_dz<DIR>(shvx, r, cc, dzn, false);
+            dvzz = mac_dz<DIR>(shvz, r, cc, dzn, false);
+            kp = -Km * (xx * dvxx + zx * dvxz + xz * dvzx + zz * dvzz);
+        }
+        // ---- xi-PML memory (side strips)
+        if (scol >= 0) {
+            const long long ai = (long long)iz * g.W2 + scol;
+            const float Vx = S[g.off[MSFDM_FXVX] + ai], Vz = S[g.off[MSFDM_FXVZ] + ai];
+            const float kVx = -pxa * Vx + pxb * Dx;
+            const float kVz = -pxa * Vz + pxb * Dz;
+            kvx += ((pxc - 1.0f) * Dx - pxc * Vx) * rinv;
+            kvz += ((pxc - 1.0f) * Dz - pxc * Vz) * rinv;
+            const float Ex = S[g.off[MSFDM_FXEX] + ai], Ez = S[g.off[MSFDM_FXEZ] + ai];
+            if (dist_fs != 0) {
+                const float gxize = xx * zx + xz * zz;
+                const float Dvx = dvxx + (gxize / gxixi) * dvxz;
+                const float Dvz = dvzx + (gxize / gxixi) * dvzz;
+                kEx = -pxa * Ex + pxb * Dvx;
+                kEz = -pxa * Ez + pxb * Dvz;
+                kp += Km * (-(pxc - 1.0f) * xx * Dvx - (pxc - 1.0f) * xz * Dvz + pxc * xx * Ex + pxc * xz * Ez);
+            }
+            rk<STAGE>(U, A, O, g.off[MSFDM_FXVX] + ai, Vx, kVx, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_FXVZ] + ai, Vz, kVz, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_FXEX] + ai, Ex, kEx, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_FXEZ] + ai, Ez, kEz, ct, cb);
+        }
+        // ---- source: 5x5 Gaussian-weighted Ricker on p_k (include/tools_func.cuh:21-37)
+        {
+            const int sxo = ix - ctl.srcx, szo = iz - ctl.srcz;
+            if (sxo >= -2 && sxo <= 2 && szo >= -2 && szo <= 2 && iz <= nzf - 4)
+                kp += amp * SRC_W[szo + 2][sxo + 2];
+        }
+        if (dist_flso == 0) {
+            kif[0 * (long long)P + ix] = kvx;
+            kif[1 * (long long)P + ix] = kvz;
+            kif[2 * (long long)P + ix] = kp;
+            if constexpr (STAGE != 0) continue;
+        }
+        rk<STAGE>(U, A, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct, cb);
+        rk<STAGE>(U, A, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct, cb);
+        rk<STAGE>(U, A, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct, cb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Solid stage: rows 0..nzs-4, columns 3..nx-4.  Metrics at global row iz+nzf-1.
+// ---------------------------------------------------------------------------
+template <int DIR, int STAGE, int TZ>
+__global__ void __launch_bounds__(NTHR)
+solid_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U,
+            float *__restrict__ A, float *__restrict__ O, float *__restrict__ kif, float ct, float cb) {
+    constexpr int LZ = TZ + 2 * HALO;
+    __shared__ float sh[5][LZ][LXP];   // vx, vz, sx, sz, sxz
+    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
+    load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
+    __syncthreads();
+    const int ix = x0 + tx;
+    if (ix < 3 || ix > nx - 4) return;
+    const int scol = side_col(ix, nx, npd);
+    const int cc = tx + HALO;
+    const float (*svx)[LXP] = sh[0];
+    const float (*svz)[LXP] = sh[1];
+    const float (*ssx)[LXP] = sh[2];
+    const float (*ssz)[LXP] = sh[3];
+    const float (*ssxz)[LXP] = sh[4];
+    const long long moff = (long long)(g.nzf - 1) * P;   // solid row -> global metric row
+    float pxa = 0.f, pxb = 0.f, pxc = 0.f;
+    if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
+    const int zb0 = nzs - 3 - npd;   // first bottom-PML row
+
+    for (int j = 0; j < TZ / 4; ++j) {
+        const int lz = ty + 4 * j;
+        const int iz = z0 + lz;
+        if (iz > nzs - 4) break;
+        const int r = lz + HALO;
+        const long long si = (long long)iz * P + ix;
+        const long long gi = moff + si;
+        const float xx = c.xix[gi], xz = c.xiz[gi], zx = c.zex[gi], zz = c.zez[gi];
+        const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
+        const bool side = scol >= 0, bottom = iz >= zb0;
+
+        // ---- stress (include/pml_fs_kernels.cuh:509-628)
+        const float dvxx = dx_lds<DIR>(svx[r], cc);
+        const float dvxz = mac_dz<DIR>(svx, r, cc, iz, true);
+        const float dvzx = dx_lds<DIR>(svz[r], cc);
+        const float dvzz = mac_dz<DIR>(svz, r, cc, iz, true);
+        const float c11 = c.C11[si], c13 = c.C13[si], c33 = c.C33[si], c55 = c.C55[si];
+        const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
+        float ksx = c11 * exx + c13 * ezz;
+        float ksz = c13 * exx + c33 * ezz;
+        float ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+
+        // ---- velocity (include/pml_fs_kernels.cuh:631-752)
+        float rvx, rvz, Dxx, Dxz, Dzx, Dzz;
+        if (iz < 3) {
+            // zeta traction flux at real solid row rr of this column
+            auto fze_row = [&](int rr, float &fx, float &fz) {
+                const long long q = moff + (long long)rr * P + ix;
+                const int tr = r + (rr - iz);
+                const float Jq = c.J[q], zxq = c.zex[q], zzq = c.zez[q];
+                fx = Jq * (zxq * ssx[tr][cc] + zzq * ssxz[tr][cc]);
+                fz = Jq * (zxq * ssxz[tr][cc] + zzq * ssz[tr][cc]);
+            };
+            float ax[5], az[5], bx[5], bz[5];
+            constexpr int o = lo_off<DIR>();
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int m = o + q;
+                const long long gq = gi + m;
+                const float Jq = c.J[gq], xxq = c.xix[gq], xzq = c.xiz[gq];
+                ax[q] = Jq * (xxq * ssx[r][cc + m] + xzq * ssxz[r][cc + m]);
+                az[q] = Jq * (xxq * ssxz[r][cc + m] + xzq * ssz[r][cc + m]);
+                const int rr = iz + m;
+                if (rr >= 0) fze_row(rr, bx[q], bz[q]);
+                else {   // Gs ghost n = -rr: 2 F(0) - F(n)
+                    float fx0, fz0, fxn, fzn;
+                    fze_row(0, fx0, fz0);
+                    fze_row(-rr, fxn, fzn);
+                    bx[q] = 2.0f * fx0 - fxn;
+                    bz[q] = 2.0f * fz0 - fzn;
+                }
+            }
+            const float sx = drp5<DIR>(ax[0], ax[1], ax[2], ax[3], ax[4]) + drp5<DIR>(bx[0], bx[1], bx[2], bx[3], bx[4]);
+            const float sz = drp5<DIR>(az[0], az[1], az[2], az[3], az[4]) + drp5<DIR>(bz[0], bz[1], bz[2], bz[3], bz[4]);
+            const float Jc = c.J[gi];
+            rvx = sx / Jc;
+            rvz = sz / Jc;
+            const float pxi = (xx * sx + xz * sz) / (gxixi * Jc);
+            const float pze = (zx * sx + zz * sz) / (gzeze * Jc);
+            Dxx = xx * pxi; Dxz = xz * pxi;
+            Dzx = zx * pze; Dzz = zz * pze;
+        } else {
+            const float a = dx_lds<DIR>(ssx[r], cc), b = dz_lds<DIR>(ssx, r, cc);
+            const float cz = dx_lds<DIR>(ssz[r], cc), d = dz_lds<DIR>(ssz, r, cc);
+            const float e = dx_lds<DIR>(ssxz[r], cc), f = dz_lds<DIR>(ssxz, r, cc);
+            rvx = xx * a + zx * b + xz * e + zz * f;
+            rvz = xx * e + zx * f + xz * cz + zz * d;
+            const float gx = gxize / gxixi, gz = gxize / gzeze;
+            Dxx = xx * (a + gx * b) + xz * (e + gx * f);
+            Dxz = xx * (e + gx * f) + xz * (cz + gx * d);
+            Dzx = zx * (b + gz * a) + zz * (gz * e + f);
+            Dzz = zx * (gz * e + f) + zz * (gz * cz + d);
+        }
+        const float rinv = c.rhos_inv[si];
+        float kvx = rvx * rinv, kvz = rvz * rinv;
+
+        // ---- xi-PML memory (side strips)
+        if (side) {
+            const long long ai = (long long)iz * g.W2 + scol;
+            const float Vx = S[g.off[MSFDM_SXVX] + ai], Vz = S[g.off[MSFDM_SXVZ] + ai];
+            const float Ex = S[g.off[MSFDM_SXEX] + ai], Ez = S[g.off[MSFDM_SXEZ] + ai];
+            const float Dvx = dvxx + (gxize / gxixi) * dvxz;
+            const float Dvz = dvzx + (gxize / gxixi) * dvzz;
+            const float kEx = -pxa * Ex + pxb * Dvx;
+            const float kEz = -pxa * Ez + pxb * Dvz;
+            const float q1 = pxc - 1.0f;
+            ksx += c11 * q1 * xx * Dvx + c13 * q1 * xz * Dvz - pxc * c11 * xx * Ex - pxc * c13 * xz * Ez;
+            ksz += c13 * q1 * xx * Dvx + c33 * q1 * xz * Dvz - pxc * c13 * xx * Ex - pxc * c33 * xz * Ez;
+            ksxz += c55 * q1 * (xz * Dvx + xx * Dvz) - pxc * c55 * xz * Ex - pxc * c55 * xx * Ez;
+            const float kVx = -pxa * Vx + pxb * Dxx;
+            const float kVz = -pxa * Vz + pxb * Dxz;
+            kvx += (q1 * Dxx - pxc * Vx) * rinv;
+            kvz += (q1 * Dxz - pxc * Vz) * rinv;
+            rk<STAGE>(U, A, O, g.off[MSFDM_SXVX] + ai, Vx, kVx, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_SXVZ] + ai, Vz, kVz, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_SXEX] + ai, Ex, kEx, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_SXEZ] + ai, Ez, kEz, ct, cb);
+        }
+        // ---- zeta-PML memory (bottom strip)
+        if (bottom) {
+            const long long ai = (long long)(iz - zb0) * P + ix;
+            const float pza = c.pza[iz], pzb = c.pzb[iz], pzc = c.pzc[iz];
+            const float Vx = S[g.off[MSFDM_SZVX] + ai], Vz = S[g.off[MSFDM_SZVZ] + ai];
+            const float Ex = S[g.off[MSFDM_SZEX] + ai], Ez = S[g.off[MSFDM_SZEZ] + ai];
+            const float Dvx = dvxz + (gxize / gzeze) * dvxx;
+            const float Dvz = dvzz + (gxize / gzeze) * dvzx;
+            const float kEx = -pza * Ex + pzb * Dvx;
+            const float kEz = -pza * Ez + pzb * Dvz;
+            const float q1 = pzc - 1.0f;
+            ksx += c11 * q1 * zx * Dvx + c13 * q1 * zz * Dvz - pzc * c11 * zx * Ex - pzc * c13 * zz * Ez;
+            ksz += c13 * q1 * zx * Dvx + c33 * q1 * zz * Dvz - pzc * c13 * zx * Ex - pzc * c33 * zz * Ez;
+            ksxz += c55 * q1 * (zz * Dvx + zx * Dvz) - pzc * c55 * zz * Ex - pzc * c55 * zx * Ez;
+            const float kVx = -pza * Vx + pzb * Dzx;
+            const float kVz = -pza * Vz + pzb * Dzz;
+            kvx += (q1 * Dzx - pzc * Vx) * rinv;
+            kvz += (q1 * Dzz - pzc * Vz) * rinv;
+            rk<STAGE>(U, A, O, g.off[MSFDM_SZVX] + ai, Vx, kVx, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_SZVZ] + ai, Vz, kVz, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_SZEX] + ai, Ex, kEx, ct, cb);
+            rk<STAGE>(U, A, O, g.off[MSFDM_SZEZ] + ai, Ez, kEz, ct, cb);
+        }
+        if (iz == 0) {
+            kif[3 * (long long)P + ix] = kvx;
+            kif[4 * (long long)P + ix] = kvz;
+            kif[5 * (long long)P + ix] = ksx;
+            kif[6 * (long long)P + ix] = ksz;
+            kif[7 * (long long)P + ix] = ksxz;
+            if constexpr (STAGE != 0) continue;
+        }
+        rk<STAGE>(U, A, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cb);
+        rk<STAGE>(U, A, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cb);
+        rk<STAGE>(U, A, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct, cb);
+        rk<STAGE>(U, A, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cb);
+        rk<STAGE>(U, A, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cb);
+    }
+}
+
+// fl_so_boundary<DIR> on the eight interface k values (include/pml_fs_kernels.cuh:754-827).
+// The sxz reconstruction keeps the reference's expression (no nx*nz*snn term).
+template <int DIR>
+__device__ __forceinline__ void interface_bc(float k[8], float2 n) {
+    const float n1 = n.x, n2 = n.y;
+    const float vxf = k[0], vzf = k[1];
+    float pf = k[2];
+    const float vxs = k[3], vzs = k[4], sxx = k[5], szz = k[6], sxz = k[7];
+    float vnf = n1 * vxf + n2 * vzf;
+    const float vtf = n2 * vxf - n1 * vzf;
+    float vns = n1 * vxs + n2 * vzs;
+    const float vts = n2 * vxs - n1 * vzs;
+    float snn = n1 * n1 * sxx + 2.0f * n1 * n2 * sxz + n2 * n2 * szz;
+    float snt = n1 * n2 * (sxx - szz) + (n2 * n2 - n1 * n1) * sxz;
+    const float stt = n2 * n2 * sxx - 2.0f * n1 * n2 * sxz + n1 * n1 * szz;
+    if constexpr (DIR == 0) { vnf = vns; pf = -snn; snt = 0.0f; }
+    else { vns = vnf; snn = -pf; snt = 0.0f; }
+    k[0] = n1 * vnf + n2 * vtf;
+    k[1] = n2 * vnf - n1 * vtf;
+    k[2] = pf;
+    k[3] = n1 * vns + n2 * vts;
+    k[4] = n2 * vns - n1 * vts;
+    k[5] = n1 * n1 * snn + 2.0f * n1 * n2 * snt + n2 * n2 * stt;
+    k[6] = n2 * n2 * snn - 2.0f * n1 * n2 * snt + n1 * n1 * stt;
+    k[7] = -n1 * n2 * stt + (n2 * n2 - n1 * n1) * snt;
+}
+
+__device__ __forceinline__ long long iface_index(const Geo &g, int f, int ix) {
+    // f 0..2: fluid vx, vz, p at row nzf-1; 3..7: solid vx, vz, sx, sz, sxz at row 0
+    return f < 3 ? g.off[MSFDM_FVX + f] + (long long)(g.nzf - 1) * g.P + ix : g.off[MSFDM_SVX + f - 3] + ix;
+}
+
+// Interface rows of stages 1-3 (and of the raw-k mode, STAGE 0).
+template <int DIR, int STAGE>
+__global__ void __launch_bounds__(128)
+iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U, float *__restrict__ A,
+            float *__restrict__ O, const float *__restrict__ kif, float ct, float cb) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix < 3 || ix > g.nx - 4) return;
+    float k[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) k[f] = kif[(long long)f * g.P + ix];
+    interface_bc<DIR>(k, c.nif[ix]);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+        const long long i = iface_index(g, f, ix);
+        rk<STAGE>(U, A, O, i, STAGE == 1 ? S[i] : 0.0f, k[f], ct, cb);
+    }
+}
+
+// Stage-4 interface rows + fl_so_filter (out of place) + receiver records.
+constexpr int FB = 128;
+template <int DIR>
+__global__ void __launch_bounds__(FB)
+iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ A, float *__restrict__ Uo,
+            const float *__restrict__ kif, float cb, int jstep) {
+    __shared__ float pre[8][FB + 2 * HALO];
+    const int nx = g.nx, x0 = blockIdx.x * FB;
+    for (int t = threadIdx.x; t < FB + 2 * HALO; t += FB) {
+        const int ix = x0 - HALO + t;
+        float v[8];
+        if (ix >= 3 && ix <= nx - 4) {
+            float k[8];
+#pragma unroll
+            for (int f = 0; f < 8; ++f) k[f] = kif[(long long)f * g.P + ix];
+            interface_bc<DIR>(k, c.nif[ix]);
+#pragma unroll
+            for (int f = 0; f < 8; ++f) v[f] = A[iface_index(g, f, ix)] + cb * k[f];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 8; ++f) v[f] = 0.0f;
+        }
+#pragma unroll
+        for (int f = 0; f < 8; ++f) pre[f][t] = v[f];
+    }
+    __syncthreads();
+    const int ix = x0 + threadIdx.x;
+    if (ix < 3 || ix > nx - 4) return;
+    const int t = threadIdx.x + HALO;
+    const int klo = max(-3, 3 - ix), khi = min(3, nx - 4 - ix);
+    float wsum = 0.0f;
+    for (int kk = klo; kk <= khi; ++kk) wsum += FILT_W[kk + 3];
+    float v[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+        float s = 0.0f;
+        for (int kk = klo; kk <= khi; ++kk) s += FILT_W[kk + 3] * pre[f][t + kk];
+        v[f] = s / wsum;
+    }
+    const float2 n = c.nif[ix];
+    const float n1 = n.x, n2 = n.y;
+    const float vnf = n1 * v[0] + n2 * v[1], vtf = n2 * v[0] - n1 * v[1];
+    const float vns = n1 * v[3] + n2 * v[4], vts = n2 * v[3] - n1 * v[4];
+    const float snn = n1 * n1 * v[5] + 2.0f * n1 * n2 * v[7] + n2 * n2 * v[6];
+    const float stt = n2 * n2 * v[5] - 2.0f * n1 * n2 * v[7] + n1 * n1 * v[6];
+    const float vn = 0.5f * (vnf + vns), sn = 0.5f * (snn - v[2]);
+    float o[8];
+    o[0] = n1 * vn + n2 * vtf;
+    o[1] = n2 * vn - n1 * vtf;
+    o[2] = -sn;
+    o[3] = n1 * vn + n2 * vts;
+    o[4] = n2 * vn - n1 * vts;
+    o[5] = n1 * n1 * sn + n2 * n2 * stt;
+    o[6] = n2 * n2 * sn + n1 * n1 * stt;
+    o[7] = n1 * n2 * (sn - stt);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) Uo[iface_index(g, f, ix)] = o[f];
+    const int it = ctl.base[0] + jstep;
+    if (it % ctl.gap == 0) {
+        const int slot = it / ctl.gap;
+        if (slot < ctl.rec_cap) {
+            const long long plane = (long long)ctl.rec_cap * nx;
+            float *rr = ctl.rec + (long long)slot * nx + ix;
+            rr[0] = o[3];
+            rr[plane] = o[4];
+            rr[2 * plane] = o[2];
+        }
+    }
+}
+
+__global__ void advance_base(int *base, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) base[0] += n;
+}
+
+}  // namespace dev
+}  // namespace msfdm

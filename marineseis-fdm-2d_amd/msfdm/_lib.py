@@ -1,0 +1,260 @@
+"""ctypes binding of libmsfdm.so (the C ABI declared in include/msfdm.h).
+
+The library is loaded from the package directory (built in-tree by
+``make -C marineseis-fdm-2d_amd/csrc``).  There is no fallback: a missing or
+unloadable library raises, so nothing here can silently run on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libmsfdm.so")
+BIN_PATH = os.path.join(PKG_DIR, "bin", "main_debug")
+
+OK, EINVAL, EIO, EPARSE, EDEVICE, ESTATE = range(6)
+
+FIELDS = ["FVX", "FVZ", "FP", "FXVX", "FXVZ", "FXEX", "FXEZ",
+          "SVX", "SVZ", "SSX", "SSZ", "SSXZ", "SXVX", "SXVZ", "SXEX", "SXEZ",
+          "SZVX", "SZVZ", "SZEX", "SZEZ"]
+FID = {n: i for i, n in enumerate(FIELDS)}
+NFIELD = len(FIELDS)
+ARRAYS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s", "J", "xi_x", "xi_z", "ze_x", "ze_z"]
+
+# Exported symbols (checked by the CPU tests against include/msfdm.h).
+EXPORTS = [
+    "msfdm_last_error", "msfdm_version", "msfdm_model_load_json", "msfdm_model_create",
+    "msfdm_model_get_desc", "msfdm_model_get_array", "msfdm_model_destroy",
+    "msfdm_solver_create", "msfdm_solver_destroy", "msfdm_reset_shot", "msfdm_run",
+    "msfdm_synchronize", "msfdm_step_index", "msfdm_num_records", "msfdm_get_records",
+    "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
+    "msfdm_time_kernel", "msfdm_time_run", "msfdm_bytes_per_step", "msfdm_run_cli",
+]
+
+
+class MsfdmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[msfdm error {code}] {msg}")
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz_fluid", C.c_int), ("nz_solid", C.c_int),
+                ("calc_type", C.c_int), ("boundary_type", C.c_int),
+                ("fpeak", C.c_float), ("dt", C.c_float), ("nt", C.c_int), ("snapshot", C.c_int),
+                ("npd", C.c_int), ("thickness", C.c_float), ("npow", C.c_int),
+                ("vp_max", C.c_float), ("rc", C.c_float), ("beta0", C.c_float),
+                ("nsrc", C.c_int), ("src_x", C.POINTER(C.c_int)), ("src_z", C.POINTER(C.c_int))]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libmsfdm.so once and declare the prototypes."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built (run make -C marineseis-fdm-2d_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, fp, i = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_int
+    protos = {
+        "msfdm_last_error": ([], C.c_char_p),
+        "msfdm_version": ([], C.c_char_p),
+        "msfdm_model_load_json": ([C.c_char_p, C.POINTER(vp)], i),
+        "msfdm_model_create": ([C.POINTER(Desc)] + [fp] * 12 + [C.POINTER(vp)], i),
+        "msfdm_model_get_desc": ([vp, C.POINTER(Desc)], i),
+        "msfdm_model_get_array": ([vp, i, fp], i),
+        "msfdm_model_destroy": ([vp], None),
+        "msfdm_solver_create": ([vp, i, C.POINTER(vp)], i),
+        "msfdm_solver_destroy": ([vp], None),
+        "msfdm_reset_shot": ([vp, i], i),
+        "msfdm_run": ([vp, i], i),
+        "msfdm_synchronize": ([vp], i),
+        "msfdm_step_index": ([vp, ip], i),
+        "msfdm_num_records": ([vp, ip], i),
+        "msfdm_get_records": ([vp, i, i, fp, fp, fp], i),
+        "msfdm_get_field": ([vp, i, fp], i),
+        "msfdm_set_field": ([vp, i, fp], i),
+        "msfdm_eval_rhs": ([vp, i, C.c_float], i),
+        "msfdm_get_rhs": ([vp, i, fp], i),
+        "msfdm_set_graph": ([vp, i], i),
+        "msfdm_time_kernel": ([vp, i, i, fp], i),
+        "msfdm_time_run": ([vp, i, fp], i),
+        "msfdm_bytes_per_step": ([vp, C.POINTER(C.c_double)], i),
+        "msfdm_run_cli": ([C.c_char_p, C.c_char_p], i),
+    }
+    for name, (args, res) in protos.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    return lib().msfdm_last_error().decode()
+
+
+def check(rc: int):
+    if rc != OK:
+        raise MsfdmError(rc, last_error())
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class HostModel:
+    """A msfdm_model handle (host memory only; no GPU needed)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def from_json(cls, path: str) -> "HostModel":
+        h = C.c_void_p()
+        check(lib().msfdm_model_load_json(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_model(cls, m) -> "HostModel":
+        """From a ``msfdm.modelgen.Model``."""
+        sx = np.ascontiguousarray(m.src_x, dtype=np.int32)
+        sz = np.ascontiguousarray(m.src_z, dtype=np.int32)
+        d = Desc(m.nx, m.nz_fluid, m.nz_solid, m.calc_type, m.boundary_type, m.fpeak, m.dt, m.nt,
+                 m.snapshot, m.npd, m.thickness, m.npow, m.vp_max, m.rc, m.beta0, len(sx),
+                 sx.ctypes.data_as(C.POINTER(C.c_int)), sz.ctypes.data_as(C.POINTER(C.c_int)))
+        arrs = [_f32(m.arrays[k]) for k in ARRAYS]
+        h = C.c_void_p()
+        check(lib().msfdm_model_create(C.byref(d), *[_fptr(a) for a in arrs], C.byref(h)))
+        return cls(h)
+
+    def desc(self) -> Desc:
+        d = Desc()
+        check(lib().msfdm_model_get_desc(self.h, C.byref(d)))
+        return d
+
+    def array(self, name: str) -> np.ndarray:
+        d = self.desc()
+        which = ARRAYS.index(name)
+        rows = d.nz_fluid if which < 2 else (d.nz_solid if which < 7 else d.nz_fluid + d.nz_solid - 1)
+        out = np.zeros((rows, d.nx), dtype=np.float32)
+        check(lib().msfdm_model_get_array(self.h, which, _fptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().msfdm_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """Device solver over a HostModel (or a modelgen.Model)."""
+
+    def __init__(self, model, device: int = 0):
+        self._hm = model if isinstance(model, HostModel) else HostModel.from_model(model)
+        d = self._hm.desc()
+        self.nx, self.nzf, self.nzs = d.nx, d.nz_fluid, d.nz_solid
+        self.nt, self.dt = d.nt, d.dt
+        h = C.c_void_p()
+        check(lib().msfdm_solver_create(self._hm.h, int(device), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().msfdm_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, f: int):
+        return (self.nzf if f <= FID["FXEZ"] else self.nzs, self.nx)
+
+    def reset(self, isrc: int = 0):
+        check(lib().msfdm_reset_shot(self.h, int(isrc)))
+
+    def run(self, nsteps: int):
+        check(lib().msfdm_run(self.h, int(nsteps)))
+
+    def sync(self):
+        check(lib().msfdm_synchronize(self.h))
+
+    @property
+    def step(self) -> int:
+        v = C.c_int()
+        check(lib().msfdm_step_index(self.h, C.byref(v)))
+        return v.value
+
+    def num_records(self) -> int:
+        v = C.c_int()
+        check(lib().msfdm_num_records(self.h, C.byref(v)))
+        return v.value
+
+    def records(self, first: int = 0, count: int | None = None):
+        n = self.num_records()
+        count = n - first if count is None else count
+        out = [np.zeros((count, self.nx), dtype=np.float32) for _ in range(3)]
+        check(lib().msfdm_get_records(self.h, int(first), int(count), *[_fptr(o) for o in out]))
+        return out   # solid vx, solid vz, fluid p
+
+    def get(self, f) -> np.ndarray:
+        f = FID[f] if isinstance(f, str) else f
+        out = np.zeros(self.shape(f), dtype=np.float32)
+        check(lib().msfdm_get_field(self.h, f, _fptr(out)))
+        return out
+
+    def set(self, f, arr):
+        f = FID[f] if isinstance(f, str) else f
+        a = _f32(arr).reshape(self.shape(f))
+        check(lib().msfdm_set_field(self.h, f, _fptr(a)))
+
+    def eval_rhs(self, direction: int, amp: float = 0.0):
+        check(lib().msfdm_eval_rhs(self.h, int(direction), float(amp)))
+
+    def get_rhs(self, f) -> np.ndarray:
+        f = FID[f] if isinstance(f, str) else f
+        out = np.zeros(self.shape(f), dtype=np.float32)
+        check(lib().msfdm_get_rhs(self.h, f, _fptr(out)))
+        return out
+
+    def set_graph(self, enable: bool):
+        check(lib().msfdm_set_graph(self.h, int(bool(enable))))
+
+    def time_kernel(self, which: int, reps: int) -> float:
+        v = C.c_float()
+        check(lib().msfdm_time_kernel(self.h, int(which), int(reps), C.byref(v)))
+        return v.value
+
+    def time_run(self, nsteps: int) -> float:
+        v = C.c_float()
+        check(lib().msfdm_time_run(self.h, int(nsteps), C.byref(v)))
+        return v.value
+
+    def bytes_per_step(self) -> float:
+        v = C.c_double()
+        check(lib().msfdm_bytes_per_step(self.h, C.byref(v)))
+        return v.value
+
+
+def run_cli(json_path: str, out_root: str) -> int:
+    """Run the main_debug flow in-process; returns its exit code."""
+    return lib().msfdm_run_cli(json_path.encode(), out_root.encode())

@@ -1,0 +1,173 @@
+"""GPU parity: the HIP stepper (through the C ABI) against the FP64 oracle.
+
+Tolerances: per-field right-hand sides on random states within 2e-5 of the
+field's max |k| (FP32 rounding of ~10-term stencil sums); seismogram records
+within a relative L2 error of 1e-4 over the whole record per component
+(vx, vz on solid row 0; p on the fluid interface row).
+"""
+import numpy as np
+import pytest
+
+import msfdm
+from msfdm import modelgen
+import pyoracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+RECORD_TOL = 1e-4
+
+
+def _random_state(m, seed):
+    rng = np.random.default_rng(seed)
+    st = {}
+    for i, name in enumerate(msfdm.FIELDS):
+        rows = m.nz_fluid if i <= msfdm.FID["FXEZ"] else m.nz_solid
+        st[name] = rng.standard_normal((rows, m.nx))
+    return st
+
+
+@pytest.mark.parametrize("preset", ["tiny_curvi", "cut201"])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_rhs_matches_oracle_on_random_state(preset, direction):
+    print(f"[rhs] {preset} dir={direction}", flush=True)
+    m = modelgen.preset(preset)
+    o = pyoracle.Oracle(m)
+    s = msfdm.Solver(m)
+    st = _random_state(m, 11 + direction)
+    for name, a in st.items():
+        a32 = a.astype(np.float32)
+        o.set(name, a32.astype(np.float64))
+        s.set(name, a32)
+    amp = 0.731
+    o.rhs(direction, amp)
+    s.eval_rhs(direction, amp)
+    for name in msfdm.FIELDS:
+        ko = o.get_k(name)
+        kg = s.get_rhs(name).astype(np.float64)
+        scale = np.abs(ko).max()
+        if scale == 0:
+            assert np.abs(kg).max() == 0, name
+            continue
+        err = np.abs(kg - ko).max() / scale
+        assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
+
+
+def _run_both(m, nsteps, isrc=0, graph=True):
+    o = pyoracle.Oracle(m)
+    o.reset(isrc)
+    o.run(nsteps)
+    s = msfdm.Solver(m)
+    s.set_graph(graph)
+    s.reset(isrc)
+    s.run(nsteps)
+    s.sync()
+    return o, s
+
+
+@pytest.mark.parametrize("preset,nsteps", [("tiny_curvi", 300), ("cut201", 600)])
+def test_records_match_oracle(preset, nsteps):
+    print(f"[records] {preset} nsteps={nsteps}", flush=True)
+    m = modelgen.preset(preset)
+    o, s = _run_both(m, nsteps)
+    ro = o.records()
+    rg = s.records()
+    assert s.num_records() == o.records()[0].shape[0]
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), rg, ro):
+        e = rel_l2(a, b)
+        assert e <= RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+    for name in ("FVX", "FVZ", "FP", "SVX", "SVZ", "SSX", "SSZ", "SSXZ", "SXVX", "SZEX"):
+        e = rel_l2(s.get(name), o.get(name))
+        assert e <= 10 * RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+
+
+def test_flat401_full_record_matches_oracle():
+    """The 401x201 flat-seafloor workload, all 2000 steps (1000 records)."""
+    print("[records] flat401 2000 steps", flush=True)
+    m = modelgen.preset("flat401")
+    o, s = _run_both(m, m.nt)
+    ro, rg = o.records(), s.records()
+    assert rg[0].shape == (1000, 401)
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), rg, ro):
+        e = rel_l2(a, b)
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+
+
+def test_second_shot_and_reset():
+    m = modelgen.preset("tiny_curvi")
+    o, s = _run_both(m, 120, isrc=1)
+    for a, b in zip(s.records(), o.records()):
+        assert rel_l2(a, b) <= RECORD_TOL
+    # a reset brings the solver back to the first shot's result bit for bit
+    s.reset(0); s.run(64); s.sync(); first = [r.copy() for r in s.records()]
+    s.reset(1); s.run(10); s.reset(0); s.run(64); s.sync()
+    for a, b in zip(first, s.records()):
+        assert np.array_equal(a, b)
+
+
+def test_graph_and_direct_launch_agree_bitwise():
+    m = modelgen.preset("tiny_curvi")
+    s1 = msfdm.Solver(m); s1.set_graph(True); s1.reset(0); s1.run(101)
+    s2 = msfdm.Solver(m); s2.set_graph(False); s2.reset(0); s2.run(101)
+    for name in msfdm.FIELDS:
+        assert np.array_equal(s1.get(name), s2.get(name)), name
+    for a, b in zip(s1.records(), s2.records()):
+        assert np.array_equal(a, b)
+
+
+def test_repeat_runs_are_bitwise_deterministic():
+    m = modelgen.preset("cut201")
+    s = msfdm.Solver(m)
+    out = []
+    for _ in range(2):
+        s.reset(0); s.run(200); s.sync()
+        out.append([s.get(n) for n in msfdm.FIELDS] + list(s.records()))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_interface_and_surface_invariants():
+    m = modelgen.preset("tiny_curvi")
+    s = msfdm.Solver(m)
+    s.reset(0)
+    s.run(150)
+    p = s.get("FP")
+    assert np.all(p[3] == 0.0), "pressure-release row must stay exactly zero"
+    for name in msfdm.FIELDS:
+        a = s.get(name)
+        if name.startswith("F"):
+            assert np.all(a[:3] == 0.0), name
+        else:
+            assert np.all(a[-3:] == 0.0), name
+        assert np.all(a[:, :3] == 0.0) and np.all(a[:, -3:] == 0.0), name
+    # after the filter: v_n and p = -sigma_nn agree across the interface
+    nzf = m.nz_fluid
+    zx, zz = m.arrays["ze_x"][nzf - 1].astype(np.float64), m.arrays["ze_z"][nzf - 1].astype(np.float64)
+    nn = np.hypot(zx, zz); n1, n2 = zx / nn, zz / nn
+    c = slice(3, m.nx - 3)
+    vnf = n1 * s.get("FVX")[nzf - 1] + n2 * s.get("FVZ")[nzf - 1]
+    vns = n1 * s.get("SVX")[0] + n2 * s.get("SVZ")[0]
+    assert np.abs(vnf - vns)[c].max() <= 1e-5 * np.abs(vns).max()
+    sxx, szz, sxz = s.get("SSX")[0], s.get("SSZ")[0], s.get("SSXZ")[0]
+    snn = n1 * n1 * sxx + 2 * n1 * n2 * sxz + n2 * n2 * szz
+    assert np.abs(p[nzf - 1] + snn)[c].max() <= 1e-5 * np.abs(snn).max()
+
+
+def test_state_roundtrip_layout():
+    m = modelgen.preset("tiny_curvi")
+    s = msfdm.Solver(m)
+    rng = np.random.default_rng(3)
+    for name in ("FP", "SSXZ", "FXEX", "SZVZ"):
+        a = rng.standard_normal(s.shape(msfdm.FID[name])).astype(np.float32)
+        s.set(name, a)
+        b = s.get(name)
+        if name in ("FXEX",):
+            npd = m.npd
+            cols = list(range(3, 3 + npd)) + list(range(m.nx - 3 - npd, m.nx - 3))
+            assert np.array_equal(b[:, cols], a[:, cols])
+        elif name == "SZVZ":
+            z0 = m.nz_solid - 3 - m.npd
+            assert np.array_equal(b[z0:z0 + m.npd], a[z0:z0 + m.npd])
+        else:
+            assert np.array_equal(a, b)
