@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark of the fluid-solid RK4 stepper on one MI355X.
+
+A bench "step" is one RK4 time step (4 stages + interface filter + receiver
+record) over the whole grid of the workload.  Default workload: the 2001 x 1001
+body-fitted seafloor model (z = 500 + 40 sin(2 pi x / 800) m, 5 % random solid,
+20 Hz, dt = 0.25 ms, 30-cell CFS-PML), K = 6000 steps = one full shot record.
+
+Prints one JSON line: value = Gcell-updates/s (nx * nz * K / wall time, inputs
+resident in HBM), plus the roofline of the dominant kernel (the solid stage
+kernel, HIP-event timed on the solver's stream) and the FP64 CPU baseline on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "marineseis-fdm-2d_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_baseline(model, nsteps: int):
+    """FP64 oracle (OpenMP over rows) timed on `nsteps` steps of the same model."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if cap > 0:
+        threads = min(threads, cap)
+    o = pyoracle.Oracle(model, threads=threads)
+    o.reset(0)
+    o.run(2)   # first-touch / page-in outside the timed sample
+    t0 = time.perf_counter()
+    o.run(nsteps)
+    dt = time.perf_counter() - t0
+    cells = model.nx * model.nz
+    return {"value": cells * nsteps / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{nsteps} FP64 RK4 steps of the same {model.nx}x{model.nz} model "
+                      f"(after 2 untimed steps), {dt:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="seafloor2001")
+    ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--kernel-reps", type=int, default=100)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (x5) to calibrate HBM byte counters")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots shard across GPUs with one gradient all-reduce; not built here)", file=sys.stderr)
+        sys.exit(2)
+
+    import msfdm
+    from msfdm import modelgen
+
+    model = modelgen.preset(args.workload)
+    s = msfdm.Solver(model)
+    s.reset(0)
+    s.run(args.warmup)
+    s.sync()
+    # graph instantiation outside the timed region: the first replay builds it
+    t0 = time.perf_counter()
+    s.run(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    cells = model.nx * model.nz
+    value = cells * args.steps / wall / 1e9
+    bytes_step = s.bytes_per_step()
+
+    # dominant kernel: the solid stage kernel (the largest share of every step)
+    reps = args.kernel_reps
+    t_solid = s.time_kernel(1, reps)
+    t_fluid = s.time_kernel(0, reps)
+    t_if = s.time_kernel(2, reps)
+    t_fin = s.time_kernel(3, reps)
+    b_solid = s.kernel_bytes(1)
+    achieved = b_solid / (t_solid * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(model, args.cpu_steps)
+
+    out = {
+        "metric": "Gcell-updates/s (plain FP32 fluid-solid solver, 1 GPU)",
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (generated body-fitted model, seeded random solid)",
+        "config": {
+            "workload": f"{args.workload}: {model.nx}x{model.nz} grid (fluid {model.nz_fluid} + solid "
+                        f"{model.nz_solid} rows), npd={model.npd}, dt={model.dt}, fpeak={model.fpeak}",
+            "nx": model.nx, "nz": model.nz, "time_steps": args.steps,
+            "hbm_GBps_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9, 1),
+            "hbm_frac_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
+            "bytes_per_step_alg": bytes_step,
+            "kernel_ms": {"solid_stage": round(t_solid, 5), "fluid_stage": round(t_fluid, 5),
+                          "iface_stage": round(t_if, 5), "iface_final": round(t_fin, 5)},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "kernel": "solid_stage<F, stage 2>",
+            "alg_bytes_per_launch": b_solid,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

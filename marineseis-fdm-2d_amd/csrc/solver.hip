@@ -63,7 +63,7 @@ struct msfdm_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double bytes_step = 0.0;
-    std::vector<double> amp_host;   // Ricker amplitudes (double), 4 per step
+    double bytes_kernel[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 namespace {
@@ -413,6 +413,11 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     const double cf = (double)(nzf - 3) * (nx - 6), cs = (double)(nzs - 3) * (nx - 6);
     const double pmlf = (double)(nzf - 3) * 2 * npd, pmls = (double)(nzs - 3) * 2 * npd + (double)npd * (nx - 6);
     s->bytes_step = 4.0 * (cf * (16 * 3 + 4 * 6) + cs * (16 * 5 + 4 * 9) + (pmlf + pmls) * 16 * 4);
+    // stage-2 launches: 5 state floats per field + coefficients (+5 per PML memory field)
+    s->bytes_kernel[0] = 4.0 * (cf * (5 * 3 + 6) + pmlf * 5 * 4);
+    s->bytes_kernel[1] = 4.0 * (cs * (5 * 5 + 9) + pmls * 5 * 4);
+    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 5 + 8 + 2);          // k, state, normal
+    s->bytes_kernel[3] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);              // k, acc, out, normal
     CK(hipStreamSynchronize(s->stream));
 #undef CK
     *out = s;
@@ -621,6 +626,13 @@ extern "C" int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = t;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes) {
+    if (!s || !bytes) return set_error(MSFDM_EINVAL, "null argument");
+    if (which < 0 || which > 3) return set_error(MSFDM_EINVAL, "bad kernel id");
+    *bytes = s->bytes_kernel[which];
     return MSFDM_OK;
 }
 

@@ -31,7 +31,8 @@ EXPORTS = [
     "msfdm_solver_create", "msfdm_solver_destroy", "msfdm_reset_shot", "msfdm_run",
     "msfdm_synchronize", "msfdm_step_index", "msfdm_num_records", "msfdm_get_records",
     "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
-    "msfdm_time_kernel", "msfdm_time_run", "msfdm_bytes_per_step", "msfdm_run_cli",
+    "msfdm_time_kernel", "msfdm_time_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
+    "msfdm_run_cli",
 ]
 
 
@@ -86,6 +87,7 @@ def lib():
         "msfdm_time_kernel": ([vp, i, i, fp], i),
         "msfdm_time_run": ([vp, i, fp], i),
         "msfdm_bytes_per_step": ([vp, C.POINTER(C.c_double)], i),
+        "msfdm_kernel_bytes": ([vp, i, C.POINTER(C.c_double)], i),
         "msfdm_run_cli": ([C.c_char_p, C.c_char_p], i),
     }
     for name, (args, res) in protos.items():
@@ -247,6 +249,11 @@ class Solver:
     def time_run(self, nsteps: int) -> float:
         v = C.c_float()
         check(lib().msfdm_time_run(self.h, int(nsteps), C.byref(v)))
+        return v.value
+
+    def kernel_bytes(self, which: int) -> float:
+        v = C.c_double()
+        check(lib().msfdm_kernel_bytes(self.h, int(which), C.byref(v)))
         return v.value
 
     def bytes_per_step(self) -> float:
