@@ -136,11 +136,16 @@ int msfdm_set_field(msfdm_solver *s, int field, const float *in);
 int msfdm_eval_rhs(msfdm_solver *s, int dir, float amp);
 int msfdm_get_rhs(msfdm_solver *s, int field, float *out);
 
+/* Tile height (grid rows per 256-thread block) of the stage kernels: 8 or 16.
+ * The default is picked from the grid size; the autotuner overrides it. */
+int msfdm_set_tile_rows(msfdm_solver *s, int rows);
+int msfdm_get_tile_rows(const msfdm_solver *s, int *rows);
 /* Enable (1) / disable (0) hipGraph replay of the time loop (default 1). */
 int msfdm_set_graph(msfdm_solver *s, int enable);
-/* Time `reps` launches of one kernel class on the solver's stream with HIP
- * events (which: 0 = fluid stage, 1 = solid stage, 2 = interface stage,
- * 3 = interface stage + filter); returns the mean ms per launch. */
+/* Time `reps` launches of one kernel on the solver's stream with HIP events
+ * (which: 0 = RK stage-2 kernel over both media, 1 = interface-row stage,
+ * 2 = stage-4 interface rows + filter + record); returns the mean ms per
+ * launch.  Overwrites the stage buffers: reset the shot afterwards. */
 int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms);
 /* Time nsteps of msfdm_run between HIP events on the solver's stream. */
 int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);

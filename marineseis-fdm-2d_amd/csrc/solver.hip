@@ -58,12 +58,12 @@ struct msfdm_solver {
     int host_it = 0;
     int isrc = 0;
     int nrec_done = 0;
-    int tz_f = 16, tz_s = 16;
+    int tz = 16;                 // tile height (rows per block)
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double bytes_step = 0.0;
-    double bytes_kernel[4] = {0.0, 0.0, 0.0, 0.0};
+    double bytes_kernel[3] = {0.0, 0.0, 0.0};
 };
 
 namespace {
@@ -159,24 +159,20 @@ int grow_tables(msfdm_solver *s, int steps_needed) {
 // Launches
 // ---------------------------------------------------------------------------
 template <int DIR, int STAGE, int TZ>
-void launch_fluid(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
+void launch_media(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
                   float amp) {
-    const dim3 grid((s->d.nx - 4) / D::TX + 1, (s->d.nz_fluid - 3 + TZ - 1) / TZ);
-    D::fluid_stage<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, A, O, s->kif, j,
-                                                                      ct, cb, amp);
-}
-template <int DIR, int STAGE, int TZ>
-void launch_solid(msfdm_solver *s, const float *S, const float *U, float *A, float *O, float ct, float cb) {
-    const dim3 grid((s->d.nx - 4) / D::TX + 1, (s->d.nz_solid - 3 + TZ - 1) / TZ);
-    D::solid_stage<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, S, U, A, O, s->kif, ct, cb);
+    const int nbx = (s->d.nx - 4) / D::TX + 1;
+    const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
+    const int ntiles = nbx * (nbf + nbs);
+    const int grid = 8 * ((ntiles + 7) / 8);
+    D::stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, A, O, s->kif, j,
+                                                                      ct, cb, amp, nbx, nbf, ntiles);
 }
 template <int DIR, int STAGE>
 void launch_stage(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
                   float amp) {
-    if (s->tz_f == 8) launch_fluid<DIR, STAGE, 8>(s, S, U, A, O, j, ct, cb, amp);
-    else launch_fluid<DIR, STAGE, 16>(s, S, U, A, O, j, ct, cb, amp);
-    if (s->tz_s == 8) launch_solid<DIR, STAGE, 8>(s, S, U, A, O, ct, cb);
-    else launch_solid<DIR, STAGE, 16>(s, S, U, A, O, ct, cb);
+    if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, A, O, j, ct, cb, amp);
+    else launch_media<DIR, STAGE, 16>(s, S, U, A, O, j, ct, cb, amp);
     if constexpr (STAGE == 4) {
         D::iface_final<DIR><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, A, O,
                                                                                     s->kif, cb, j);
@@ -405,7 +401,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
 
     // Tile heights: small grids need more blocks to fill 256 CUs.
     const long long cells = (long long)nx * nz;
-    s->tz_f = s->tz_s = cells < 1000000 ? 8 : 16;
+    s->tz = cells < 1000000 ? 8 : 16;
 
     // Algorithmic bytes per step: 16 state + 4 coefficient floats per field
     // (fluid 3 fields / 6 coefficients, solid 5 / 9) per updated cell, plus
@@ -413,11 +409,10 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     const double cf = (double)(nzf - 3) * (nx - 6), cs = (double)(nzs - 3) * (nx - 6);
     const double pmlf = (double)(nzf - 3) * 2 * npd, pmls = (double)(nzs - 3) * 2 * npd + (double)npd * (nx - 6);
     s->bytes_step = 4.0 * (cf * (16 * 3 + 4 * 6) + cs * (16 * 5 + 4 * 9) + (pmlf + pmls) * 16 * 4);
-    // stage-2 launches: 5 state floats per field + coefficients (+5 per PML memory field)
-    s->bytes_kernel[0] = 4.0 * (cf * (5 * 3 + 6) + pmlf * 5 * 4);
-    s->bytes_kernel[1] = 4.0 * (cs * (5 * 5 + 9) + pmls * 5 * 4);
-    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 5 + 8 + 2);          // k, state, normal
-    s->bytes_kernel[3] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);              // k, acc, out, normal
+    // stage-2 launch: 5 state floats per field + coefficients (+5 per PML memory field)
+    s->bytes_kernel[0] = 4.0 * (cf * (5 * 3 + 6) + cs * (5 * 5 + 9) + (pmlf + pmls) * 5 * 4);
+    s->bytes_kernel[1] = 4.0 * (double)(nx - 6) * (8 * 5 + 8 + 2);          // k, state, normal
+    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);              // k, acc, out, normal
     CK(hipStreamSynchronize(s->stream));
 #undef CK
     *out = s;
@@ -556,6 +551,23 @@ extern "C" int msfdm_get_rhs(msfdm_solver *s, int field, float *out) {
     return copy_field(s, s->kbuf, field, out, nullptr);
 }
 
+extern "C" int msfdm_set_tile_rows(msfdm_solver *s, int rows) {
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (rows != 8 && rows != 16) return set_error(MSFDM_EINVAL, "tile rows must be 8 or 16");
+    if (rows != s->tz) {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        drop_graph(s);
+        s->tz = rows;
+    }
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_get_tile_rows(const msfdm_solver *s, int *rows) {
+    if (!s || !rows) return set_error(MSFDM_EINVAL, "null argument");
+    *rows = s->tz;
+    return MSFDM_OK;
+}
+
 extern "C" int msfdm_set_graph(msfdm_solver *s, int enable) {
     if (!s) return set_error(MSFDM_EINVAL, "null solver");
     s->use_graph = enable != 0;
@@ -573,14 +585,10 @@ extern "C" int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms
     auto one = [&]() {
         switch (which) {
             case 0:
-                if (s->tz_f == 8) launch_fluid<0, 2, 8>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
-                else launch_fluid<0, 2, 16>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
+                if (s->tz == 8) launch_media<0, 2, 8>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
+                else launch_media<0, 2, 16>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
                 break;
             case 1:
-                if (s->tz_s == 8) launch_solid<0, 2, 8>(s, T1, U, A, T2, dt / 2, dt / 3);
-                else launch_solid<0, 2, 16>(s, T1, U, A, T2, dt / 2, dt / 3);
-                break;
-            case 2:
                 D::iface_stage<0, 2><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, T1, U, A, T2, s->kif, dt / 2, dt / 3);
                 break;
             default: {
@@ -631,7 +639,7 @@ extern "C" int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms) {
 
 extern "C" int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes) {
     if (!s || !bytes) return set_error(MSFDM_EINVAL, "null argument");
-    if (which < 0 || which > 3) return set_error(MSFDM_EINVAL, "bad kernel id");
+    if (which < 0 || which > 2) return set_error(MSFDM_EINVAL, "bad kernel id");
     *bytes = s->bytes_kernel[which];
     return MSFDM_OK;
 }

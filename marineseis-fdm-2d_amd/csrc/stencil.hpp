@@ -1,11 +1,12 @@
 // Device code of the fluid-solid RK4 stepper for gfx950.
 //
-// One RK stage = three launches:
-//   fluid_stage<DIR,STAGE,TZ>  fluid RHS (update_velocity_fluid + update_stress_fluid,
+// One RK stage = two launches:
+//   stage_kernel<DIR,STAGE,TZ> runs, per 64 x TZ tile,
+//   fluid_body                 fluid RHS (update_velocity_fluid + update_stress_fluid,
 //                              include/pml_fs_kernels.cuh:333-507) + xi-PML memory
 //                              + source (src/pml_fs_kernels.cu:527-531, 618-621)
 //                              + the RK4 combination (src/pml_fs_kernels.cu:663-763)
-//   solid_stage<DIR,STAGE,TZ>  solid RHS (update_stress_solid + update_velocity_solid,
+//   solid_body                 solid RHS (update_stress_solid + update_velocity_solid,
 //                              include/pml_fs_kernels.cuh:509-752) + xi/zeta-PML memory
 //                              + the RK4 combination
 //   iface_stage<DIR,STAGE>     fl_so_boundary (include/pml_fs_kernels.cuh:754-827)
@@ -166,14 +167,14 @@ __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const
 // Fluid stage: rows 3..nzf-1, columns 3..nx-4.
 // ---------------------------------------------------------------------------
 template <int DIR, int STAGE, int TZ>
-__global__ void __launch_bounds__(NTHR)
-fluid_stage(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
-            float *__restrict__ A, float *__restrict__ O, float *__restrict__ kif, int jstep, float ct,
-            float cb, float amp_direct) {
-    constexpr int LZ = TZ + 2 * HALO;
-    __shared__ float sh[3][LZ][LXP];   // vx, vz, p
+__device__ __forceinline__ void
+fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
+           const float *__restrict__ S, const float *__restrict__ U, float *__restrict__ A,
+           float *__restrict__ O, float *__restrict__ kif, int jstep, float ct, float cb,
+           float amp_direct, int bx, int bz) {
+    // sh[0..2] = vx, vz, p of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    const int x0 = blockIdx.x * TX, z0 = 3 + blockIdx.y * TZ;
+    const int x0 = bx * TX, z0 = 3 + bz * TZ;
     const int P = g.P, nx = g.nx, nzf = g.nzf;
     load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
     __syncthreads();
@@ -315,13 +316,13 @@ fluid_stage(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__
 // Solid stage: rows 0..nzs-4, columns 3..nx-4.  Metrics at global row iz+nzf-1.
 // ---------------------------------------------------------------------------
 template <int DIR, int STAGE, int TZ>
-__global__ void __launch_bounds__(NTHR)
-solid_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U,
-            float *__restrict__ A, float *__restrict__ O, float *__restrict__ kif, float ct, float cb) {
-    constexpr int LZ = TZ + 2 * HALO;
-    __shared__ float sh[5][LZ][LXP];   // vx, vz, sx, sz, sxz
+__device__ __forceinline__ void
+solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const float *__restrict__ S,
+           const float *__restrict__ U, float *__restrict__ A, float *__restrict__ O,
+           float *__restrict__ kif, float ct, float cb, int bx, int bz) {
+    // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int x0 = bx * TX, z0 = bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
@@ -474,6 +475,27 @@ solid_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict_
         rk<STAGE>(U, A, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cb);
         rk<STAGE>(U, A, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cb);
     }
+}
+
+// One RK stage over both media in a single launch.  The grid is 1-D with a
+// multiple of 8 blocks; consecutive blocks go round-robin to the 8 XCDs, so
+// block b takes logical tile (b % 8) * (grid / 8) + b / 8: each XCD works on a
+// contiguous run of tiles (row bands of 64-column tiles) and finds the halo
+// rows of its neighbours in its own L2.  Tiles: fluid bands first, then solid.
+template <int DIR, int STAGE, int TZ>
+__global__ void __launch_bounds__(NTHR)
+stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
+             float *__restrict__ A, float *__restrict__ O, float *__restrict__ kif, int jstep, float ct,
+             float cb, float amp_direct, int nbx, int nbz_f, int ntiles) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    const int per_xcd = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int band = tile / nbx, bx = tile - band * nbx;
+    if (band < nbz_f)
+        fluid_body<DIR, STAGE, TZ>(sh, g, c, ctl, S, U, A, O, kif, jstep, ct, cb, amp_direct, bx, band);
+    else
+        solid_body<DIR, STAGE, TZ>(sh, g, c, S, U, A, O, kif, ct, cb, bx, band - nbz_f);
 }
 
 // fl_so_boundary<DIR> on the eight interface k values (include/pml_fs_kernels.cuh:754-827).

@@ -31,6 +31,7 @@ EXPORTS = [
     "msfdm_solver_create", "msfdm_solver_destroy", "msfdm_reset_shot", "msfdm_run",
     "msfdm_synchronize", "msfdm_step_index", "msfdm_num_records", "msfdm_get_records",
     "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
+    "msfdm_set_tile_rows", "msfdm_get_tile_rows",
     "msfdm_time_kernel", "msfdm_time_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
     "msfdm_run_cli",
 ]
@@ -84,6 +85,8 @@ def lib():
         "msfdm_eval_rhs": ([vp, i, C.c_float], i),
         "msfdm_get_rhs": ([vp, i, fp], i),
         "msfdm_set_graph": ([vp, i], i),
+        "msfdm_set_tile_rows": ([vp, i], i),
+        "msfdm_get_tile_rows": ([vp, ip], i),
         "msfdm_time_kernel": ([vp, i, i, fp], i),
         "msfdm_time_run": ([vp, i, fp], i),
         "msfdm_bytes_per_step": ([vp, C.POINTER(C.c_double)], i),
@@ -237,6 +240,16 @@ class Solver:
         out = np.zeros(self.shape(f), dtype=np.float32)
         check(lib().msfdm_get_rhs(self.h, f, _fptr(out)))
         return out
+
+    @property
+    def tile_rows(self) -> int:
+        v = C.c_int()
+        check(lib().msfdm_get_tile_rows(self.h, C.byref(v)))
+        return v.value
+
+    @tile_rows.setter
+    def tile_rows(self, rows: int):
+        check(lib().msfdm_set_tile_rows(self.h, int(rows)))
 
     def set_graph(self, enable: bool):
         check(lib().msfdm_set_graph(self.h, int(bool(enable))))

@@ -29,11 +29,13 @@ def _random_state(m, seed):
 
 @pytest.mark.parametrize("preset", ["tiny_curvi", "cut201"])
 @pytest.mark.parametrize("direction", [0, 1])
-def test_rhs_matches_oracle_on_random_state(preset, direction):
-    print(f"[rhs] {preset} dir={direction}", flush=True)
+@pytest.mark.parametrize("tile_rows", [8, 16])
+def test_rhs_matches_oracle_on_random_state(preset, direction, tile_rows):
+    print(f"[rhs] {preset} dir={direction} tile_rows={tile_rows}", flush=True)
     m = modelgen.preset(preset)
     o = pyoracle.Oracle(m)
     s = msfdm.Solver(m)
+    s.tile_rows = tile_rows
     st = _random_state(m, 11 + direction)
     for name, a in st.items():
         a32 = a.astype(np.float32)
@@ -53,11 +55,13 @@ def test_rhs_matches_oracle_on_random_state(preset, direction):
         assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
 
 
-def _run_both(m, nsteps, isrc=0, graph=True):
+def _run_both(m, nsteps, isrc=0, graph=True, tile_rows=None):
     o = pyoracle.Oracle(m)
     o.reset(isrc)
     o.run(nsteps)
     s = msfdm.Solver(m)
+    if tile_rows:
+        s.tile_rows = tile_rows
     s.set_graph(graph)
     s.reset(isrc)
     s.run(nsteps)
@@ -65,11 +69,12 @@ def _run_both(m, nsteps, isrc=0, graph=True):
     return o, s
 
 
-@pytest.mark.parametrize("preset,nsteps", [("tiny_curvi", 300), ("cut201", 600)])
-def test_records_match_oracle(preset, nsteps):
-    print(f"[records] {preset} nsteps={nsteps}", flush=True)
+@pytest.mark.parametrize("preset,nsteps,tile_rows", [("tiny_curvi", 300, 8), ("tiny_curvi", 300, 16),
+                                                     ("cut201", 600, 16)])
+def test_records_match_oracle(preset, nsteps, tile_rows):
+    print(f"[records] {preset} nsteps={nsteps} tile_rows={tile_rows}", flush=True)
     m = modelgen.preset(preset)
-    o, s = _run_both(m, nsteps)
+    o, s = _run_both(m, nsteps, tile_rows=tile_rows)
     ro = o.records()
     rg = s.records()
     assert s.num_records() == o.records()[0].shape[0]
