@@ -152,9 +152,8 @@ int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
 /* Algorithmic HBM bytes of one time step (state + coefficients + PML memory). */
 int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
 /* Algorithmic HBM bytes of one launch timed by msfdm_time_kernel(which): an
- * RK stage-2 launch reads the stage state, U^n and the accumulator and writes
- * the accumulator and the next stage (5 floats per field) plus the
- * coefficients once, per updated cell. */
+ * RK stage-2 launch reads the stage state and U^n and writes the next stage
+ * state (3 floats per field) plus the coefficients once, per updated cell. */
 int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
 
 /* --- Process boundary ---------------------------------------------------- */

@@ -30,7 +30,7 @@ namespace {
 
 constexpr int GRAPH_STEPS = 32;   // steps captured per graph replay (even: F/B parity is baked in)
 
-enum Role { R_U = 0, R_A = 1, R_T1 = 2, R_T2 = 3 };
+enum Role { R_U = 0, R_T2 = 1, R_T3 = 2, R_T4 = 3 };
 
 inline long long align64(long long n) { return (n + 63) / 64 * 64; }
 
@@ -47,7 +47,7 @@ struct msfdm_solver {
     long long buf_floats = 0;
     float *buf[4] = {nullptr, nullptr, nullptr, nullptr};
     float *kbuf = nullptr;       // raw-k output of msfdm_eval_rhs
-    float *kif = nullptr;        // [8][P] interface-row k
+    float *kif = nullptr;        // [8][P] interface-row k + [8][P] interface rows of U^n
     float *coefmem = nullptr;
     float *amp = nullptr;
     int *base = nullptr;
@@ -159,26 +159,25 @@ int grow_tables(msfdm_solver *s, int steps_needed) {
 // Launches
 // ---------------------------------------------------------------------------
 template <int DIR, int STAGE, int TZ>
-void launch_media(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
-                  float amp) {
+void launch_media(msfdm_solver *s, const float *S, const float *U, const float *T2, const float *T3, float *O, int j,
+                  float ct, float amp) {
     const int nbx = (s->d.nx - 4) / D::TX + 1;
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
     const int grid = 8 * ((ntiles + 7) / 8);
-    D::stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, A, O, s->kif, j,
-                                                                      ct, cb, amp, nbx, nbf, ntiles);
+    D::stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif,
+                                                                      j, ct, amp, nbx, nbf, ntiles);
 }
 template <int DIR, int STAGE>
-void launch_stage(msfdm_solver *s, const float *S, const float *U, float *A, float *O, int j, float ct, float cb,
-                  float amp) {
-    if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, A, O, j, ct, cb, amp);
-    else launch_media<DIR, STAGE, 16>(s, S, U, A, O, j, ct, cb, amp);
+void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, float amp) {
+    const float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3];
+    if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, T2, T3, O, j, ct, amp);
+    else launch_media<DIR, STAGE, 16>(s, S, U, T2, T3, O, j, ct, amp);
     if constexpr (STAGE == 4) {
-        D::iface_final<DIR><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, A, O,
-                                                                                    s->kif, cb, j);
+        D::iface_final<DIR><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, T2, T3,
+                                                                                    O, s->kif, ct, j);
     } else {
-        D::iface_stage<DIR, STAGE><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, S, U, A, O, s->kif,
-                                                                                 ct, cb);
+        D::iface_stage<DIR, STAGE><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, S, U, O, s->kif, ct);
     }
 }
 
@@ -188,11 +187,11 @@ template <int D1>
 void enqueue_step_dir(msfdm_solver *s, int j) {
     constexpr int D2 = 1 - D1;
     const float dt = s->d.dt;
-    float *U = s->buf[R_U], *A = s->buf[R_A], *T1 = s->buf[R_T1], *T2 = s->buf[R_T2];
-    launch_stage<D1, 1>(s, U, U, A, T1, j, dt / 2, dt / 6, 0.f);
-    launch_stage<D2, 2>(s, T1, U, A, T2, j, dt / 2, dt / 3, 0.f);
-    launch_stage<D1, 3>(s, T2, U, A, T1, j, dt, dt / 3, 0.f);
-    launch_stage<D2, 4>(s, T1, U, A, U, j, 0.f, dt / 6, 0.f);
+    float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3], *T4 = s->buf[R_T4];
+    launch_stage<D1, 1>(s, U, T2, j, dt / 2, 0.f);
+    launch_stage<D2, 2>(s, T2, T3, j, dt / 2, 0.f);
+    launch_stage<D1, 3>(s, T3, T4, j, dt, 0.f);
+    launch_stage<D2, 4>(s, T4, U, j, dt / 6, 0.f);
 }
 void enqueue_steps(msfdm_solver *s, int it0, int n) {
     for (int j = 0; j < n; ++j) {
@@ -322,8 +321,8 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
         CK(hipMalloc(&s->buf[r], off * sizeof(float)));
         CK(hipMemset(s->buf[r], 0, off * sizeof(float)));
     }
-    CK(hipMalloc(&s->kif, 8 * P * sizeof(float)));
-    CK(hipMemset(s->kif, 0, 8 * P * sizeof(float)));
+    CK(hipMalloc(&s->kif, 16 * P * sizeof(float)));
+    CK(hipMemset(s->kif, 0, 16 * P * sizeof(float)));
     CK(hipMalloc(&s->base, 64 * sizeof(int)));
     CK(hipMemset(s->base, 0, 64 * sizeof(int)));
 
@@ -403,16 +402,18 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     const long long cells = (long long)nx * nz;
     s->tz = cells < 1000000 ? 8 : 16;
 
-    // Algorithmic bytes per step: 16 state + 4 coefficient floats per field
-    // (fluid 3 fields / 6 coefficients, solid 5 / 9) per updated cell, plus
-    // 16 floats x 4 memory fields per active PML direction.
+    // Algorithmic bytes per step: 13 state floats per field (stage 1: read U,
+    // write T2; stages 2-3: read stage state + U, write next; stage 4: read
+    // T4, U, T2, T3, write U) + 4 coefficient floats (fluid 3 fields /
+    // 6 coefficients, solid 5 / 9) per updated cell, plus 13 floats x 4
+    // memory fields per active PML direction.
     const double cf = (double)(nzf - 3) * (nx - 6), cs = (double)(nzs - 3) * (nx - 6);
     const double pmlf = (double)(nzf - 3) * 2 * npd, pmls = (double)(nzs - 3) * 2 * npd + (double)npd * (nx - 6);
-    s->bytes_step = 4.0 * (cf * (16 * 3 + 4 * 6) + cs * (16 * 5 + 4 * 9) + (pmlf + pmls) * 16 * 4);
-    // stage-2 launch: 5 state floats per field + coefficients (+5 per PML memory field)
-    s->bytes_kernel[0] = 4.0 * (cf * (5 * 3 + 6) + cs * (5 * 5 + 9) + (pmlf + pmls) * 5 * 4);
-    s->bytes_kernel[1] = 4.0 * (double)(nx - 6) * (8 * 5 + 8 + 2);          // k, state, normal
-    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);              // k, acc, out, normal
+    s->bytes_step = 4.0 * (cf * (13 * 3 + 4 * 6) + cs * (13 * 5 + 4 * 9) + (pmlf + pmls) * 13 * 4);
+    // stage-2 launch: 3 state floats per field + coefficients (+3 per PML memory field)
+    s->bytes_kernel[0] = 4.0 * (cf * (3 * 3 + 6) + cs * (3 * 5 + 9) + (pmlf + pmls) * 3 * 4);
+    s->bytes_kernel[1] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);          // k, U, out, normal
+    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 6 + 2);          // k, U^n, T2, T3, T4, out, normal
     CK(hipStreamSynchronize(s->stream));
 #undef CK
     *out = s;
@@ -520,7 +521,7 @@ extern "C" int msfdm_set_field(msfdm_solver *s, int field, const float *in) {
     HIPCHK(hipSetDevice(s->device));
     // The stage buffers keep the same values in cells the stepper never
     // updates (frozen bands), as the reference's RK4_reset copies do.
-    for (int r : {R_U, R_T1, R_T2})
+    for (int r : {R_U, R_T2, R_T3, R_T4})
         if (int rc = copy_field(s, s->buf[r], field, nullptr, in)) return rc;
     return MSFDM_OK;
 }
@@ -535,8 +536,8 @@ extern "C" int msfdm_eval_rhs(msfdm_solver *s, int dir, float amp) {
     }
     HIPCHK(hipMemsetAsync(s->kbuf, 0, s->buf_floats * sizeof(float), s->stream));
     float *U = s->buf[R_U];
-    if (dir == 0) launch_stage<0, 0>(s, U, U, s->buf[R_A], s->kbuf, 0, 0.f, 0.f, amp);
-    else launch_stage<1, 0>(s, U, U, s->buf[R_A], s->kbuf, 0, 0.f, 0.f, amp);
+    if (dir == 0) launch_stage<0, 0>(s, U, s->kbuf, 0, 0.f, amp);
+    else launch_stage<1, 0>(s, U, s->kbuf, 0, 0.f, amp);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     return MSFDM_OK;
@@ -580,21 +581,21 @@ extern "C" int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms
     if (!s || !ms || reps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
     HIPCHK(hipSetDevice(s->device));
     if (int rc = grow_tables(s, s->host_it + 1)) return rc;   // the stage reads amplitude row host_it
-    float *U = s->buf[R_U], *A = s->buf[R_A], *T1 = s->buf[R_T1], *T2 = s->buf[R_T2];
+    float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3], *T4 = s->buf[R_T4];
     const float dt = s->d.dt;
     auto one = [&]() {
         switch (which) {
             case 0:
-                if (s->tz == 8) launch_media<0, 2, 8>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
-                else launch_media<0, 2, 16>(s, T1, U, A, T2, 0, dt / 2, dt / 3, 0.f);
+                if (s->tz == 8) launch_media<0, 2, 8>(s, T2, U, T2, T3, T3, 0, dt / 2, 0.f);
+                else launch_media<0, 2, 16>(s, T2, U, T2, T3, T3, 0, dt / 2, 0.f);
                 break;
             case 1:
-                D::iface_stage<0, 2><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, T1, U, A, T2, s->kif, dt / 2, dt / 3);
+                D::iface_stage<0, 2><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, T2, U, T3, s->kif, dt / 2);
                 break;
             default: {
                 D::Ctl quiet = s->ctl;   // no record slot is written while timing
                 quiet.gap = 0x7fffffff;
-                D::iface_final<0><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, A, T2, s->kif, dt / 6, 1);
+                D::iface_final<0><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, T4, T2, T3, U, s->kif, dt / 6, 1);
                 break;
             }
         }

@@ -110,20 +110,26 @@ __device__ __forceinline__ float mac_dz(const float (*t)[LXP], int r, int c, int
     return solid_side ? (-t[r][c] + t[r + 1][c]) : -(-t[r][c] + t[r - 1][c]);
 }
 
-// RK4 combination of one value (src/pml_fs_kernels.cu:663-763 with
-// MemManager::RK4_reset folded in): stage 1 reads U from the stage state.
+// Classical RK4 (src/pml_fs_kernels.cu:663-763) on stored stage states:
+//   T2 = U + dt/2 k1,  T3 = U + dt/2 k2,  T4 = U + dt k3,
+//   U' = U + dt/6 (k1 + 2 k2 + 2 k3 + k4)
+//      = U + ((T2 - U) + 2 (T3 - U) + (T4 - U)) / 3 + dt/6 k4.
+// Writing the stage states instead of an accumulator moves 13 instead of 16
+// floats per field and step; the rounding of U' is no larger than that of the
+// accumulator form.  s is the stage state at the cell (U in stage 1, T4 in
+// stage 4); STAGE 0 writes the raw k.
 template <int STAGE>
-__device__ __forceinline__ void rk(const float *__restrict__ U, float *__restrict__ A, float *__restrict__ O,
-                                   long long i, float s, float k, float ct, float cb) {
+__device__ __forceinline__ void rk(const float *__restrict__ U, const float *__restrict__ T2,
+                                   const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
+                                   float k, float ct) {
     if constexpr (STAGE == 0) {
         O[i] = k;
     } else if constexpr (STAGE == 1) {
-        A[i] = s + cb * k;
         O[i] = s + ct * k;
     } else if constexpr (STAGE == 4) {
-        O[i] = A[i] + cb * k;
+        const float u = U[i];
+        O[i] = u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k;
     } else {
-        A[i] = A[i] + cb * k;
         O[i] = U[i] + ct * k;
     }
 }
@@ -169,9 +175,9 @@ __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const
 template <int DIR, int STAGE, int TZ>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
-           const float *__restrict__ S, const float *__restrict__ U, float *__restrict__ A,
-           float *__restrict__ O, float *__restrict__ kif, int jstep, float ct, float cb,
-           float amp_direct, int bx, int bz) {
+           const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
+           const float *__restrict__ T3, float *__restrict__ O, float *__restrict__ kif, int jstep,
+           float ct, float amp_direct, int bx, int bz) {
     // sh[0..2] = vx, vz, p of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = 3 + bz * TZ;
@@ -289,10 +295,10 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 kEz = -pxa * Ez + pxb * Dvz;
                 kp += Km * (-(pxc - 1.0f) * xx * Dvx - (pxc - 1.0f) * xz * Dvz + pxc * xx * Ex + pxc * xz * Ez);
             }
-            rk<STAGE>(U, A, O, g.off[MSFDM_FXVX] + ai, Vx, kVx, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_FXVZ] + ai, Vz, kVz, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_FXEX] + ai, Ex, kEx, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_FXEZ] + ai, Ez, kEz, ct, cb);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXVX] + ai, Vx, kVx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXVZ] + ai, Vz, kVz, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXEX] + ai, Ex, kEx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXEZ] + ai, Ez, kEz, ct);
         }
         // ---- source: 5x5 Gaussian-weighted Ricker on p_k (include/tools_func.cuh:21-37)
         {
@@ -306,9 +312,9 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             kif[2 * (long long)P + ix] = kp;
             if constexpr (STAGE != 0) continue;
         }
-        rk<STAGE>(U, A, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct, cb);
-        rk<STAGE>(U, A, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct, cb);
-        rk<STAGE>(U, A, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct, cb);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct);
     }
 }
 
@@ -318,8 +324,8 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 template <int DIR, int STAGE, int TZ>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const float *__restrict__ S,
-           const float *__restrict__ U, float *__restrict__ A, float *__restrict__ O,
-           float *__restrict__ kif, float ct, float cb, int bx, int bz) {
+           const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
+           float *__restrict__ O, float *__restrict__ kif, float ct, int bx, int bz) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = bz * TZ;
@@ -433,10 +439,10 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             const float kVz = -pxa * Vz + pxb * Dxz;
             kvx += (q1 * Dxx - pxc * Vx) * rinv;
             kvz += (q1 * Dxz - pxc * Vz) * rinv;
-            rk<STAGE>(U, A, O, g.off[MSFDM_SXVX] + ai, Vx, kVx, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_SXVZ] + ai, Vz, kVz, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_SXEX] + ai, Ex, kEx, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_SXEZ] + ai, Ez, kEz, ct, cb);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SXVX] + ai, Vx, kVx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SXVZ] + ai, Vz, kVz, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SXEX] + ai, Ex, kEx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SXEZ] + ai, Ez, kEz, ct);
         }
         // ---- zeta-PML memory (bottom strip)
         if (bottom) {
@@ -456,10 +462,10 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             const float kVz = -pza * Vz + pzb * Dzz;
             kvx += (q1 * Dzx - pzc * Vx) * rinv;
             kvz += (q1 * Dzz - pzc * Vz) * rinv;
-            rk<STAGE>(U, A, O, g.off[MSFDM_SZVX] + ai, Vx, kVx, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_SZVZ] + ai, Vz, kVz, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_SZEX] + ai, Ex, kEx, ct, cb);
-            rk<STAGE>(U, A, O, g.off[MSFDM_SZEZ] + ai, Ez, kEz, ct, cb);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZVX] + ai, Vx, kVx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZVZ] + ai, Vz, kVz, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZEX] + ai, Ex, kEx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZEZ] + ai, Ez, kEz, ct);
         }
         if (iz == 0) {
             kif[3 * (long long)P + ix] = kvx;
@@ -469,11 +475,11 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             kif[7 * (long long)P + ix] = ksxz;
             if constexpr (STAGE != 0) continue;
         }
-        rk<STAGE>(U, A, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cb);
-        rk<STAGE>(U, A, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cb);
-        rk<STAGE>(U, A, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct, cb);
-        rk<STAGE>(U, A, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cb);
-        rk<STAGE>(U, A, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cb);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct);
+        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct);
     }
 }
 
@@ -482,20 +488,23 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
 // block b takes logical tile (b % 8) * (grid / 8) + b / 8: each XCD works on a
 // contiguous run of tiles (row bands of 64-column tiles) and finds the halo
 // rows of its neighbours in its own L2.  Tiles: fluid bands first, then solid.
+#ifndef MSFDM_STAGE_WAVES
+#define MSFDM_STAGE_WAVES 0   // minimum waves per SIMD requested for stage_kernel (0 = compiler choice)
+#endif
 template <int DIR, int STAGE, int TZ>
-__global__ void __launch_bounds__(NTHR)
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
-             float *__restrict__ A, float *__restrict__ O, float *__restrict__ kif, int jstep, float ct,
-             float cb, float amp_direct, int nbx, int nbz_f, int ntiles) {
+             const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
+             float *__restrict__ kif, int jstep, float ct, float amp_direct, int nbx, int nbz_f, int ntiles) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
     const int per_xcd = gridDim.x >> 3;
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
     if (band < nbz_f)
-        fluid_body<DIR, STAGE, TZ>(sh, g, c, ctl, S, U, A, O, kif, jstep, ct, cb, amp_direct, bx, band);
+        fluid_body<DIR, STAGE, TZ>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, bx, band);
     else
-        solid_body<DIR, STAGE, TZ>(sh, g, c, S, U, A, O, kif, ct, cb, bx, band - nbz_f);
+        solid_body<DIR, STAGE, TZ>(sh, g, c, S, U, T2, T3, O, kif, ct, bx, band - nbz_f);
 }
 
 // fl_so_boundary<DIR> on the eight interface k values (include/pml_fs_kernels.cuh:754-827).
@@ -530,11 +539,13 @@ __device__ __forceinline__ long long iface_index(const Geo &g, int f, int ix) {
     return f < 3 ? g.off[MSFDM_FVX + f] + (long long)(g.nzf - 1) * g.P + ix : g.off[MSFDM_SVX + f - 3] + ix;
 }
 
-// Interface rows of stages 1-3 (and of the raw-k mode, STAGE 0).
+// Interface rows of stages 1-3 (and of the raw-k mode, STAGE 0).  kif holds
+// [8][P] interface k values followed by [8][P] copies of U^n on the interface
+// rows, saved in stage 1 for the stage-4 kernel (which overwrites U there).
 template <int DIR, int STAGE>
 __global__ void __launch_bounds__(128)
-iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U, float *__restrict__ A,
-            float *__restrict__ O, const float *__restrict__ kif, float ct, float cb) {
+iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U,
+            float *__restrict__ O, float *__restrict__ kif, float ct) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     if (ix < 3 || ix > g.nx - 4) return;
     float k[8];
@@ -544,7 +555,9 @@ iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict_
 #pragma unroll
     for (int f = 0; f < 8; ++f) {
         const long long i = iface_index(g, f, ix);
-        rk<STAGE>(U, A, O, i, STAGE == 1 ? S[i] : 0.0f, k[f], ct, cb);
+        const float sv = STAGE == 1 ? S[i] : 0.0f;
+        if constexpr (STAGE == 1) kif[(long long)(8 + f) * g.P + ix] = sv;
+        rk<STAGE>(U, nullptr, nullptr, O, i, sv, k[f], ct);
     }
 }
 
@@ -552,8 +565,9 @@ iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict_
 constexpr int FB = 128;
 template <int DIR>
 __global__ void __launch_bounds__(FB)
-iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ A, float *__restrict__ Uo,
-            const float *__restrict__ kif, float cb, int jstep) {
+iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ T4, const float *__restrict__ T2,
+            const float *__restrict__ T3, float *__restrict__ Uo, const float *__restrict__ kif, float cb,
+            int jstep) {
     __shared__ float pre[8][FB + 2 * HALO];
     const int nx = g.nx, x0 = blockIdx.x * FB;
     for (int t = threadIdx.x; t < FB + 2 * HALO; t += FB) {
@@ -565,7 +579,11 @@ iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ A, float *__restri
             for (int f = 0; f < 8; ++f) k[f] = kif[(long long)f * g.P + ix];
             interface_bc<DIR>(k, c.nif[ix]);
 #pragma unroll
-            for (int f = 0; f < 8; ++f) v[f] = A[iface_index(g, f, ix)] + cb * k[f];
+            for (int f = 0; f < 8; ++f) {
+                const long long i = iface_index(g, f, ix);
+                const float u = kif[(long long)(8 + f) * g.P + ix];   // U^n saved in stage 1
+                v[f] = u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (T4[i] - u)) * (1.0f / 3.0f) + cb * k[f];
+            }
         } else {
 #pragma unroll
             for (int f = 0; f < 8; ++f) v[f] = 0.0f;
