@@ -156,6 +156,35 @@ int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
  * state (3 floats per field) plus the coefficients once, per updated cell. */
 int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
 
+/* Tile-shape autotuner: times nsteps steps for each tile height, keeps the
+ * fastest and resets the shot (no reference counterpart). */
+int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
+
+/* --- Batched shots ------------------------------------------------------- */
+/* Independent shots, each with its own model (grid size, medium, sources) and
+ * source index, stepped together: one launch per RK stage covers the tiles of
+ * every shot, and the time loop is replayed from a hipGraph.  All models must
+ * share dt.  Replaces running src/main.cu's shot loop (src/main.cu:94-213)
+ * one shot at a time.  Results per shot are bit-identical to msfdm_run. */
+typedef struct msfdm_batch msfdm_batch;
+int msfdm_batch_create(const msfdm_model *const *models, const int *shot_index, int nshots, int device,
+                       msfdm_batch **out);
+void msfdm_batch_destroy(msfdm_batch *b);
+/* Zero every shot's wavefield and records. */
+int msfdm_batch_reset(msfdm_batch *b);
+/* Advance every shot nsteps steps (asynchronous). */
+int msfdm_batch_run(msfdm_batch *b, int nsteps);
+int msfdm_batch_synchronize(msfdm_batch *b);
+int msfdm_batch_size(const msfdm_batch *b, int *nshots);
+/* Solver handle of one shot (for records / fields); waits for queued work.
+ * The handle stays owned by the batch. */
+int msfdm_batch_shot(msfdm_batch *b, int shot, msfdm_solver **out);
+int msfdm_batch_set_tile_rows(msfdm_batch *b, int rows);
+int msfdm_batch_get_tile_rows(const msfdm_batch *b, int *rows);
+int msfdm_batch_set_graph(msfdm_batch *b, int enable);
+int msfdm_batch_time_run(msfdm_batch *b, int nsteps, float *ms);
+int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows);
+
 /* --- Process boundary ---------------------------------------------------- */
 
 /* The whole bin/main_debug run: argument guards, output-dir reset, shot loop,

@@ -650,3 +650,320 @@ extern "C" int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes) {
     *bytes = s->bytes_step;
     return MSFDM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched shots: independent shots (own grid, model, source and buffers) that
+// share dt are stepped together, one launch per RK stage for all of them, the
+// time loop replayed from a hipGraph.  Each shot is a full msfdm_solver whose
+// device state the batch drives; records and fields are read per shot.
+// ---------------------------------------------------------------------------
+struct msfdm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<msfdm_solver *> shots;
+    std::vector<D::Shot> host_shots;
+    D::Shot *d_shots = nullptr;
+    int4 *d_tiles[2] = {nullptr, nullptr};   // tile tables for 8- and 16-row tiles
+    int ntiles[2] = {0, 0};
+    int tz = 16;
+    int maxnx = 0;
+    int *base = nullptr;
+    int host_it = 0;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    float dt = 0.f;
+};
+
+namespace {
+
+void batch_drop_graph(msfdm_batch *b) {
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
+// Refresh the device shot table (table pointers change when a solver grows its
+// amplitude / record tables).
+int batch_upload(msfdm_batch *b) {
+    std::vector<D::Shot> h(b->shots.size());
+    for (size_t i = 0; i < b->shots.size(); ++i) {
+        msfdm_solver *s = b->shots[i];
+        D::Shot &t = h[i];
+        t.g = s->geo;
+        t.c = s->coef;
+        t.ctl = s->ctl;
+        t.ctl.base = b->base;
+        for (int r = 0; r < 4; ++r) t.buf[r] = s->buf[r];
+        t.kif = s->kif;
+    }
+    if (h.size() == b->host_shots.size() && std::memcmp(h.data(), b->host_shots.data(), h.size() * sizeof(D::Shot)) == 0)
+        return MSFDM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->host_shots = h;
+    HIPCHK(hipMemcpy(b->d_shots, b->host_shots.data(), h.size() * sizeof(D::Shot), hipMemcpyHostToDevice));
+    batch_drop_graph(b);
+    return MSFDM_OK;
+}
+
+template <int DIR, int STAGE, int TZ>
+void batch_launch_media(msfdm_batch *b, int rS, int rO, int j, float ct) {
+    const int k = TZ == 8 ? 0 : 1;
+    const int grid = 8 * ((b->ntiles[k] + 7) / 8);
+    D::batch_stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, b->stream>>>(b->d_shots, b->d_tiles[k], b->ntiles[k], rS,
+                                                                            rO, j, ct);
+}
+template <int DIR, int STAGE>
+void batch_launch_stage(msfdm_batch *b, int rS, int rO, int j, float ct) {
+    if (b->tz == 8) batch_launch_media<DIR, STAGE, 8>(b, rS, rO, j, ct);
+    else batch_launch_media<DIR, STAGE, 16>(b, rS, rO, j, ct);
+    const int ns = (int)b->shots.size();
+    if constexpr (STAGE == 4) {
+        D::batch_iface_final<DIR><<<dim3((b->maxnx + D::FB - 1) / D::FB, ns), D::FB, 0, b->stream>>>(b->d_shots, ct, j);
+    } else {
+        D::batch_iface_stage<DIR, STAGE><<<dim3((b->maxnx + 127) / 128, ns), 128, 0, b->stream>>>(b->d_shots, rS, rO, ct);
+    }
+}
+template <int D1>
+void batch_step_dir(msfdm_batch *b, int j) {
+    constexpr int D2 = 1 - D1;
+    const float dt = b->dt;
+    batch_launch_stage<D1, 1>(b, R_U, R_T2, j, dt / 2);
+    batch_launch_stage<D2, 2>(b, R_T2, R_T3, j, dt / 2);
+    batch_launch_stage<D1, 3>(b, R_T3, R_T4, j, dt);
+    batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6);
+}
+void batch_enqueue(msfdm_batch *b, int it0, int n) {
+    for (int j = 0; j < n; ++j) {
+        if ((it0 + j) & 1) batch_step_dir<1>(b, j);
+        else batch_step_dir<0>(b, j);
+    }
+    D::advance_base<<<1, 64, 0, b->stream>>>(b->base, n);
+}
+
+void batch_free(msfdm_batch *b) {
+    batch_drop_graph(b);
+    for (msfdm_solver *s : b->shots) msfdm_solver_destroy(s);
+    for (void *p : {(void *)b->d_shots, (void *)b->d_tiles[0], (void *)b->d_tiles[1], (void *)b->base})
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+}  // namespace
+
+extern "C" int msfdm_batch_create(const msfdm_model *const *models, const int *shot_index, int nshots, int device,
+                                  msfdm_batch **out) {
+    clear_error();
+    if (!models || !out || nshots <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    *out = nullptr;
+    msfdm_batch *b = new msfdm_batch();
+    b->device = device;
+    auto fail = [&](int rc) { std::string msg = msfdm_last_error(); batch_free(b); delete b; return set_error(rc, msg); };
+    for (int i = 0; i < nshots; ++i) {
+        if (!models[i]) return fail(set_error(MSFDM_EINVAL, "null model " + std::to_string(i)));
+        if (i > 0 && models[i]->d.dt != models[0]->d.dt)
+            return fail(set_error(MSFDM_EINVAL, "all shots of a batch must share dt"));
+        msfdm_solver *s = nullptr;
+        if (int rc = msfdm_solver_create(models[i], device, &s)) return fail(rc);
+        b->shots.push_back(s);
+        const int isrc = shot_index ? shot_index[i] : 0;
+        if (int rc = msfdm_reset_shot(s, isrc)) return fail(rc);
+        s->use_graph = false;
+        b->maxnx = std::max(b->maxnx, s->d.nx);
+    }
+    b->dt = models[0]->d.dt;
+#define BCK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return fail(set_error(MSFDM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); } while (0)
+    BCK(hipSetDevice(device));
+    BCK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BCK(hipMalloc(&b->base, 64 * sizeof(int)));
+    BCK(hipMemset(b->base, 0, 64 * sizeof(int)));
+    BCK(hipMalloc(&b->d_shots, nshots * sizeof(D::Shot)));
+    for (int k = 0; k < 2; ++k) {
+        const int tz = k == 0 ? 8 : 16;
+        std::vector<int4> tiles;
+        for (int i = 0; i < nshots; ++i) {
+            const msfdm_solver *s = b->shots[i];
+            const int nbx = (s->d.nx - 4) / D::TX + 1;
+            const int nbf = (s->d.nz_fluid - 3 + tz - 1) / tz, nbs = (s->d.nz_solid - 3 + tz - 1) / tz;
+            for (int band = 0; band < nbf + nbs; ++band)
+                for (int bx = 0; bx < nbx; ++bx) tiles.push_back(make_int4(i, band, bx, nbf));
+        }
+        b->ntiles[k] = (int)tiles.size();
+        BCK(hipMalloc(&b->d_tiles[k], tiles.size() * sizeof(int4)));
+        BCK(hipMemcpy(b->d_tiles[k], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+#undef BCK
+    long long cells = 0;
+    for (auto *s : b->shots) cells += (long long)s->d.nx * (s->d.nz_fluid + s->d.nz_solid - 1);
+    b->tz = cells < 1000000 ? 8 : 16;
+    if (int rc = batch_upload(b)) return fail(rc);
+    *out = b;
+    return MSFDM_OK;
+}
+
+extern "C" void msfdm_batch_destroy(msfdm_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
+}
+
+extern "C" int msfdm_batch_reset(msfdm_batch *b) {
+    clear_error();
+    if (!b) return set_error(MSFDM_EINVAL, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (msfdm_solver *s : b->shots)
+        if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
+    HIPCHK(hipMemset(b->base, 0, sizeof(int)));
+    b->host_it = 0;
+    return batch_upload(b);
+}
+
+extern "C" int msfdm_batch_run(msfdm_batch *b, int nsteps) {
+    clear_error();
+    if (!b) return set_error(MSFDM_EINVAL, "null batch");
+    if (nsteps < 0) return set_error(MSFDM_EINVAL, "negative step count");
+    HIPCHK(hipSetDevice(b->device));
+    bool grow = false;
+    for (msfdm_solver *s : b->shots)
+        grow |= (b->host_it + nsteps > s->amp_cap) || ((b->host_it + nsteps - 1) / s->gap + 1 > s->rec_cap);
+    if (grow) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (msfdm_solver *s : b->shots)
+            if (int rc = grow_tables(s, b->host_it + nsteps)) return rc;
+    }
+    if (int rc = batch_upload(b)) return rc;
+    int left = nsteps;
+    while (left > 0) {
+        if (b->use_graph && left >= GRAPH_STEPS && (b->host_it & 1) == 0) {
+            if (!b->gexec) {
+                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                batch_enqueue(b, 0, GRAPH_STEPS);
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamEndCapture(b->stream, &g));
+                b->graph = g;
+                HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+            b->host_it += GRAPH_STEPS;
+            left -= GRAPH_STEPS;
+        } else {
+            const int n = (b->use_graph && (b->host_it & 1)) ? 1 : std::min(left, GRAPH_STEPS);
+            batch_enqueue(b, b->host_it, n);
+            HIPCHK(hipGetLastError());
+            b->host_it += n;
+            left -= n;
+        }
+    }
+    for (msfdm_solver *s : b->shots) {
+        s->host_it = b->host_it;
+        s->nrec_done = b->host_it > 0 ? (b->host_it - 1) / s->gap + 1 : 0;
+    }
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_synchronize(msfdm_batch *b) {
+    if (!b) return set_error(MSFDM_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_size(const msfdm_batch *b, int *nshots) {
+    if (!b || !nshots) return set_error(MSFDM_EINVAL, "null argument");
+    *nshots = (int)b->shots.size();
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_shot(msfdm_batch *b, int shot, msfdm_solver **out) {
+    if (!b || !out) return set_error(MSFDM_EINVAL, "null argument");
+    if (shot < 0 || shot >= (int)b->shots.size()) return set_error(MSFDM_EINVAL, "shot out of range");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    *out = b->shots[shot];
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_set_tile_rows(msfdm_batch *b, int rows) {
+    if (!b) return set_error(MSFDM_EINVAL, "null batch");
+    if (rows != 8 && rows != 16) return set_error(MSFDM_EINVAL, "tile rows must be 8 or 16");
+    if (rows != b->tz) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        batch_drop_graph(b);
+        b->tz = rows;
+    }
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_get_tile_rows(const msfdm_batch *b, int *rows) {
+    if (!b || !rows) return set_error(MSFDM_EINVAL, "null argument");
+    *rows = b->tz;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_set_graph(msfdm_batch *b, int enable) {
+    if (!b) return set_error(MSFDM_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->use_graph = enable != 0;
+    if (!b->use_graph) batch_drop_graph(b);
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_time_run(msfdm_batch *b, int nsteps, float *ms) {
+    clear_error();
+    if (!b || !ms) return set_error(MSFDM_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    if (int rc = msfdm_batch_run(b, 0)) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, b->stream));
+    if (int rc = msfdm_batch_run(b, nsteps)) return rc;
+    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t;
+    return MSFDM_OK;
+}
+
+// Tile-shape autotuner: time `nsteps` steps (after a graph-building warm-up)
+// for each tile height, keep the fastest, reset the state.
+extern "C" int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows) {
+    clear_error();
+    if (!b || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    float best = 1e30f;
+    int arg = b->tz;
+    for (int rows : {8, 16}) {
+        if (int rc = msfdm_batch_set_tile_rows(b, rows)) return rc;
+        if (int rc = msfdm_batch_reset(b)) return rc;
+        if (int rc = msfdm_batch_run(b, 2 * GRAPH_STEPS)) return rc;
+        float ms = 0.f;
+        if (int rc = msfdm_batch_time_run(b, nsteps, &ms)) return rc;
+        if (ms < best) { best = ms; arg = rows; }
+    }
+    if (int rc = msfdm_batch_set_tile_rows(b, arg)) return rc;
+    if (int rc = msfdm_batch_reset(b)) return rc;
+    if (best_rows) *best_rows = arg;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows) {
+    clear_error();
+    if (!s || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    float best = 1e30f;
+    int arg = s->tz;
+    for (int rows : {8, 16}) {
+        if (int rc = msfdm_set_tile_rows(s, rows)) return rc;
+        if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
+        if (int rc = msfdm_run(s, 2 * GRAPH_STEPS)) return rc;
+        float ms = 0.f;
+        if (int rc = msfdm_time_run(s, nsteps, &ms)) return rc;
+        if (ms < best) { best = ms; arg = rows; }
+    }
+    if (int rc = msfdm_set_tile_rows(s, arg)) return rc;
+    if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
+    if (best_rows) *best_rows = arg;
+    return MSFDM_OK;
+}

@@ -539,14 +539,13 @@ __device__ __forceinline__ long long iface_index(const Geo &g, int f, int ix) {
     return f < 3 ? g.off[MSFDM_FVX + f] + (long long)(g.nzf - 1) * g.P + ix : g.off[MSFDM_SVX + f - 3] + ix;
 }
 
-// Interface rows of stages 1-3 (and of the raw-k mode, STAGE 0).  kif holds
-// [8][P] interface k values followed by [8][P] copies of U^n on the interface
-// rows, saved in stage 1 for the stage-4 kernel (which overwrites U there).
+// Interface rows of stages 1-3 (and of the raw-k mode, STAGE 0) at column ix.
+// kif holds [8][P] interface k values followed by [8][P] copies of U^n on the
+// interface rows, saved in stage 1 for the stage-4 kernel (which overwrites U there).
 template <int DIR, int STAGE>
-__global__ void __launch_bounds__(128)
-iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U,
-            float *__restrict__ O, float *__restrict__ kif, float ct) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void iface_stage_body(const Geo &g, const Coef &c, const float *__restrict__ S,
+                                                 const float *__restrict__ U, float *__restrict__ O,
+                                                 float *__restrict__ kif, float ct, int ix) {
     if (ix < 3 || ix > g.nx - 4) return;
     float k[8];
 #pragma unroll
@@ -561,15 +560,23 @@ iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict_
     }
 }
 
-// Stage-4 interface rows + fl_so_filter (out of place) + receiver records.
+template <int DIR, int STAGE>
+__global__ void __launch_bounds__(128)
+iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U,
+            float *__restrict__ O, float *__restrict__ kif, float ct) {
+    iface_stage_body<DIR, STAGE>(g, c, S, U, O, kif, ct, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Stage-4 interface rows + fl_so_filter (out of place) + receiver records for
+// the FB columns starting at x0.
 constexpr int FB = 128;
 template <int DIR>
-__global__ void __launch_bounds__(FB)
-iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ T4, const float *__restrict__ T2,
-            const float *__restrict__ T3, float *__restrict__ Uo, const float *__restrict__ kif, float cb,
-            int jstep) {
-    __shared__ float pre[8][FB + 2 * HALO];
-    const int nx = g.nx, x0 = blockIdx.x * FB;
+__device__ __forceinline__ void iface_final_body(float (*pre)[FB + 2 * HALO], const Geo &g, const Coef &c,
+                                                 const Ctl &ctl, const float *__restrict__ T4,
+                                                 const float *__restrict__ T2, const float *__restrict__ T3,
+                                                 float *__restrict__ Uo, const float *__restrict__ kif, float cb,
+                                                 int jstep, int x0) {
+    const int nx = g.nx;
     for (int t = threadIdx.x; t < FB + 2 * HALO; t += FB) {
         const int ix = x0 - HALO + t;
         float v[8];
@@ -634,6 +641,66 @@ iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ T4, const float *_
             rr[2 * plane] = o[2];
         }
     }
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(FB)
+iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ T4, const float *__restrict__ T2,
+            const float *__restrict__ T3, float *__restrict__ Uo, const float *__restrict__ kif, float cb,
+            int jstep) {
+    __shared__ float pre[8][FB + 2 * HALO];
+    iface_final_body<DIR>(pre, g, c, ctl, T4, T2, T3, Uo, kif, cb, jstep, blockIdx.x * FB);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: many shots (each with its own grid, model and buffers)
+// stepped together, one launch per RK stage for all of them.
+// ---------------------------------------------------------------------------
+struct Shot {
+    Geo g;
+    Coef c;
+    Ctl ctl;
+    float *buf[4];    // U, T2, T3, T4
+    float *kif;
+};
+
+// tiles[t] = (shot, band, bx, fluid bands of that shot)
+template <int DIR, int STAGE, int TZ>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
+batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
+                   int jstep, float ct) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    const int per_xcd = gridDim.x >> 3;
+    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    const int4 tl = tiles[t];
+    const Shot &s = shots[tl.x];
+    const float *S = s.buf[rS];
+    float *O = s.buf[rO];
+    if (tl.y < tl.w)
+        fluid_body<DIR, STAGE, TZ>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
+                                   tl.z, tl.y);
+    else
+        solid_body<DIR, STAGE, TZ>(sh, s.g, s.c, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, ct, tl.z, tl.y - tl.w);
+}
+
+// grid (column blocks of the widest shot, shots)
+template <int DIR, int STAGE>
+__global__ void __launch_bounds__(128)
+batch_iface_stage(const Shot *__restrict__ shots, int rS, int rO, float ct) {
+    const Shot &s = shots[blockIdx.y];
+    iface_stage_body<DIR, STAGE>(s.g, s.c, s.buf[rS], s.buf[0], s.buf[rO], s.kif, ct,
+                                 blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(FB)
+batch_iface_final(const Shot *__restrict__ shots, float cb, int jstep) {
+    __shared__ float pre[8][FB + 2 * HALO];
+    const Shot &s = shots[blockIdx.y];
+    if ((int)blockIdx.x * FB > s.g.nx) return;   // whole block beyond this shot's grid
+    iface_final_body<DIR>(pre, s.g, s.c, s.ctl, s.buf[3], s.buf[1], s.buf[2], s.buf[0], s.kif, cb, jstep,
+                          blockIdx.x * FB);
 }
 
 __global__ void advance_base(int *base, int n) {

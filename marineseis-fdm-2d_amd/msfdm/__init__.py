@@ -3,8 +3,8 @@
 ``msfdm.Solver`` drives libmsfdm.so (hand-written HIP kernels for gfx950) through
 its C ABI; ``msfdm.modelgen`` builds and writes solver input models.
 """
-from ._lib import (ARRAYS, BIN_PATH, FID, FIELDS, LIB_PATH, NFIELD, HostModel, MsfdmError,
+from ._lib import (ARRAYS, BIN_PATH, FID, FIELDS, LIB_PATH, NFIELD, Batch, HostModel, MsfdmError,
                    Solver, last_error, lib, run_cli)
 
-__all__ = ["ARRAYS", "BIN_PATH", "FID", "FIELDS", "LIB_PATH", "NFIELD", "HostModel", "MsfdmError",
+__all__ = ["ARRAYS", "BIN_PATH", "FID", "FIELDS", "LIB_PATH", "NFIELD", "Batch", "HostModel", "MsfdmError",
            "Solver", "last_error", "lib", "run_cli"]

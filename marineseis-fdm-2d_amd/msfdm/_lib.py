@@ -33,7 +33,10 @@ EXPORTS = [
     "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
     "msfdm_set_tile_rows", "msfdm_get_tile_rows",
     "msfdm_time_kernel", "msfdm_time_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
-    "msfdm_run_cli",
+    "msfdm_run_cli", "msfdm_autotune", "msfdm_batch_create", "msfdm_batch_destroy",
+    "msfdm_batch_reset", "msfdm_batch_run", "msfdm_batch_synchronize", "msfdm_batch_size",
+    "msfdm_batch_shot", "msfdm_batch_set_tile_rows", "msfdm_batch_get_tile_rows",
+    "msfdm_batch_set_graph", "msfdm_batch_time_run", "msfdm_batch_autotune",
 ]
 
 
@@ -92,6 +95,19 @@ def lib():
         "msfdm_bytes_per_step": ([vp, C.POINTER(C.c_double)], i),
         "msfdm_kernel_bytes": ([vp, i, C.POINTER(C.c_double)], i),
         "msfdm_run_cli": ([C.c_char_p, C.c_char_p], i),
+        "msfdm_autotune": ([vp, i, ip], i),
+        "msfdm_batch_create": ([C.POINTER(vp), ip, i, i, C.POINTER(vp)], i),
+        "msfdm_batch_destroy": ([vp], None),
+        "msfdm_batch_reset": ([vp], i),
+        "msfdm_batch_run": ([vp, i], i),
+        "msfdm_batch_synchronize": ([vp], i),
+        "msfdm_batch_size": ([vp, ip], i),
+        "msfdm_batch_shot": ([vp, i, C.POINTER(vp)], i),
+        "msfdm_batch_set_tile_rows": ([vp, i], i),
+        "msfdm_batch_get_tile_rows": ([vp, ip], i),
+        "msfdm_batch_set_graph": ([vp, i], i),
+        "msfdm_batch_time_run": ([vp, i, fp], i),
+        "msfdm_batch_autotune": ([vp, i, ip], i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
@@ -171,19 +187,28 @@ class HostModel:
 class Solver:
     """Device solver over a HostModel (or a modelgen.Model)."""
 
-    def __init__(self, model, device: int = 0):
+    def __init__(self, model, device: int = 0, _handle=None, _owner=None):
         self._hm = model if isinstance(model, HostModel) else HostModel.from_model(model)
         d = self._hm.desc()
         self.nx, self.nzf, self.nzs = d.nx, d.nz_fluid, d.nz_solid
         self.nt, self.dt = d.nt, d.dt
+        self._owner = _owner      # a Batch owns the handles of its shots
+        if _handle is not None:
+            self.h = _handle
+            return
         h = C.c_void_p()
         check(lib().msfdm_solver_create(self._hm.h, int(device), C.byref(h)))
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self._owner is None:
             lib().msfdm_solver_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def autotune(self, nsteps: int = 64) -> int:
+        v = C.c_int()
+        check(lib().msfdm_autotune(self.h, int(nsteps), C.byref(v)))
+        return v.value
 
     def __del__(self):
         try:
@@ -272,6 +297,73 @@ class Solver:
     def bytes_per_step(self) -> float:
         v = C.c_double()
         check(lib().msfdm_bytes_per_step(self.h, C.byref(v)))
+        return v.value
+
+
+class Batch:
+    """Shots of (possibly) different models stepped together on one device."""
+
+    def __init__(self, models, shot_index=None, device: int = 0):
+        self._hms = [m if isinstance(m, HostModel) else HostModel.from_model(m) for m in models]
+        n = len(self._hms)
+        arr = (C.c_void_p * n)(*[hm.h.value if isinstance(hm.h, C.c_void_p) else hm.h for hm in self._hms])
+        idx = (C.c_int * n)(*(shot_index if shot_index is not None else [0] * n))
+        h = C.c_void_p()
+        check(lib().msfdm_batch_create(arr, idx, n, int(device), C.byref(h)))
+        self.h = h
+        self.shots = []
+        for i in range(n):
+            sh = C.c_void_p()
+            check(lib().msfdm_batch_shot(self.h, i, C.byref(sh)))
+            self.shots.append(Solver(self._hms[i], _handle=sh, _owner=self))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for s in self.shots:
+                s.h = None
+            lib().msfdm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        check(lib().msfdm_batch_reset(self.h))
+
+    def run(self, nsteps: int):
+        check(lib().msfdm_batch_run(self.h, int(nsteps)))
+
+    def sync(self):
+        check(lib().msfdm_batch_synchronize(self.h))
+
+    def shot(self, i: int) -> Solver:
+        self.sync()
+        return self.shots[i]
+
+    @property
+    def tile_rows(self) -> int:
+        v = C.c_int()
+        check(lib().msfdm_batch_get_tile_rows(self.h, C.byref(v)))
+        return v.value
+
+    @tile_rows.setter
+    def tile_rows(self, rows: int):
+        check(lib().msfdm_batch_set_tile_rows(self.h, int(rows)))
+
+    def set_graph(self, enable: bool):
+        check(lib().msfdm_batch_set_graph(self.h, int(bool(enable))))
+
+    def time_run(self, nsteps: int) -> float:
+        v = C.c_float()
+        check(lib().msfdm_batch_time_run(self.h, int(nsteps), C.byref(v)))
+        return v.value
+
+    def autotune(self, nsteps: int = 64) -> int:
+        v = C.c_int()
+        check(lib().msfdm_batch_autotune(self.h, int(nsteps), C.byref(v)))
         return v.value
 
 

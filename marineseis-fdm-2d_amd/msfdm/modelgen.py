@@ -290,3 +290,24 @@ def read_model(json_path: str, cwd: str | None = None) -> Model:
                  beta0=p["beta0"], src_x=[int(v) for v in g["src_pos"]["posx"]],
                  src_z=[int(v) for v in g["src_pos"]["posz"]], arrays=arrays,
                  calc_type=int(g["calc_type"]), boundary_type=int(g["boundary_type"]))
+
+
+def batch_shots(n: int = 64, seed: int = 64, nx_range=(800, 2000), nz_range=(400, 1000), nt: int = 4000,
+                **over) -> list:
+    """Shot windows of the flat 5 m water-over-solid medium (200 m of water,
+    vp/vs/rho 3000/1700/2200, 15 Hz, dt 0.5 ms, 20-cell PML) with grid sizes drawn
+    uniformly from nx_range x nz_range and the source column drawn uniformly over
+    each window (10 m above the seafloor), all from one splitmix64 stream."""
+    u = (splitmix64(np.arange(3 * n, dtype=np.uint64), seed) >> np.uint64(11)).astype(np.float64) / (1 << 53)
+    shots = []
+    npd = over.get("npd", 20)
+    for i in range(n):
+        nx = nx_range[0] + int(u[3 * i] * (nx_range[1] - nx_range[0] + 1))
+        nz = nz_range[0] + int(u[3 * i + 1] * (nz_range[1] - nz_range[0] + 1))
+        lo, hi = npd + 8, nx - npd - 9
+        sx = lo + int(u[3 * i + 2] * (hi - lo + 1))
+        kw = dict(nx=nx, nz=nz, h=5.0, water_depth=200.0, npd=npd, fpeak=15.0, dt=5e-4, nt=nt,
+                  src_x=[sx], src_height=10.0, solid=(3000.0, 1700.0, 2200.0), vp_max=3000.0)
+        kw.update(over)
+        shots.append(build_model(**kw))
+    return shots
