@@ -46,6 +46,49 @@ def cpu_baseline(model, nsteps: int):
                       f"(after 2 untimed steps), {dt:.2f} s wall"}
 
 
+def bench_batch(args):
+    """64 shot windows of uneven size: one batched run vs the same shots one at a
+    time with the single-shot solver, both on `args.batch_steps` steps per shot
+    (shots/s scaled to the full 4000-step record)."""
+    import msfdm
+    from msfdm import modelgen
+    models = modelgen.batch_shots(n=args.batch_shots, nt=4000)
+    cells = sum(m.nx * m.nz for m in models)
+    nsteps = args.batch_steps
+    b = msfdm.Batch(models)
+    best = b.autotune(64)
+    b.reset()
+    b.run(64)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(nsteps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    t_seq = 0.0
+    for m in models:
+        s = msfdm.Solver(m)
+        s.tile_rows = best
+        s.reset(0)
+        s.run(64)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(nsteps)
+        s.sync()
+        t_seq += time.perf_counter() - t0
+        s.close()
+    scale = 4000.0 / nsteps
+    out = {"metric": "batched forward shots/s (64 uneven shot windows, 4000-step records)",
+           "value": round(len(models) / (t_batch * scale), 4), "unit": "shots/s",
+           "sequential_value": round(len(models) / (t_seq * scale), 4),
+           "speedup_vs_sequential": round(t_seq / t_batch, 3),
+           "batched_Gcell_per_s": round(cells * nsteps / t_batch / 1e9, 3),
+           "sequential_Gcell_per_s": round(cells * nsteps / t_seq / 1e9, 3),
+           "tile_rows": best, "steps_timed": nsteps, "shots": len(models), "total_cells": cells,
+           "note": f"timed {nsteps} of 4000 steps per shot after 64 warm-up steps; shots/s scaled x{scale:g}"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +97,9 @@ def main():
     ap.add_argument("--workload", default="seafloor2001")
     ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--kernel-reps", type=int, default=100)
+    ap.add_argument("--mode", default="solver", choices=["solver", "batch"])
+    ap.add_argument("--batch-shots", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5) to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -66,6 +112,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}))
+        return
+    if args.mode == "batch":
+        bench_batch(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
