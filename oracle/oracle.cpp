@@ -115,6 +115,10 @@ struct Oracle {
     // records
     std::vector<double> rec_vx, rec_vz, rec_p;
     int nrec = 0;
+    // adjoint: input / output adjoint fields, model gradients (K, rho_f, C11,
+    // C13, C33, C55, rho_s)
+    std::vector<double> Ain[NFIELD], Aout[NFIELD];
+    std::vector<double> G[7];
 
     size_t fsize(int f) const { return (size_t)(is_fluid(f) ? nzf : nzs) * nx; }
     // metric index of solid row iz (src/tools_func.cu:22, 57)
@@ -555,6 +559,471 @@ void step(Oracle &o) {
     o.it++;
 }
 
+
+// ===========================================================================
+// Discrete adjoint (transpose of the scheme above, race-free semantics).
+// Scatter form: every forward expression k[i] = sum_j a_ij S[j] becomes
+// Sbar[j] += a_ij * qbar[i], branch by branch.  Model gradients are the
+// pointwise terms dk/dm . qbar evaluated on the forward stage state.
+// ===========================================================================
+enum Grad { G_K = 0, G_RHOF, G_C11, G_C13, G_C33, G_C55, G_RHOS };
+
+// Adjoint of the x / z DRP line operators: f at offset m gets w_m * g.
+template <class Add> inline void drp_adj(int dir, double g, Add add) {
+    if (dir == F) { add(-1, DRP_M1 * g); add(0, DRP_0 * g); add(1, DRP_1 * g); add(2, DRP_2 * g); add(3, DRP_3 * g); }
+    else { add(1, -DRP_M1 * g); add(0, -DRP_0 * g); add(-1, -DRP_1 * g); add(-2, -DRP_2 * g); add(-3, -DRP_3 * g); }
+}
+template <class Add> inline void mac_dze_adj(int dir, int dist, bool solid_side, double g, Add add) {
+    if (dist >= 3) { drp_adj(dir, g, add); return; }
+    if (dist == 2) {
+        const double a0 = -7.0 / 6.0, a1 = 8.0 / 6.0, a2 = -1.0 / 6.0;
+        if (dir == F) { add(0, a0 * g); add(1, a1 * g); add(2, a2 * g); }
+        else { add(0, -a0 * g); add(-1, -a1 * g); add(-2, -a2 * g); }
+        return;
+    }
+    int d = dir;
+    if (dist == 0) d = solid_side ? F : B;
+    if (d == F) { add(0, -g); add(1, g); }
+    else { add(0, g); add(-1, -g); }
+}
+
+// Transpose of update_velocity_fluid (include/pml_fs_kernels.cuh:333-450);
+// with S the forward stage state, also dk/d(rho_f) . qbar.
+void velocity_fluid_adj(Oracle &o, const double *const *S, const double *const *Q, double *const *Sb, int dir) {
+    const int nx = o.nx, nzf = o.nzf;
+    for (int iz = 3; iz < nzf; ++iz) {
+        for (int ix = 3; ix <= nx - 4; ++ix) {
+            const size_t idx = (size_t)iz * nx + ix, g = o.gf(iz, ix);
+            const double J = o.J[g], xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+            const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
+            const double gxixi = xx * xx + xz * xz, rho = o.rhof[idx];
+            const double qvx = Q[FVX][idx], qvz = Q[FVZ][idx];
+            const bool side = in_side(o, ix);
+            double Dxb = 0, Dzb = 0;
+            double rvxb = qvx / rho, rvzb = qvz / rho;
+            double Dx = 0, Dz = 0, rvx = 0, rvz = 0;   // forward values for the gradient
+            double qVx = 0, qVz = 0, invb = 1, dpb = 0, apdp = 0;
+            if (side) {
+                const double be = o.be_xi[ix];
+                dpb = o.dp_xi[ix] / be; apdp = o.al_xi[ix] + dpb; invb = 1.0 / be;
+                qVx = Q[FXVX][idx]; qVz = Q[FXVZ][idx];
+                Dxb = (invb - 1.0) / rho * qvx + dpb * qVx;
+                Dzb = (invb - 1.0) / rho * qvz + dpb * qVz;
+                Sb[FXVX][idx] += -invb / rho * qvx - apdp * qVx;
+                Sb[FXVZ][idx] += -invb / rho * qvz - apdp * qVz;
+            }
+            if (dist_flso < 3 || dist_fs < 3) {
+                const bool at_if = dist_flso < 3;
+                // forward (for the gradient)
+                auto fze_fwd = [&](int r, double &fx, double &fz) {
+                    if (at_if) {
+                        if (r <= nzf - 1) fze_f_row(o, S[FP], r, ix, fx, fz);
+                        else { double ax, az, bx, bz; fze_f_row(o, S[FP], nzf - 1, ix, ax, az); fze_f_row(o, S[FP], 2 * (nzf - 1) - r, ix, bx, bz); fx = 2 * ax - bx; fz = 2 * az - bz; }
+                    } else {
+                        if (r >= 3) fze_f_row(o, S[FP], r, ix, fx, fz);
+                        else { double bx, bz; fze_f_row(o, S[FP], 6 - r, ix, bx, bz); fx = -bx; fz = -bz; }
+                    }
+                };
+                const double sx = drp(dir, [&](int m) { double a, b; fxi_f(o, S[FP], iz, ix + m, a, b); return a; }) +
+                                  drp(dir, [&](int m) { double a, b; fze_fwd(iz + m, a, b); return a; });
+                const double sz = drp(dir, [&](int m) { double a, b; fxi_f(o, S[FP], iz, ix + m, a, b); return b; }) +
+                                  drp(dir, [&](int m) { double a, b; fze_fwd(iz + m, a, b); return b; });
+                rvx = sx / J; rvz = sz / J;
+                const double proj = (xx * sx + xz * sz) / (gxixi * J);
+                Dx = xx * proj; Dz = xz * proj;
+                // reverse
+                const double projb = xx * Dxb + xz * Dzb;
+                const double sxb = rvxb / J + xx * projb / (gxixi * J);
+                const double szb = rvzb / J + xz * projb / (gxixi * J);
+                // xi fluxes at (iz, ix+m): -J p (xi_x, xi_z)
+                drp_adj(dir, 1.0, [&](int m, double w) {
+                    const size_t gq = o.gf(iz, ix + m);
+                    Sb[FP][(size_t)iz * nx + ix + m] += -o.J[gq] * (o.xix[gq] * w * sxb + o.xiz[gq] * w * szb);
+                });
+                auto fze_add = [&](int r, double bx, double bz) {   // bx, bz: adjoint of the flux at real row r
+                    const size_t gq = o.gf(r, ix);
+                    Sb[FP][(size_t)r * nx + ix] += -o.J[gq] * (o.zex[gq] * bx + o.zez[gq] * bz);
+                };
+                drp_adj(dir, 1.0, [&](int m, double w) {
+                    const int r = iz + m;
+                    const double bx = w * sxb, bz = w * szb;
+                    if (at_if) {
+                        if (r <= nzf - 1) fze_add(r, bx, bz);
+                        else { fze_add(nzf - 1, 2 * bx, 2 * bz); fze_add(2 * (nzf - 1) - r, -bx, -bz); }
+                    } else {
+                        if (r >= 3) fze_add(r, bx, bz);
+                        else fze_add(6 - r, -bx, -bz);
+                    }
+                });
+            } else {
+                const double *p = S[FP];
+                const double dpx = drp(dir, [&](int m) { return p[idx + m]; });
+                const double dpz = drp(dir, [&](int m) { return p[idx + (std::ptrdiff_t)m * nx]; });
+                const double gxize = xx * zx + xz * zz;
+                rvx = -(xx * dpx + zx * dpz); rvz = -(xz * dpx + zz * dpz);
+                const double t = (gxixi * dpx + gxize * dpz) / gxixi;
+                Dx = -xx * t; Dz = -xz * t;
+                const double tb = -xx * Dxb - xz * Dzb;
+                const double dpxb = -xx * rvxb - xz * rvzb + tb;
+                const double dpzb = -zx * rvxb - zz * rvzb + tb * gxize / gxixi;
+                drp_adj(dir, dpxb, [&](int m, double v) { Sb[FP][idx + m] += v; });
+                drp_adj(dir, dpzb, [&](int m, double v) { Sb[FP][idx + (std::ptrdiff_t)m * nx] += v; });
+            }
+            // gradient wrt rho_f: k_v = X / rho  =>  dk/drho = -k_v / rho
+            double kvx = rvx, kvz = rvz;
+            if (side) {
+                kvx += (invb - 1.0) * Dx - invb * S[FXVX][idx];
+                kvz += (invb - 1.0) * Dz - invb * S[FXVZ][idx];
+            }
+            o.G[G_RHOF][idx] += -(qvx * kvx + qvz * kvz) / (rho * rho);
+        }
+    }
+}
+
+// Transpose of update_stress_fluid (include/pml_fs_kernels.cuh:452-507) + dk/dK.
+void stress_fluid_adj(Oracle &o, const double *const *S, const double *const *Q, double *const *Sb, int dir) {
+    const int nx = o.nx, nzf = o.nzf;
+    for (int iz = 4; iz < nzf; ++iz) {   // row 3: p_k = 0, no dependence
+        for (int ix = 3; ix <= nx - 4; ++ix) {
+            const size_t idx = (size_t)iz * nx + ix, g = o.gf(iz, ix);
+            const double xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+            const double Km = o.Kf[idx], qp = Q[FP][idx];
+            const int dzn = std::min(iz - 3, nzf - 1 - iz);
+            const double *vx = S[FVX], *vz = S[FVZ];
+            const double dvxx = drp(dir, [&](int m) { return vx[idx + m]; });
+            const double dvzx = drp(dir, [&](int m) { return vz[idx + m]; });
+            const double dvxz = mac_dze(dir, dzn, false, [&](int m) { return vx[idx + (std::ptrdiff_t)m * nx]; });
+            const double dvzz = mac_dze(dir, dzn, false, [&](int m) { return vz[idx + (std::ptrdiff_t)m * nx]; });
+            double bracket = -(xx * dvxx + zx * dvxz + xz * dvzx + zz * dvzz);
+            double a_xx = -Km * xx * qp, a_xz = -Km * zx * qp, a_zx = -Km * xz * qp, a_zz = -Km * zz * qp;
+            if (in_side(o, ix)) {
+                const double gxixi = xx * xx + xz * xz, gxize = xx * zx + xz * zz, gg = gxize / gxixi;
+                const double be = o.be_xi[ix], dpb = o.dp_xi[ix] / be, apdp = o.al_xi[ix] + dpb, invb = 1.0 / be;
+                const double qEx = Q[FXEX][idx], qEz = Q[FXEZ][idx];
+                const double Dvxb = -Km * (invb - 1.0) * xx * qp + dpb * qEx;
+                const double Dvzb = -Km * (invb - 1.0) * xz * qp + dpb * qEz;
+                Sb[FXEX][idx] += Km * invb * xx * qp - apdp * qEx;
+                Sb[FXEZ][idx] += Km * invb * xz * qp - apdp * qEz;
+                a_xx += Dvxb; a_xz += gg * Dvxb; a_zx += Dvzb; a_zz += gg * Dvzb;
+                const double Dvx = dvxx + gg * dvxz, Dvz = dvzx + gg * dvzz;
+                bracket += -(invb - 1.0) * xx * Dvx - (invb - 1.0) * xz * Dvz + invb * xx * S[FXEX][idx] + invb * xz * S[FXEZ][idx];
+            }
+            drp_adj(dir, a_xx, [&](int m, double v) { Sb[FVX][idx + m] += v; });
+            drp_adj(dir, a_zx, [&](int m, double v) { Sb[FVZ][idx + m] += v; });
+            mac_dze_adj(dir, dzn, false, a_xz, [&](int m, double v) { Sb[FVX][idx + (std::ptrdiff_t)m * nx] += v; });
+            mac_dze_adj(dir, dzn, false, a_zz, [&](int m, double v) { Sb[FVZ][idx + (std::ptrdiff_t)m * nx] += v; });
+            o.G[G_K][idx] += qp * bracket;
+        }
+    }
+}
+
+// Transpose of update_stress_solid (include/pml_fs_kernels.cuh:509-628) + dk/dC.
+void stress_solid_adj(Oracle &o, const double *const *S, const double *const *Q, double *const *Sb, int dir) {
+    const int nx = o.nx, nzs = o.nzs, npd = o.npd;
+    for (int iz = 0; iz < nzs - 3; ++iz) {
+        for (int ix = 3; ix <= nx - 4; ++ix) {
+            const size_t idx = (size_t)iz * nx + ix, g = o.gs(iz, ix);
+            const double *vx = S[SVX], *vz = S[SVZ];
+            const double dvxx = drp(dir, [&](int m) { return vx[idx + m]; });
+            const double dvxz = mac_dze(dir, iz, true, [&](int m) { return vx[idx + (std::ptrdiff_t)m * nx]; });
+            const double dvzx = drp(dir, [&](int m) { return vz[idx + m]; });
+            const double dvzz = mac_dze(dir, iz, true, [&](int m) { return vz[idx + (std::ptrdiff_t)m * nx]; });
+            const double xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+            const double c11 = o.C11[idx], c13 = o.C13[idx], c33 = o.C33[idx], c55 = o.C55[idx];
+            const double qsx = Q[SSX][idx], qsz = Q[SSZ][idx], qsxz = Q[SSXZ][idx];
+            const double exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
+            double A11 = exx, A13 = ezz, A55 = xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz;
+            const double exxb = c11 * qsx + c13 * qsz, ezzb = c13 * qsx + c33 * qsz;
+            double bxx = xx * exxb + c55 * xz * qsxz, bxz = zx * exxb + c55 * zz * qsxz;
+            double bzx = xz * ezzb + c55 * xx * qsxz, bzz = zz * ezzb + c55 * zx * qsxz;
+            const bool side = in_side(o, ix), bottom = iz >= nzs - 3 - npd;
+            const double gxixi = xx * xx + xz * xz, gxize = xx * zx + xz * zz, gzeze = zx * zx + zz * zz;
+            if (side) {
+                const double gx = gxize / gxixi;
+                const double be = o.be_xi[ix], dpb = o.dp_xi[ix] / be, apdp = o.al_xi[ix] + dpb, invb = 1.0 / be, q1 = invb - 1.0;
+                const double Ex = S[SXEX][idx], Ez = S[SXEZ][idx], qEx = Q[SXEX][idx], qEz = Q[SXEZ][idx];
+                const double Dvxb = q1 * xx * exxb + c55 * q1 * xz * qsxz + dpb * qEx;
+                const double Dvzb = q1 * xz * ezzb + c55 * q1 * xx * qsxz + dpb * qEz;
+                Sb[SXEX][idx] += -invb * xx * exxb - invb * c55 * xz * qsxz - apdp * qEx;
+                Sb[SXEZ][idx] += -invb * xz * ezzb - invb * c55 * xx * qsxz - apdp * qEz;
+                bxx += Dvxb; bxz += gx * Dvxb; bzx += Dvzb; bzz += gx * Dvzb;
+                const double Dvx = dvxx + gx * dvxz, Dvz = dvzx + gx * dvzz;
+                A11 += q1 * xx * Dvx - invb * xx * Ex;
+                A13 += q1 * xz * Dvz - invb * xz * Ez;
+                A55 += q1 * (xz * Dvx + xx * Dvz) - invb * xz * Ex - invb * xx * Ez;
+            }
+            if (bottom) {
+                const double gz = gxize / gzeze;
+                const double be = o.be_zs[iz], dpb = o.dp_zs[iz] / be, apdp = o.al_zs[iz] + dpb, invb = 1.0 / be, q1 = invb - 1.0;
+                const double Ex = S[SZEX][idx], Ez = S[SZEZ][idx], qEx = Q[SZEX][idx], qEz = Q[SZEZ][idx];
+                const double Dvxb = q1 * zx * exxb + c55 * q1 * zz * qsxz + dpb * qEx;
+                const double Dvzb = q1 * zz * ezzb + c55 * q1 * zx * qsxz + dpb * qEz;
+                Sb[SZEX][idx] += -invb * zx * exxb - invb * c55 * zz * qsxz - apdp * qEx;
+                Sb[SZEZ][idx] += -invb * zz * ezzb - invb * c55 * zx * qsxz - apdp * qEz;
+                bxz += Dvxb; bxx += gz * Dvxb; bzz += Dvzb; bzx += gz * Dvzb;
+                const double Dvx = dvxz + gz * dvxx, Dvz = dvzz + gz * dvzx;
+                A11 += q1 * zx * Dvx - invb * zx * Ex;
+                A13 += q1 * zz * Dvz - invb * zz * Ez;
+                A55 += q1 * (zz * Dvx + zx * Dvz) - invb * zz * Ex - invb * zx * Ez;
+            }
+            drp_adj(dir, bxx, [&](int m, double v) { Sb[SVX][idx + m] += v; });
+            drp_adj(dir, bzx, [&](int m, double v) { Sb[SVZ][idx + m] += v; });
+            mac_dze_adj(dir, iz, true, bxz, [&](int m, double v) { Sb[SVX][idx + (std::ptrdiff_t)m * nx] += v; });
+            mac_dze_adj(dir, iz, true, bzz, [&](int m, double v) { Sb[SVZ][idx + (std::ptrdiff_t)m * nx] += v; });
+            o.G[G_C11][idx] += qsx * A11;
+            o.G[G_C13][idx] += qsx * A13 + qsz * A11;
+            o.G[G_C33][idx] += qsz * A13;
+            o.G[G_C55][idx] += qsxz * A55;
+        }
+    }
+}
+
+// Transpose of update_velocity_solid (include/pml_fs_kernels.cuh:631-752) + dk/d(rho_s).
+void velocity_solid_adj(Oracle &o, const double *const *S, const double *const *Q, double *const *Sb, int dir) {
+    const int nx = o.nx, nzs = o.nzs, npd = o.npd;
+    for (int iz = 0; iz < nzs - 3; ++iz) {
+        for (int ix = 3; ix <= nx - 4; ++ix) {
+            const size_t idx = (size_t)iz * nx + ix, g = o.gs(iz, ix);
+            const double J = o.J[g], xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+            const double gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
+            const double rho = o.rhos[idx], qvx = Q[SVX][idx], qvz = Q[SVZ][idx];
+            const bool side = in_side(o, ix), bottom = iz >= nzs - 3 - npd;
+            const double rvxb = qvx / rho, rvzb = qvz / rho;
+            double Dxxb = 0, Dxzb = 0, Dzxb = 0, Dzzb = 0;
+            double q1x = 0, pxc = 1, q1z = 0, pzc = 1;
+            if (side) {
+                const double be = o.be_xi[ix], dpb = o.dp_xi[ix] / be, apdp = o.al_xi[ix] + dpb;
+                pxc = 1.0 / be; q1x = pxc - 1.0;
+                const double qVx = Q[SXVX][idx], qVz = Q[SXVZ][idx];
+                Dxxb = q1x / rho * qvx + dpb * qVx;
+                Dxzb = q1x / rho * qvz + dpb * qVz;
+                Sb[SXVX][idx] += -pxc / rho * qvx - apdp * qVx;
+                Sb[SXVZ][idx] += -pxc / rho * qvz - apdp * qVz;
+            }
+            if (bottom) {
+                const double be = o.be_zs[iz], dpb = o.dp_zs[iz] / be, apdp = o.al_zs[iz] + dpb;
+                pzc = 1.0 / be; q1z = pzc - 1.0;
+                const double qVx = Q[SZVX][idx], qVz = Q[SZVZ][idx];
+                Dzxb = q1z / rho * qvx + dpb * qVx;
+                Dzzb = q1z / rho * qvz + dpb * qVz;
+                Sb[SZVX][idx] += -pzc / rho * qvx - apdp * qVx;
+                Sb[SZVZ][idx] += -pzc / rho * qvz - apdp * qVz;
+            }
+            double rvx, rvz, Dxx, Dxz, Dzx, Dzz;
+            if (iz < 3) {
+                auto fze_fwd = [&](int r, double &fx, double &fz) {
+                    if (r >= 0) fze_s_row(o, S, r, ix, fx, fz);
+                    else { double ax, az, bx, bz; fze_s_row(o, S, 0, ix, ax, az); fze_s_row(o, S, -r, ix, bx, bz); fx = 2 * ax - bx; fz = 2 * az - bz; }
+                };
+                const double sx = drp(dir, [&](int m) { double a, b; fxi_s(o, S, iz, ix + m, a, b); return a; }) +
+                                  drp(dir, [&](int m) { double a, b; fze_fwd(iz + m, a, b); return a; });
+                const double sz = drp(dir, [&](int m) { double a, b; fxi_s(o, S, iz, ix + m, a, b); return b; }) +
+                                  drp(dir, [&](int m) { double a, b; fze_fwd(iz + m, a, b); return b; });
+                rvx = sx / J; rvz = sz / J;
+                const double pxi = (xx * sx + xz * sz) / (gxixi * J), pze = (zx * sx + zz * sz) / (gzeze * J);
+                Dxx = xx * pxi; Dxz = xz * pxi; Dzx = zx * pze; Dzz = zz * pze;
+                const double pxib = xx * Dxxb + xz * Dxzb, pzeb = zx * Dzxb + zz * Dzzb;
+                const double sxb = rvxb / J + xx * pxib / (gxixi * J) + zx * pzeb / (gzeze * J);
+                const double szb = rvzb / J + xz * pxib / (gxixi * J) + zz * pzeb / (gzeze * J);
+                drp_adj(dir, 1.0, [&](int m, double w) {
+                    const size_t gq = o.gs(iz, ix + m), i = idx + m;
+                    const double Fx = w * sxb, Fz = w * szb;   // adjoints of the xi flux components
+                    Sb[SSX][i] += o.J[gq] * o.xix[gq] * Fx;
+                    Sb[SSXZ][i] += o.J[gq] * (o.xiz[gq] * Fx + o.xix[gq] * Fz);
+                    Sb[SSZ][i] += o.J[gq] * o.xiz[gq] * Fz;
+                });
+                auto fze_add = [&](int r, double Fx, double Fz) {
+                    const size_t gq = o.gs(r, ix), i = (size_t)r * nx + ix;
+                    Sb[SSX][i] += o.J[gq] * o.zex[gq] * Fx;
+                    Sb[SSXZ][i] += o.J[gq] * (o.zez[gq] * Fx + o.zex[gq] * Fz);
+                    Sb[SSZ][i] += o.J[gq] * o.zez[gq] * Fz;
+                };
+                drp_adj(dir, 1.0, [&](int m, double w) {
+                    const int r = iz + m;
+                    const double Fx = w * sxb, Fz = w * szb;
+                    if (r >= 0) fze_add(r, Fx, Fz);
+                    else { fze_add(0, 2 * Fx, 2 * Fz); fze_add(-r, -Fx, -Fz); }
+                });
+            } else {
+                const double *sxx = S[SSX], *szz = S[SSZ], *sxz = S[SSXZ];
+                auto dx = [&](const double *f) { return drp(dir, [&](int m) { return f[idx + m]; }); };
+                auto dz = [&](const double *f) { return drp(dir, [&](int m) { return f[idx + (std::ptrdiff_t)m * nx]; }); };
+                const double a = dx(sxx), b = dz(sxx), c = dx(szz), d = dz(szz), e = dx(sxz), f = dz(sxz);
+                const double gx = gxize / gxixi, gz = gxize / gzeze;
+                rvx = xx * a + zx * b + xz * e + zz * f;
+                rvz = xx * e + zx * f + xz * c + zz * d;
+                Dxx = xx * (a + gx * b) + xz * (e + gx * f);
+                Dxz = xx * (e + gx * f) + xz * (c + gx * d);
+                Dzx = zx * (b + gz * a) + zz * (gz * e + f);
+                Dzz = zx * (gz * e + f) + zz * (gz * c + d);
+                const double ab = xx * rvxb + xx * Dxxb + zx * gz * Dzxb;
+                const double bb = zx * rvxb + xx * gx * Dxxb + zx * Dzxb;
+                const double cb = xz * rvzb + xz * Dxzb + zz * gz * Dzzb;
+                const double db = zz * rvzb + xz * gx * Dxzb + zz * Dzzb;
+                const double eb = xz * rvxb + xx * rvzb + xz * Dxxb + xx * Dxzb + zz * gz * Dzxb + zx * gz * Dzzb;
+                const double fb = zz * rvxb + zx * rvzb + xz * gx * Dxxb + xx * gx * Dxzb + zz * Dzxb + zx * Dzzb;
+                auto ax = [&](int fld, double gv) { drp_adj(dir, gv, [&](int m, double v) { Sb[fld][idx + m] += v; }); };
+                auto az = [&](int fld, double gv) { drp_adj(dir, gv, [&](int m, double v) { Sb[fld][idx + (std::ptrdiff_t)m * nx] += v; }); };
+                ax(SSX, ab); az(SSX, bb); ax(SSZ, cb); az(SSZ, db); ax(SSXZ, eb); az(SSXZ, fb);
+            }
+            double kvx = rvx, kvz = rvz;
+            if (side) { kvx += q1x * Dxx - pxc * S[SXVX][idx]; kvz += q1x * Dxz - pxc * S[SXVZ][idx]; }
+            if (bottom) { kvx += q1z * Dzx - pzc * S[SZVX][idx]; kvz += q1z * Dzz - pzc * S[SZVZ][idx]; }
+            o.G[G_RHOS][idx] += -(qvx * kvx + qvz * kvz) / (rho * rho);
+        }
+    }
+}
+
+// The interface BC and the filter are per-column 8x8 linear maps; their
+// transposes are built by probing the forward maps with unit vectors.
+void bc_matrix(double n1, double n2, int dir, double M[8][8]) {
+    for (int j = 0; j < 8; ++j) {
+        double k[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        k[j] = 1.0;
+        const double vxf = k[0], vzf = k[1], vxs = k[3], vzs = k[4], sxx = k[5], szz = k[6], sxz = k[7];
+        double pf = k[2];
+        double vnf = n1 * vxf + n2 * vzf;
+        const double vtf = n2 * vxf - n1 * vzf;
+        double vns = n1 * vxs + n2 * vzs;
+        const double vts = n2 * vxs - n1 * vzs;
+        double snn = n1 * n1 * sxx + 2 * n1 * n2 * sxz + n2 * n2 * szz;
+        double snt = n1 * n2 * (sxx - szz) + (n2 * n2 - n1 * n1) * sxz;
+        const double stt = n2 * n2 * sxx - 2 * n1 * n2 * sxz + n1 * n1 * szz;
+        if (dir == F) { vnf = vns; pf = -snn; snt = 0; }
+        else { vns = vnf; snn = -pf; snt = 0; }
+        const double out[8] = {n1 * vnf + n2 * vtf, n2 * vnf - n1 * vtf, pf, n1 * vns + n2 * vts, n2 * vns - n1 * vts,
+                               n1 * n1 * snn + 2 * n1 * n2 * snt + n2 * n2 * stt,
+                               n2 * n2 * snn - 2 * n1 * n2 * snt + n1 * n1 * stt,
+                               -n1 * n2 * stt + (n2 * n2 - n1 * n1) * snt};
+        for (int i = 0; i < 8; ++i) M[i][j] = out[i];
+    }
+}
+void filter_matrix(double n1, double n2, double M[8][8]) {
+    for (int j = 0; j < 8; ++j) {
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        v[j] = 1.0;
+        const double vnf = n1 * v[0] + n2 * v[1], vtf = n2 * v[0] - n1 * v[1];
+        const double vns = n1 * v[3] + n2 * v[4], vts = n2 * v[3] - n1 * v[4];
+        const double snn = n1 * n1 * v[5] + 2.0 * n1 * n2 * v[7] + n2 * n2 * v[6];
+        const double stt = n2 * n2 * v[5] - 2.0 * n1 * n2 * v[7] + n1 * n1 * v[6];
+        const double vn = 0.5 * (vnf + vns), sn = 0.5 * (snn - v[2]);
+        const double out[8] = {n1 * vn + n2 * vtf, n2 * vn - n1 * vtf, -sn, n1 * vn + n2 * vts, n2 * vn - n1 * vts,
+                               n1 * n1 * sn + n2 * n2 * stt, n2 * n2 * sn + n1 * n1 * stt, n1 * n2 * (sn - stt)};
+        for (int i = 0; i < 8; ++i) M[i][j] = out[i];
+    }
+}
+
+inline double *iface_ptr(Oracle &o, double *const *A, int f, int ix) {
+    static const int fid[8] = {FVX, FVZ, FP, SVX, SVZ, SSX, SSZ, SSXZ};
+    const size_t row = f < 3 ? (size_t)(o.nzf - 1) * o.nx : 0;
+    return &A[fid[f]][row + ix];
+}
+
+// kbar -> qbar = B^T kbar on the interface rows (in place).
+void bc_adj(Oracle &o, double *const *A, int dir) {
+    for (int ix = 3; ix <= o.nx - 4; ++ix) {
+        double M[8][8];
+        bc_matrix(o.nifx[ix], o.nifz[ix], dir, M);
+        double kb[8], qb[8];
+        for (int i = 0; i < 8; ++i) kb[i] = *iface_ptr(o, A, i, ix);
+        for (int j = 0; j < 8; ++j) { qb[j] = 0; for (int i = 0; i < 8; ++i) qb[j] += M[i][j] * kb[i]; }
+        for (int j = 0; j < 8; ++j) *iface_ptr(o, A, j, ix) = qb[j];
+    }
+}
+
+// lambda -> Filter^T lambda on the interface rows (out of place).
+void filter_adj(Oracle &o, double *const *A) {
+    const int nx = o.nx, le = 3, ri = nx - 4;
+    std::vector<double> out(8 * (size_t)nx, 0.0);
+    for (int ix = le; ix <= ri; ++ix) {
+        double M[8][8];
+        filter_matrix(o.nifx[ix], o.nifz[ix], M);
+        double yb[8], vb[8];
+        for (int i = 0; i < 8; ++i) yb[i] = *iface_ptr(o, A, i, ix);
+        for (int j = 0; j < 8; ++j) { vb[j] = 0; for (int i = 0; i < 8; ++i) vb[j] += M[i][j] * yb[i]; }
+        const int klo = std::max(-3, le - ix), khi = std::min(3, ri - ix);
+        double wsum = 0.0;
+        for (int k = klo; k <= khi; ++k) wsum += W7[k + 3];
+        for (int f = 0; f < 8; ++f)
+            for (int k = klo; k <= khi; ++k) out[(size_t)f * nx + ix + k] += W7[k + 3] / wsum * vb[f];
+    }
+    for (int f = 0; f < 8; ++f)
+        for (int ix = le; ix <= ri; ++ix) *iface_ptr(o, A, f, ix) = out[(size_t)f * nx + ix];
+}
+
+// Sbar += L_dir^T(qbar) with gradients on the forward stage state S.
+void rhs_adj(Oracle &o, const double *const *S, const double *const *Q, double *const *Sb, int dir) {
+    velocity_fluid_adj(o, S, Q, Sb, dir);
+    stress_fluid_adj(o, S, Q, Sb, dir);
+    velocity_solid_adj(o, S, Q, Sb, dir);
+    stress_solid_adj(o, S, Q, Sb, dir);
+}
+
+struct Field20 {
+    std::vector<double> v[NFIELD];
+    std::vector<double> &operator[](int i) { return v[i]; }
+    const std::vector<double> &operator[](int i) const { return v[i]; }
+};
+
+// Forward step keeping the stage states (U^n, T2, T3, T4) of step it.
+void step_keep(Oracle &o, Field20 &T2, Field20 &T3, Field20 &T4) {
+    double *S[NFIELD], *Kd[NFIELD];
+    for (int f = 0; f < NFIELD; ++f) { Kd[f] = o.K[f].data(); }
+    const int d1 = (o.it & 1) ? B : F, d2 = (o.it & 1) ? F : B;
+    const int dirs[4] = {d1, d2, d1, d2};
+    const double dt = o.dt, ct[3] = {dt / 2, dt / 2, dt};
+    Field20 *outs[3] = {&T2, &T3, &T4};
+    for (int f = 0; f < NFIELD; ++f) o.Bacc[f] = o.U[f];
+    for (int s = 0; s < 4; ++s) {
+        for (int f = 0; f < NFIELD; ++f) S[f] = s == 0 ? o.U[f].data() : (*outs[s - 1])[f].data();
+        rhs(o, S, Kd, dirs[s], stage_amp(o, o.it, s));
+        const double cb = s == 0 || s == 3 ? dt / 6 : dt / 3;
+        for (int f = 0; f < NFIELD; ++f) {
+            if (s < 3) axpy((*outs[s])[f], o.U[f], ct[s], o.K[f]);
+            self_axpy(o.Bacc[f], cb, o.K[f]);
+        }
+    }
+    for (int f = 0; f < NFIELD; ++f) o.U[f] = o.Bacc[f];
+    double *Ud[NFIELD];
+    for (int f = 0; f < NFIELD; ++f) Ud[f] = o.U[f].data();
+    filter(o, Ud);
+    o.it++;
+}
+
+// Adjoint of one step (reverse of step_keep), lam in/out.
+void step_adj(Oracle &o, int it, const Field20 &Un, const Field20 &T2, const Field20 &T3, const Field20 &T4, Field20 &lam) {
+    const int nfz = NFIELD;
+    double *L[NFIELD];
+    for (int f = 0; f < nfz; ++f) L[f] = lam[f].data();
+    filter_adj(o, L);                               // lambda'
+    const int d1 = (it & 1) ? B : F, d2 = (it & 1) ? F : B;
+    const double dt = o.dt;
+    Field20 acc, kb, tb;
+    for (int f = 0; f < nfz; ++f) { acc[f] = lam[f]; kb[f].assign(lam[f].size(), 0.0); tb[f].assign(lam[f].size(), 0.0); }
+    const Field20 *states[4] = {&Un, &T2, &T3, &T4};
+    const int dirs[4] = {d1, d2, d1, d2};
+    const double clam[4] = {dt / 6, dt / 3, dt / 3, dt / 6};   // coefficient of lambda' in kbar_s
+    const double cprev[4] = {dt / 2, dt / 2, dt, 0.0};         // coefficient of Tbar_{s+1}
+    for (int s = 3; s >= 0; --s) {
+        for (int f = 0; f < nfz; ++f)
+            for (size_t i = 0; i < kb[f].size(); ++i) kb[f][i] = clam[s] * lam[f][i] + cprev[s] * tb[f][i];
+        double *K[NFIELD], *TB[NFIELD];
+        const double *S[NFIELD], *Q[NFIELD];
+        for (int f = 0; f < nfz; ++f) { K[f] = kb[f].data(); }
+        bc_adj(o, K, dirs[s]);
+        for (int f = 0; f < nfz; ++f) {
+            tb[f].assign(lam[f].size(), 0.0);
+            TB[f] = tb[f].data();
+            S[f] = (*states[s])[f].data();
+            Q[f] = kb[f].data();
+        }
+        rhs_adj(o, S, Q, TB, dirs[s]);
+        for (int f = 0; f < nfz; ++f)
+            for (size_t i = 0; i < acc[f].size(); ++i) acc[f][i] += tb[f][i];
+    }
+    for (int f = 0; f < nfz; ++f) lam[f].swap(acc[f]);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -611,6 +1080,8 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
         o->K[f].assign(o->fsize(f), 0.0);
     }
     for (auto *g : {&o->Gfx, &o->Gfz, &o->Gsx, &o->Gsz, &o->Gfsx, &o->Gfsz}) g->assign(3 * (size_t)o->nx, 0.0);
+    for (int f = 0; f < NFIELD; ++f) { o->Ain[f].assign(o->fsize(f), 0.0); o->Aout[f].assign(o->fsize(f), 0.0); }
+    for (int g = 0; g < 7; ++g) o->G[g].assign((size_t)(g < 2 ? o->nzf : o->nzs) * o->nx, 0.0);
     return o;
 }
 
@@ -694,6 +1165,116 @@ void orc_normals(void *h, double *nxo, double *nzo) {
     std::memcpy(nzo, o->nifz.data(), o->nx * sizeof(double));
 }
 double orc_stage_amp(void *h, int it, int stage) { return stage_amp(*static_cast<Oracle *>(h), it, stage); }
+
+
+// --- adjoint probes ------------------------------------------------------
+void orc_set_adj_in(void *h, int f, const double *in) {
+    Oracle *o = static_cast<Oracle *>(h);
+    std::memcpy(o->Ain[f].data(), in, o->Ain[f].size() * sizeof(double));
+}
+void orc_get_adj_out(void *h, int f, double *out) {
+    Oracle *o = static_cast<Oracle *>(h);
+    std::memcpy(out, o->Aout[f].data(), o->Aout[f].size() * sizeof(double));
+}
+void orc_get_grad(void *h, int which, double *out) {
+    Oracle *o = static_cast<Oracle *>(h);
+    std::memcpy(out, o->G[which].data(), o->G[which].size() * sizeof(double));
+}
+void orc_zero_grad(void *h) {
+    Oracle *o = static_cast<Oracle *>(h);
+    for (auto &g : o->G) std::fill(g.begin(), g.end(), 0.0);
+}
+// op: 0 = L_dir^T (forward stage state = U), 1 = B_dir^T, 2 = Filter^T,
+// 3 = the transpose of one whole step it = dir (forward state = U, source off).
+void orc_adjoint_op(void *h, int op, int dir) {
+    Oracle *o = static_cast<Oracle *>(h);
+    for (int f = 0; f < NFIELD; ++f) o->Aout[f] = o->Ain[f];
+    double *A[NFIELD];
+    for (int f = 0; f < NFIELD; ++f) A[f] = o->Aout[f].data();
+    if (op == 0) {
+        std::vector<double> Sb[NFIELD];
+        double *SB[NFIELD];
+        const double *S[NFIELD], *Q[NFIELD];
+        for (int f = 0; f < NFIELD; ++f) { Sb[f].assign(o->Ain[f].size(), 0.0); SB[f] = Sb[f].data(); S[f] = o->U[f].data(); Q[f] = o->Ain[f].data(); }
+        rhs_adj(*o, S, Q, SB, dir);
+        for (int f = 0; f < NFIELD; ++f) o->Aout[f] = Sb[f];
+    } else if (op == 1) {
+        bc_adj(*o, A, dir);
+    } else if (op == 2) {
+        filter_adj(*o, A);
+    } else {
+        Field20 Un, T2, T3, T4;
+        for (int f = 0; f < NFIELD; ++f) { Un[f] = o->U[f]; T2[f].assign(o->U[f].size(), 0.0); T3[f] = T2[f]; T4[f] = T2[f]; }
+        const int save_it = o->it;
+        std::vector<int> sx = o->srcx;
+        o->srcx.clear();
+        o->it = dir;
+        step_keep(*o, T2, T3, T4);
+        Field20 lam;
+        for (int f = 0; f < NFIELD; ++f) lam[f] = o->Ain[f];
+        step_adj(*o, dir, Un, T2, T3, T4, lam);
+        for (int f = 0; f < NFIELD; ++f) { o->Aout[f] = lam[f]; o->U[f] = Un[f]; }
+        o->srcx = sx;
+        o->it = save_it;
+    }
+}
+// Forward in place of the linear step map (source off) for the dot-product test.
+void orc_step_nosrc(void *h, int it) {
+    Oracle *o = static_cast<Oracle *>(h);
+    std::vector<int> sx = o->srcx;
+    o->srcx.clear();
+    o->it = it;
+    step(*o);
+    o->srcx = sx;
+}
+
+// Gradient of J = sum_r <res_r, rec_r> (res = dJ/drec, records [nrec][nx] for
+// solid vx, vz and fluid p) with respect to K, rho_f, C11, C13, C33, C55,
+// rho_s for shot isrc and nsteps steps from a zero wavefield.  Checkpoints
+// every `every` steps, recomputing each segment.
+void orc_gradient(void *h, int isrc, int nsteps, int every, const double *res_vx, const double *res_vz,
+                  const double *res_p) {
+    Oracle *o = static_cast<Oracle *>(h);
+    orc_reset(h, isrc);
+    orc_zero_grad(h);
+    if (every < 1) every = 1;
+    std::vector<Field20 *> ckpt;
+    for (int it = 0; it < nsteps; ++it) {
+        if (it % every == 0) { auto *c = new Field20; for (int f = 0; f < NFIELD; ++f) (*c)[f] = o->U[f]; ckpt.push_back(c); }
+        step(*o);
+    }
+    Field20 lam;
+    for (int f = 0; f < NFIELD; ++f) lam[f].assign(o->U[f].size(), 0.0);
+    const int nx = o->nx, gap = o->gap;
+    const size_t rf = (size_t)(o->nzf - 1) * nx;
+    for (int seg = (int)ckpt.size() - 1; seg >= 0; --seg) {
+        const int it0 = seg * every, it1 = std::min(nsteps, it0 + every);
+        std::vector<Field20 *> un, t2, t3, t4;
+        for (int f = 0; f < NFIELD; ++f) o->U[f] = (*ckpt[seg])[f];
+        o->it = it0;
+        for (int it = it0; it < it1; ++it) {
+            auto *a = new Field20, *b = new Field20, *c = new Field20, *d = new Field20;
+            for (int f = 0; f < NFIELD; ++f) { (*a)[f] = o->U[f]; (*b)[f].assign(o->U[f].size(), 0.0); (*c)[f] = (*b)[f]; (*d)[f] = (*b)[f]; }
+            step_keep(*o, *b, *c, *d);
+            un.push_back(a); t2.push_back(b); t3.push_back(c); t4.push_back(d);
+        }
+        for (int it = it1 - 1; it >= it0; --it) {
+            if (it % gap == 0) {
+                const int r = it / gap;
+                for (int ix = 0; ix < nx; ++ix) {
+                    lam[SVX][ix] += res_vx[(size_t)r * nx + ix];
+                    lam[SVZ][ix] += res_vz[(size_t)r * nx + ix];
+                    lam[FP][rf + ix] += res_p[(size_t)r * nx + ix];
+                }
+            }
+            const int k = it - it0;
+            step_adj(*o, it, *un[k], *t2[k], *t3[k], *t4[k], lam);
+            delete un[k]; delete t2[k]; delete t3[k]; delete t4[k];
+        }
+        delete ckpt[seg];
+    }
+    for (int f = 0; f < NFIELD; ++f) o->Aout[f] = lam[f];   // adjoint of the initial state
+}
 
 // 1-D operator probes for pinning: kind 0 = DRP, 1 = M42, 2 = M22,
 // 3 = distance-dependent zeta operator (dist, solid_side); f sampled at i.

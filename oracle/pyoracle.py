@@ -57,6 +57,10 @@ def lib():
             ("orc_pml", [vp, C.c_int, dp, dp, dp], None), ("orc_normals", [vp, dp, dp], None),
             ("orc_stage_amp", [vp, C.c_int, C.c_int], C.c_double),
             ("orc_op1d", [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int], C.c_double),
+            ("orc_set_adj_in", [vp, C.c_int, dp], None), ("orc_get_adj_out", [vp, C.c_int, dp], None),
+            ("orc_get_grad", [vp, C.c_int, dp], None), ("orc_zero_grad", [vp], None),
+            ("orc_adjoint_op", [vp, C.c_int, C.c_int], None), ("orc_step_nosrc", [vp, C.c_int], None),
+            ("orc_gradient", [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp], None),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -166,6 +170,47 @@ class Oracle:
 
     def stage_amp(self, it: int, stage: int) -> float:
         return lib().orc_stage_amp(self.h, int(it), int(stage))
+
+
+GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
+
+
+def _adj_methods():
+    def set_adj_in(self, f, arr):
+        f = FID[f] if isinstance(f, str) else f
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape(f))
+        lib().orc_set_adj_in(self.h, f, _dptr(a))
+
+    def get_adj_out(self, f):
+        f = FID[f] if isinstance(f, str) else f
+        out = np.zeros(self.shape(f))
+        lib().orc_get_adj_out(self.h, f, _dptr(out))
+        return out
+
+    def adjoint_op(self, op, direction):
+        lib().orc_adjoint_op(self.h, int(op), int(direction))
+
+    def step_nosrc(self, it):
+        lib().orc_step_nosrc(self.h, int(it))
+
+    def grad(self, name):
+        i = GRADS.index(name)
+        rows = self.nzf if i < 2 else self.nzs
+        out = np.zeros((rows, self.nx))
+        lib().orc_get_grad(self.h, i, _dptr(out))
+        return out
+
+    def gradient(self, isrc, nsteps, res, every=16):
+        """Gradient of sum_r <res_r, rec_r> w.r.t. the model arrays (dict)."""
+        rs = [np.ascontiguousarray(r, dtype=np.float64) for r in res]
+        lib().orc_gradient(self.h, int(isrc), int(nsteps), int(every), *[_dptr(r) for r in rs])
+        return {n: self.grad(n) for n in GRADS}
+
+    for fn in (set_adj_in, get_adj_out, adjoint_op, step_nosrc, grad, gradient):
+        setattr(Oracle, fn.__name__, fn)
+
+
+_adj_methods()
 
 
 def op1d(kind: int, direction: int, f: np.ndarray, i: int, dist: int = 3, solid_side: bool = False) -> float:
