@@ -160,6 +160,31 @@ int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
  * fastest and resets the shot (no reference counterpart). */
 int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
 
+/* --- Discrete adjoint (no reference counterpart) ------------------------- */
+/* Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
+ * arrays for solid vx, solid vz, fluid p as returned by msfdm_get_records) with
+ * respect to the model arrays K, rho_f ([nz_f][nx]) and C11, C13, C33, C55,
+ * rho_s ([nz_s][nx]), all doubles, for shot isrc and nsteps steps from a zero
+ * wavefield.  The transpose of the FP32 scheme (RHS, interface BC, filter,
+ * RK4) runs backwards in time; the forward is recomputed per segment from
+ * checkpoints taken every `every` steps (0: about sqrt(nsteps)), and the
+ * gradients accumulate in FP64.  Any output pointer may be NULL.  Leaves the
+ * records of the forward run readable; the wavefield is undefined afterwards. */
+int msfdm_gradient(msfdm_solver *s, int isrc, int nsteps, int every, const float *res_vx, const float *res_vz,
+                   const float *res_p, double *gK, double *grho_f, double *gC11, double *gC13, double *gC33,
+                   double *gC55, double *grho_s);
+/* The two halves of msfdm_gradient: a checkpointed forward run (records
+ * available afterwards) and the reverse sweep for given residuals. */
+int msfdm_adjoint_forward(msfdm_solver *s, int isrc, int nsteps, int every);
+int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, const float *res_vz, const float *res_p,
+                           double *gK, double *grho_f, double *gC11, double *gC13, double *gC33, double *gC55,
+                           double *grho_s);
+/* Operator probes: set an input field, apply op (0 = L_dir^T B_dir^T at the
+ * current wavefield, 1 = Filter^T), read an output field. */
+int msfdm_adjoint_set_input(msfdm_solver *s, int field, const float *in);
+int msfdm_adjoint_apply(msfdm_solver *s, int op, int dir);
+int msfdm_adjoint_get_output(msfdm_solver *s, int field, float *out);
+
 /* --- Batched shots ------------------------------------------------------- */
 /* Independent shots, each with its own model (grid size, medium, sources) and
  * source index, stepped together: one launch per RK stage covers the tiles of

@@ -15,6 +15,7 @@
 
 #include "model.hpp"
 #include "stencil.hpp"
+#include "adjoint.hpp"
 
 using namespace msfdm;
 namespace D = msfdm::dev;
@@ -64,6 +65,14 @@ struct msfdm_solver {
     hipGraphExec_t gexec = nullptr;
     double bytes_step = 0.0;
     double bytes_kernel[3] = {0.0, 0.0, 0.0};
+    // adjoint state: lambda, lambda', accumulator, two Tbar buffers, interface qbar
+    float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *qif = nullptr;
+    double *gmem = nullptr;          // 7 gradient arrays, [rows][P] doubles
+    D::Grad grad{};
+    std::vector<float *> ckpt;       // checkpoints of the full state every adj_every steps
+    std::vector<float *> seg;        // per-step U^n, T2, T3, T4 of one segment
+    int adj_every = 0, adj_nsteps = 0, adj_isrc = 0;
 };
 
 namespace {
@@ -265,8 +274,19 @@ int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float
     return MSFDM_OK;
 }
 
+void free_adjoint(msfdm_solver *s) {
+    for (auto &p : s->adj) if (p) { (void)hipFree(p); p = nullptr; }
+    if (s->qif) { (void)hipFree(s->qif); s->qif = nullptr; }
+    if (s->gmem) { (void)hipFree(s->gmem); s->gmem = nullptr; }
+    for (float *p : s->ckpt) (void)hipFree(p);
+    for (float *p : s->seg) (void)hipFree(p);
+    s->ckpt.clear();
+    s->seg.clear();
+}
+
 void free_all(msfdm_solver *s) {
     drop_graph(s);
+    free_adjoint(s);
     for (auto &b : s->buf) if (b) { (void)hipFree(b); b = nullptr; }
     for (void *p : {(void *)s->kbuf, (void *)s->kif, (void *)s->coefmem, (void *)s->amp, (void *)s->base, (void *)s->rec})
         if (p) (void)hipFree(p);
@@ -966,4 +986,239 @@ extern "C" int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows) {
     if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
     if (best_rows) *best_rows = arg;
     return MSFDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint: gradient of J = sum_r <res_r, rec_r> with respect to the
+// model arrays, by reverse-time stepping through the transposed scheme with
+// checkpointed forward recomputation.  No reference counterpart (the
+// reference has no adjoint); the forward it differentiates is msfdm_run.
+// ---------------------------------------------------------------------------
+namespace {
+
+enum AdjRole { A_LAM = 0, A_LP = 1, A_ACC = 2, A_TBA = 3, A_TBB = 4 };
+
+int ensure_adjoint(msfdm_solver *s) {
+    const size_t bytes = s->buf_floats * sizeof(float);
+    for (auto &p : s->adj)
+        if (!p) { HIPCHK(hipMalloc(&p, bytes)); HIPCHK(hipMemset(p, 0, bytes)); }
+    const long long P = s->geo.P;
+    if (!s->qif) { HIPCHK(hipMalloc(&s->qif, 8 * P * sizeof(float))); HIPCHK(hipMemset(s->qif, 0, 8 * P * sizeof(float))); }
+    if (!s->gmem) {
+        const long long nF = (long long)s->d.nz_fluid * P, nS = (long long)s->d.nz_solid * P;
+        const long long total = 2 * nF + 5 * nS;
+        HIPCHK(hipMalloc(&s->gmem, total * sizeof(double)));
+        HIPCHK(hipMemset(s->gmem, 0, total * sizeof(double)));
+        double *p = s->gmem;
+        s->grad.K = p; p += nF;
+        s->grad.rho_f = p; p += nF;
+        s->grad.C11 = p; p += nS;
+        s->grad.C13 = p; p += nS;
+        s->grad.C33 = p; p += nS;
+        s->grad.C55 = p; p += nS;
+        s->grad.rho_s = p;
+    }
+    return MSFDM_OK;
+}
+
+template <int DIR, int TZ>
+void launch_adj_media(msfdm_solver *s, const D::AdjIn &a) {
+    const int nbx = (s->d.nx - 4) / D::TX + 1;
+    const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
+    const int ntiles = nbx * (nbf + nbs);
+    D::adj_stage_kernel<DIR, TZ><<<8 * ((ntiles + 7) / 8), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, s->grad, nbx, nbf,
+                                                                                   ntiles);
+}
+
+// One adjoint stage: TB = L_dir^T B_dir^T (clam LP + cprev TBp) at forward state S.
+template <int DIR>
+int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, float clam, float cprev) {
+    D::AdjIn a;
+    a.S = S; a.LP = s->adj[A_LP]; a.TBp = TBp; a.qif = s->qif; a.TB = TB; a.clam = clam; a.cprev = cprev;
+    HIPCHK(hipMemsetAsync(TB, 0, s->buf_floats * sizeof(float), s->stream));
+    D::adj_iface_q<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, a, s->qif);
+    if (s->tz == 8) launch_adj_media<DIR, 8>(s, a);
+    else launch_adj_media<DIR, 16>(s, a);
+    HIPCHK(hipGetLastError());
+    return MSFDM_OK;
+}
+
+// Reverse of one RK4 step `it` (forward stage states Un, T2, T3, T4):
+// lambda <- lambda' + sum of the four stage transposes, lambda' = Filter^T lambda.
+int adj_step(msfdm_solver *s, int it, const float *Un, const float *T2, const float *T3, const float *T4) {
+    const size_t bytes = s->buf_floats * sizeof(float);
+    float *LAM = s->adj[A_LAM], *LP = s->adj[A_LP], *ACC = s->adj[A_ACC], *TA = s->adj[A_TBA], *TBb = s->adj[A_TBB];
+    HIPCHK(hipMemcpyAsync(LP, LAM, bytes, hipMemcpyDeviceToDevice, s->stream));
+    D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, LAM, LP);
+    HIPCHK(hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream));
+    const float dt = s->d.dt;
+    const bool odd = it & 1;                 // forward stage directions: d1 d2 d1 d2
+    const long long n = s->buf_floats;
+    const int ag = (int)std::min<long long>((n + 255) / 256, 8192);
+    auto acc = [&](const float *tb) { D::axpy_kernel<<<ag, 256, 0, s->stream>>>(ACC, tb, n); };
+    int rc;
+    // stage 4 (d2, state T4), stage 3 (d1, T3), stage 2 (d2, T2), stage 1 (d1, U^n)
+    if ((rc = odd ? adj_stage<0>(s, T4, nullptr, TA, dt / 6, 0.f) : adj_stage<1>(s, T4, nullptr, TA, dt / 6, 0.f))) return rc;
+    acc(TA);
+    if ((rc = odd ? adj_stage<1>(s, T3, TA, TBb, dt / 3, dt) : adj_stage<0>(s, T3, TA, TBb, dt / 3, dt))) return rc;
+    acc(TBb);
+    if ((rc = odd ? adj_stage<0>(s, T2, TBb, TA, dt / 3, dt / 2) : adj_stage<1>(s, T2, TBb, TA, dt / 3, dt / 2))) return rc;
+    acc(TA);
+    if ((rc = odd ? adj_stage<1>(s, Un, TA, TBb, dt / 6, dt / 2) : adj_stage<0>(s, Un, TA, TBb, dt / 6, dt / 2))) return rc;
+    acc(TBb);
+    std::swap(s->adj[A_LAM], s->adj[A_ACC]);
+    HIPCHK(hipGetLastError());
+    return MSFDM_OK;
+}
+
+int copy_grad_out(msfdm_solver *s, const double *dsrc, int rows, double *out) {
+    if (!out) return MSFDM_OK;
+    HIPCHK(hipMemcpy2D(out, s->d.nx * sizeof(double), dsrc, s->geo.P * sizeof(double), s->d.nx * sizeof(double), rows,
+                       hipMemcpyDeviceToHost));
+    return MSFDM_OK;
+}
+
+}  // namespace
+
+extern "C" int msfdm_adjoint_forward(msfdm_solver *s, int isrc, int nsteps, int every) {
+    clear_error();
+    if (!s || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(s->device));
+    if (every <= 0) every = std::max(2, (int)std::ceil(std::sqrt((double)nsteps)));
+    if (every & 1) ++every;                  // keep segment starts even (F/B parity of the graph)
+    if (int rc = ensure_adjoint(s)) return rc;
+    for (float *p : s->ckpt) (void)hipFree(p);
+    s->ckpt.clear();
+    if (int rc = msfdm_reset_shot(s, isrc)) return rc;
+    const size_t bytes = s->buf_floats * sizeof(float);
+    for (int it0 = 0; it0 < nsteps; it0 += every) {
+        float *c = nullptr;
+        if (hipMalloc(&c, bytes) != hipSuccess)
+            return set_error(MSFDM_EDEVICE, "out of device memory for adjoint checkpoints");
+        s->ckpt.push_back(c);
+        HIPCHK(hipMemcpyAsync(c, s->buf[R_U], bytes, hipMemcpyDeviceToDevice, s->stream));
+        if (int rc = msfdm_run(s, std::min(every, nsteps - it0))) return rc;
+    }
+    s->adj_every = every;
+    s->adj_nsteps = nsteps;
+    s->adj_isrc = isrc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, const float *res_vz, const float *res_p,
+                                      double *gK, double *grho_f, double *gC11, double *gC13, double *gC33,
+                                      double *gC55, double *grho_s) {
+    clear_error();
+    if (!s || !res_vx || !res_vz || !res_p) return set_error(MSFDM_EINVAL, "null argument");
+    if (s->ckpt.empty()) return set_error(MSFDM_ESTATE, "msfdm_adjoint_forward has not been called");
+    HIPCHK(hipSetDevice(s->device));
+    const int nsteps = s->adj_nsteps, every = s->adj_every, gap = s->gap;
+    const int nrec = (nsteps - 1) / gap + 1;
+    const size_t bytes = s->buf_floats * sizeof(float);
+    const size_t nx = (size_t)s->d.nx;
+    // segment storage: U^n, T2, T3, T4 of each step of one segment
+    if ((int)s->seg.size() < 4 * every) {
+        for (int i = (int)s->seg.size(); i < 4 * every; ++i) {
+            float *p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess)
+                return set_error(MSFDM_EDEVICE, "out of device memory for adjoint segment storage");
+            HIPCHK(hipMemset(p, 0, bytes));
+            s->seg.push_back(p);
+        }
+    }
+    float *res = nullptr;
+    HIPCHK(hipMalloc(&res, 3 * (size_t)nrec * nx * sizeof(float)));
+    HIPCHK(hipMemcpy(res, res_vx, (size_t)nrec * nx * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(res + (size_t)nrec * nx, res_vz, (size_t)nrec * nx * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(res + 2 * (size_t)nrec * nx, res_p, (size_t)nrec * nx * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(s->adj[A_LAM], 0, bytes, s->stream));
+    const long long P = s->geo.P;
+    HIPCHK(hipMemsetAsync(s->gmem, 0, (2 * (long long)s->d.nz_fluid * P + 5 * (long long)s->d.nz_solid * P) * sizeof(double), s->stream));
+    int rc = MSFDM_OK;
+    for (int seg = (int)s->ckpt.size() - 1; seg >= 0 && rc == MSFDM_OK; --seg) {
+        const int it0 = seg * every, it1 = std::min(nsteps, it0 + every);
+        // recompute the segment, keeping every step's stage states
+        if (hipMemcpyAsync(s->buf[R_U], s->ckpt[seg], bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+            rc = set_error(MSFDM_EDEVICE, "checkpoint restore failed");
+            break;
+        }
+        D::set_base<<<1, 64, 0, s->stream>>>(s->base, it0);
+        float *keep[3] = {s->buf[R_T2], s->buf[R_T3], s->buf[R_T4]};
+        for (int it = it0; it < it1; ++it) {
+            const int j = it - it0;
+            (void)hipMemcpyAsync(s->seg[4 * j], s->buf[R_U], bytes, hipMemcpyDeviceToDevice, s->stream);
+            s->buf[R_T2] = s->seg[4 * j + 1];
+            s->buf[R_T3] = s->seg[4 * j + 2];
+            s->buf[R_T4] = s->seg[4 * j + 3];
+            enqueue_steps(s, it, 1);
+        }
+        s->buf[R_T2] = keep[0]; s->buf[R_T3] = keep[1]; s->buf[R_T4] = keep[2];
+        // reverse sweep over the segment
+        for (int it = it1 - 1; it >= it0 && rc == MSFDM_OK; --it) {
+            if (it % gap == 0)
+                D::adj_inject<<<((int)nx + 255) / 256, 256, 0, s->stream>>>(s->geo, s->adj[A_LAM], res, nrec, it / gap);
+            const int j = it - it0;
+            rc = adj_step(s, it, s->seg[4 * j], s->seg[4 * j + 1], s->seg[4 * j + 2], s->seg[4 * j + 3]);
+        }
+    }
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(res);
+    if (rc) return rc;
+    // leave the step counter where the forward run ended
+    s->host_it = nsteps;
+    s->nrec_done = nrec;
+    D::set_base<<<1, 64, 0, s->stream>>>(s->base, nsteps);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
+    if ((rc = copy_grad_out(s, s->grad.K, nzf, gK)) || (rc = copy_grad_out(s, s->grad.rho_f, nzf, grho_f)) ||
+        (rc = copy_grad_out(s, s->grad.C11, nzs, gC11)) || (rc = copy_grad_out(s, s->grad.C13, nzs, gC13)) ||
+        (rc = copy_grad_out(s, s->grad.C33, nzs, gC33)) || (rc = copy_grad_out(s, s->grad.C55, nzs, gC55)) ||
+        (rc = copy_grad_out(s, s->grad.rho_s, nzs, grho_s)))
+        return rc;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_gradient(msfdm_solver *s, int isrc, int nsteps, int every, const float *res_vx,
+                              const float *res_vz, const float *res_p, double *gK, double *grho_f, double *gC11,
+                              double *gC13, double *gC33, double *gC55, double *grho_s) {
+    if (int rc = msfdm_adjoint_forward(s, isrc, nsteps, every)) return rc;
+    return msfdm_adjoint_backward(s, res_vx, res_vz, res_p, gK, grho_f, gC11, gC13, gC33, gC55, grho_s);
+}
+
+// Adjoint probes: the input field set is kept in lambda'; op 0 applies
+// L_dir^T B_dir^T (forward state = U), op 1 applies Filter^T.
+extern "C" int msfdm_adjoint_set_input(msfdm_solver *s, int field, const float *in) {
+    clear_error();
+    if (!s || !in || field < 0 || field >= MSFDM_NFIELD) return set_error(MSFDM_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = ensure_adjoint(s)) return rc;
+    return copy_field(s, s->adj[A_LP], field, nullptr, in);
+}
+
+extern "C" int msfdm_adjoint_apply(msfdm_solver *s, int op, int dir) {
+    clear_error();
+    if (!s || (op != 0 && op != 1) || (dir != 0 && dir != 1)) return set_error(MSFDM_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = ensure_adjoint(s)) return rc;
+    float *out = s->adj[A_TBA];
+    if (op == 0) {
+        const int rc = dir == 0 ? adj_stage<0>(s, s->buf[R_U], nullptr, out, 1.0f, 0.0f)
+                                : adj_stage<1>(s, s->buf[R_U], nullptr, out, 1.0f, 0.0f);
+        if (rc) return rc;
+    } else {
+        HIPCHK(hipMemcpyAsync(out, s->adj[A_LP], s->buf_floats * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+        D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->adj[A_LP], out);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_adjoint_get_output(msfdm_solver *s, int field, float *out) {
+    clear_error();
+    if (!s || !out || field < 0 || field >= MSFDM_NFIELD) return set_error(MSFDM_EINVAL, "bad argument");
+    if (!s->adj[A_TBA]) return set_error(MSFDM_ESTATE, "no adjoint output yet");
+    HIPCHK(hipSetDevice(s->device));
+    return copy_field(s, s->adj[A_TBA], field, out, nullptr);
 }

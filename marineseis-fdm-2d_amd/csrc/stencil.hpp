@@ -707,5 +707,9 @@ __global__ void advance_base(int *base, int n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) base[0] += n;
 }
 
+__global__ void set_base(int *base, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) base[0] = v;
+}
+
 }  // namespace dev
 }  // namespace msfdm

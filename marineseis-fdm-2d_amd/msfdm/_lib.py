@@ -37,7 +37,10 @@ EXPORTS = [
     "msfdm_batch_reset", "msfdm_batch_run", "msfdm_batch_synchronize", "msfdm_batch_size",
     "msfdm_batch_shot", "msfdm_batch_set_tile_rows", "msfdm_batch_get_tile_rows",
     "msfdm_batch_set_graph", "msfdm_batch_time_run", "msfdm_batch_autotune",
+    "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
+    "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
 ]
+GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
 
 
 class MsfdmError(RuntimeError):
@@ -108,6 +111,12 @@ def lib():
         "msfdm_batch_set_graph": ([vp, i], i),
         "msfdm_batch_time_run": ([vp, i, fp], i),
         "msfdm_batch_autotune": ([vp, i, ip], i),
+        "msfdm_gradient": ([vp, i, i, i, fp, fp, fp] + [C.POINTER(C.c_double)] * 7, i),
+        "msfdm_adjoint_forward": ([vp, i, i, i], i),
+        "msfdm_adjoint_backward": ([vp, fp, fp, fp] + [C.POINTER(C.c_double)] * 7, i),
+        "msfdm_adjoint_set_input": ([vp, i, fp], i),
+        "msfdm_adjoint_apply": ([vp, i, i], i),
+        "msfdm_adjoint_get_output": ([vp, i, fp], i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
@@ -278,6 +287,44 @@ class Solver:
 
     def set_graph(self, enable: bool):
         check(lib().msfdm_set_graph(self.h, int(bool(enable))))
+
+    # --- adjoint ------------------------------------------------------------
+    def _grad_arrays(self):
+        return {n: np.zeros(((self.nzf if k < 2 else self.nzs), self.nx)) for k, n in enumerate(GRADS)}
+
+    def adjoint_forward(self, isrc: int, nsteps: int, every: int = 0):
+        check(lib().msfdm_adjoint_forward(self.h, int(isrc), int(nsteps), int(every)))
+
+    def adjoint_backward(self, res):
+        rs = [_f32(r) for r in res]
+        g = self._grad_arrays()
+        dp = C.POINTER(C.c_double)
+        check(lib().msfdm_adjoint_backward(self.h, *[_fptr(r) for r in rs],
+                                           *[g[n].ctypes.data_as(dp) for n in GRADS]))
+        return g
+
+    def gradient(self, isrc: int, nsteps: int, res, every: int = 0):
+        """Gradient of sum_r <res_r, rec_r> w.r.t. the model arrays (dict of float64)."""
+        rs = [_f32(r) for r in res]
+        g = self._grad_arrays()
+        dp = C.POINTER(C.c_double)
+        check(lib().msfdm_gradient(self.h, int(isrc), int(nsteps), int(every), *[_fptr(r) for r in rs],
+                                   *[g[n].ctypes.data_as(dp) for n in GRADS]))
+        return g
+
+    def adjoint_set_input(self, f, arr):
+        f = FID[f] if isinstance(f, str) else f
+        a = _f32(arr).reshape(self.shape(f))
+        check(lib().msfdm_adjoint_set_input(self.h, f, _fptr(a)))
+
+    def adjoint_apply(self, op: int, direction: int):
+        check(lib().msfdm_adjoint_apply(self.h, int(op), int(direction)))
+
+    def adjoint_get_output(self, f) -> np.ndarray:
+        f = FID[f] if isinstance(f, str) else f
+        out = np.zeros(self.shape(f), dtype=np.float32)
+        check(lib().msfdm_adjoint_get_output(self.h, f, _fptr(out)))
+        return out
 
     def time_kernel(self, which: int, reps: int) -> float:
         v = C.c_float()
