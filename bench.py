@@ -89,6 +89,71 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_gradient(args):
+    """Adjoint-gradient workload: the 401x201 flat-seafloor model with 3000 steps;
+    observed records from the true model, gradient of 0.5||rec - obs||^2 w.r.t.
+    vp, vs, rho at the true model smoothed by a 100 m Gaussian.  Checked by central
+    differences along 16 seeded relative directions (eps 1e-3) and by the FP64 host
+    adjoint on the 201x101 cut."""
+    import numpy as np
+    import msfdm
+    from msfdm import autograd as ag
+    from msfdm import fwi, modelgen
+    true = modelgen.preset("flat401", nt=3000)
+    init = fwi.smoothed_model(true, 100.0, 5.0)
+    s = msfdm.Solver(true)
+    s.reset(0); s.run(true.nt); s.sync()
+    obs = s.records()
+    solver = msfdm.Solver(init)
+    fwi.misfit_and_gradient(solver, init, obs, nsteps=init.nt)   # warm-up (allocation, graph)
+    t0 = time.perf_counter()
+    loss, g = fwi.misfit_and_gradient(solver, init, obs, nsteps=init.nt)
+    t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.reset(0); s.run(true.nt); s.sync()
+    t_fwd = time.perf_counter() - t0
+    vel0 = ag.velocity_from_model(init)
+    names = ("vp_f", "rho_f", "vp", "vs", "rho")
+    rng = np.random.default_rng(4)
+    def loss_at(vel):
+        media = ag.media_from_velocity(*vel)
+        arrays = dict(init.arrays)
+        for k, v in media.items():
+            arrays[k] = np.ascontiguousarray(v, dtype="<f4")
+        solver.update_media(**{k: arrays[k] for k in media})
+        solver.reset(0); solver.run(init.nt); solver.sync()
+        return sum(0.5 * float(np.sum((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+                   for a, b in zip(solver.records(), obs))
+    errs = []
+    eps = 1e-3
+    for _ in range(args.fd_dirs):
+        d = [rng.standard_normal(v.shape) * v for v in vel0]
+        fd = (loss_at([v + eps * dv for v, dv in zip(vel0, d)]) - loss_at([v - eps * dv for v, dv in zip(vel0, d)])) / (2 * eps)
+        an = sum(float(np.sum(g[n] * dv)) for n, dv in zip(names, d))
+        errs.append(abs(an - fd) / abs(fd))
+    # FP64 host adjoint on the 201x101 cut
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cut_true = modelgen.preset("cut201")
+    cut = fwi.smoothed_model(cut_true, 100.0, 5.0)
+    o = pyoracle.Oracle(cut_true); o.reset(0); o.run(cut.nt)
+    obs_c = o.records()
+    oi = pyoracle.Oracle(cut); oi.reset(0); oi.run(cut.nt)
+    res = [a - b for a, b in zip(oi.records(), obs_c)]
+    go = oi.gradient(0, cut.nt, res, every=0 or 25)
+    sc = msfdm.Solver(cut)
+    gs = sc.gradient(0, cut.nt, [r.astype(np.float32) for r in res])
+    par = {n: float(np.linalg.norm(gs[n] - go[n]) / np.linalg.norm(go[n])) for n in msfdm._lib.GRADS}
+    out = {"metric": "adjoint gradient (vp, vs, rho) of the L2 record misfit, 401x201 flat seafloor, 3000 steps",
+           "value": round(1.0 / t_grad, 4), "unit": "gradients/s (1 shot: checkpointed forward + adjoint)",
+           "gradient_s": round(t_grad, 3), "forward_s": round(t_fwd, 3),
+           "gradient_over_forward": round(t_grad / t_fwd, 2), "misfit": loss,
+           "fd_check": {"directions": args.fd_dirs, "eps_rel": eps, "max_rel_err": max(errs),
+                        "median_rel_err": float(np.median(errs))},
+           "fp64_oracle_cut201": {"steps": cut.nt, "rel_l2_per_array": par}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +162,8 @@ def main():
     ap.add_argument("--workload", default="seafloor2001")
     ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--kernel-reps", type=int, default=100)
-    ap.add_argument("--mode", default="solver", choices=["solver", "batch"])
+    ap.add_argument("--mode", default="solver", choices=["solver", "batch", "gradient"])
+    ap.add_argument("--fd-dirs", type=int, default=16)
     ap.add_argument("--batch-shots", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -115,6 +181,9 @@ def main():
         return
     if args.mode == "batch":
         bench_batch(args)
+        return
+    if args.mode == "gradient":
+        bench_gradient(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

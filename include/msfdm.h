@@ -160,6 +160,12 @@ int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
  * fastest and resets the shot (no reference counterpart). */
 int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
 
+/* Replace the medium arrays of an existing solver (each [rows][nx] float32 in
+ * the model-file layout; NULL keeps an array).  Takes effect for the next
+ * steps; the wavefield is left untouched. */
+int msfdm_update_media(msfdm_solver *s, const float *K, const float *rho_f, const float *C11, const float *C13,
+                       const float *C33, const float *C55, const float *rho_s);
+
 /* --- Discrete adjoint (no reference counterpart) ------------------------- */
 /* Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
  * arrays for solid vx, solid vz, fluid p as returned by msfdm_get_records) with

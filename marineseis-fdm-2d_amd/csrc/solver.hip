@@ -1222,3 +1222,29 @@ extern "C" int msfdm_adjoint_get_output(msfdm_solver *s, int field, float *out) 
     HIPCHK(hipSetDevice(s->device));
     return copy_field(s, s->adj[A_TBA], field, out, nullptr);
 }
+
+extern "C" int msfdm_update_media(msfdm_solver *s, const float *K, const float *rho_f, const float *C11,
+                                  const float *C13, const float *C33, const float *C55, const float *rho_s) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int nx = s->d.nx, P = s->geo.P, nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
+    struct Item { const float *src; const float *dst; int rows; bool invert; } items[] = {
+        {K, s->coef.Kf, nzf, false}, {rho_f, s->coef.rhof_inv, nzf, true}, {C11, s->coef.C11, nzs, false},
+        {C13, s->coef.C13, nzs, false}, {C33, s->coef.C33, nzs, false}, {C55, s->coef.C55, nzs, false},
+        {rho_s, s->coef.rhos_inv, nzs, true}};
+    std::vector<float> h;
+    for (const Item &it : items) {
+        if (!it.src) continue;
+        h.assign((size_t)it.rows * P, 0.0f);
+        for (int r = 0; r < it.rows; ++r)
+            for (int x = 0; x < nx; ++x) {
+                const float v = it.src[(size_t)r * nx + x];
+                if (it.invert && !(v > 0.0f)) return set_error(MSFDM_EINVAL, "densities must be positive");
+                h[(size_t)r * P + x] = it.invert ? (float)(1.0 / (double)v) : v;
+            }
+        HIPCHK(hipMemcpy(const_cast<float *>(it.dst), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return MSFDM_OK;
+}

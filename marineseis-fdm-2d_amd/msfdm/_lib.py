@@ -39,6 +39,7 @@ EXPORTS = [
     "msfdm_batch_set_graph", "msfdm_batch_time_run", "msfdm_batch_autotune",
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
+    "msfdm_update_media",
 ]
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
 
@@ -117,6 +118,7 @@ def lib():
         "msfdm_adjoint_set_input": ([vp, i, fp], i),
         "msfdm_adjoint_apply": ([vp, i, i], i),
         "msfdm_adjoint_get_output": ([vp, i, fp], i),
+        "msfdm_update_media": ([vp] + [fp] * 7, i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
@@ -287,6 +289,15 @@ class Solver:
 
     def set_graph(self, enable: bool):
         check(lib().msfdm_set_graph(self.h, int(bool(enable))))
+
+    def update_media(self, **arrays):
+        """Replace medium arrays (K, rho_f, C11, C13, C33, C55, rho_s) in place."""
+        keep = {k: _f32(v) for k, v in arrays.items()}
+        unknown = set(keep) - set(GRADS)
+        if unknown:
+            raise KeyError(f"unknown medium arrays {sorted(unknown)}")
+        ptrs = [(_fptr(keep[n]) if n in keep else None) for n in GRADS]
+        check(lib().msfdm_update_media(self.h, *ptrs))
 
     # --- adjoint ------------------------------------------------------------
     def _grad_arrays(self):
