@@ -113,7 +113,9 @@ __device__ __forceinline__ void fluid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
         const int iz = z0 + ty + 4 * j;
         if (!active_col || iz > nzf - 1) continue;
         const long long gi = (long long)iz * P + ix;
-        const float J = c.J[gi], xx = c.xix[gi], xz = c.xiz[gi], zx = c.zex[gi], zz = c.zez[gi];
+        const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
+        float xx, xz;
+        xi_metrics(c, gi, xx, xz);
         const float gxixi = xx * xx + xz * xz;
         const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
         const bool at_row = dist_flso == 0;
@@ -155,8 +157,10 @@ __device__ __forceinline__ void fluid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
                 const float w = drp_w<DIR>(q);
                 const long long gq = gi + m;
                 const float mJp = -c.J[gq] * Sp[gq];
-                sx += w * mJp * c.xix[gq];
-                sz += w * mJp * c.xiz[gq];
+                float xxq, xzq;
+                xi_metrics(c, gq, xxq, xzq);
+                sx += w * mJp * xxq;
+                sz += w * mJp * xzq;
                 const int rr = iz + m;
                 float bx, bz;
                 if (at_if) {
@@ -184,7 +188,9 @@ __device__ __forceinline__ void fluid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
                 const int m = o + q;
                 const float w = drp_w<DIR>(q);
                 const long long gq = gi + m;
-                add(2, iz, ix + m, -c.J[gq] * (c.xix[gq] * w * sxb + c.xiz[gq] * w * szb));
+                float xxq, xzq;
+                xi_metrics(c, gq, xxq, xzq);
+                add(2, iz, ix + m, -c.J[gq] * (xxq * w * sxb + xzq * w * szb));
                 const int rr = iz + m;
                 const float fbx = w * sxb, fbz = w * szb;
                 if (at_if) {
@@ -270,7 +276,9 @@ __device__ __forceinline__ void solid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
         const int iz = z0 + ty + 4 * j;
         if (!active_col || iz > nzs - 4) continue;
         const long long si = (long long)iz * P + ix, gi = moff + si;
-        const float J = c.J[gi], xx = c.xix[gi], xz = c.xiz[gi], zx = c.zex[gi], zz = c.zez[gi];
+        const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
+        float xx, xz;
+        xi_metrics(c, gi, xx, xz);
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
         const bool side = scol >= 0, bottom = iz >= zb0;
         const bool at_row = iz == 0;
@@ -378,7 +386,9 @@ __device__ __forceinline__ void solid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
                     const int m = o + q;
                     const float w = drp_w<DIR>(q);
                     const long long gq = gi + m, sq = si + m;
-                    const float Jq = c.J[gq], xxq = c.xix[gq], xzq = c.xiz[gq];
+                    const float Jq = c.J[gq];
+                    float xxq, xzq;
+                    xi_metrics(c, gq, xxq, xzq);
                     sx += w * Jq * (xxq * Ssx[sq] + xzq * Ssxz[sq]);
                     sz += w * Jq * (xxq * Ssxz[sq] + xzq * Ssz[sq]);
                     const int rr = iz + m;
@@ -406,7 +416,9 @@ __device__ __forceinline__ void solid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
                     const int m = o + q;
                     const float w = drp_w<DIR>(q);
                     const long long gq = gi + m;
-                    const float Jq = c.J[gq], xxq = c.xix[gq], xzq = c.xiz[gq];
+                    const float Jq = c.J[gq];
+                    float xxq, xzq;
+                    xi_metrics(c, gq, xxq, xzq);
                     const float Fx = w * sxb, Fz = w * szb;
                     add(2, iz, ix + m, Jq * xxq * Fx);
                     add(4, iz, ix + m, Jq * (xzq * Fx + xxq * Fz));

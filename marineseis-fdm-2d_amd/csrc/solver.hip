@@ -406,6 +406,14 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
         nif[2 * ix] = (float)(zx / n);
         nif[2 * ix + 1] = (float)(zz / n);
     }
+    // vertical grid columns: xi_z == 0 and xi_x constant everywhere
+    {
+        const std::vector<float> &X = m->a[MSFDM_A_XI_X], &Z = m->a[MSFDM_A_XI_Z];
+        bool vert = !X.empty();
+        for (size_t i = 0; i < X.size() && vert; ++i) vert = (Z[i] == 0.0f) && (X[i] == X[0]);
+        c.vert = vert ? 1 : 0;
+        c.xix0 = X.empty() ? 0.0f : X[0];
+    }
     const float *nifp = upload(nif, 2 * nX);
     c.nif = reinterpret_cast<const float2 *>(nifp);
     ptrs[np++] = nifp;
@@ -429,9 +437,11 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // memory fields per active PML direction.
     const double cf = (double)(nzf - 3) * (nx - 6), cs = (double)(nzs - 3) * (nx - 6);
     const double pmlf = (double)(nzf - 3) * 2 * npd, pmls = (double)(nzs - 3) * 2 * npd + (double)npd * (nx - 6);
-    s->bytes_step = 4.0 * (cf * (13 * 3 + 4 * 6) + cs * (13 * 5 + 4 * 9) + (pmlf + pmls) * 13 * 4);
+    // Vertical-column grids read neither xi_x nor xi_z: 2 coefficients fewer.
+    const double ncf = c.vert ? 4 : 6, ncs = c.vert ? 7 : 9;
+    s->bytes_step = 4.0 * (cf * (13 * 3 + 4 * ncf) + cs * (13 * 5 + 4 * ncs) + (pmlf + pmls) * 13 * 4);
     // stage-2 launch: 3 state floats per field + coefficients (+3 per PML memory field)
-    s->bytes_kernel[0] = 4.0 * (cf * (3 * 3 + 6) + cs * (3 * 5 + 9) + (pmlf + pmls) * 3 * 4);
+    s->bytes_kernel[0] = 4.0 * (cf * (3 * 3 + ncf) + cs * (3 * 5 + ncs) + (pmlf + pmls) * 3 * 4);
     s->bytes_kernel[1] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);          // k, U, out, normal
     s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 6 + 2);          // k, U^n, T2, T3, T4, out, normal
     CK(hipStreamSynchronize(s->stream));

@@ -47,6 +47,7 @@ constexpr int TX = 64;          // tile width = one wavefront
 constexpr int HALO = 3;
 constexpr int LXW = TX + 2 * HALO;  // 70 loaded columns
 constexpr int LXP = 72;         // LDS row pitch (floats)
+constexpr int XOFF = 4;         // LDS column of the tile's first column (16-byte aligned interior)
 constexpr int NTHR = 256;       // 64 x 4 threads per stage block
 
 struct Geo {
@@ -61,7 +62,16 @@ struct Coef {
     const float *pxa, *pxb, *pxc;                        // xi PML: al+d/be, d/be, 1/be  [nx]
     const float *pza, *pzb, *pzc;                        // solid zeta PML               [nzs]
     const float2 *nif;                                   // interface normal            [nx]
+    // Vertical grid columns (x = h xi everywhere): xi_x is the constant xix0 and
+    // xi_z = 0, so neither array is read.
+    int vert;
+    float xix0;
 };
+
+__device__ __forceinline__ void xi_metrics(const Coef &c, long long gi, float &xx, float &xz) {
+    if (c.vert) { xx = c.xix0; xz = 0.0f; }
+    else { xx = c.xix[gi]; xz = c.xiz[gi]; }
+}
 
 struct Ctl {
     const float *amp;     // Ricker amplitude per (step, stage): [4 * cap]
@@ -118,6 +128,16 @@ __device__ __forceinline__ float mac_dz(const float (*t)[LXP], int r, int c, int
 // floats per field and step; the rounding of U' is no larger than that of the
 // accumulator form.  s is the stage state at the cell (U in stage 1, T4 in
 // stage 4); STAGE 0 writes the raw k.
+#ifndef MSFDM_NT_STORE
+#define MSFDM_NT_STORE 0   // 1: stage outputs bypass the caches (non-temporal stores)
+#endif
+__device__ __forceinline__ void store_out(float *p, float v) {
+#if MSFDM_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 template <int STAGE>
 __device__ __forceinline__ void rk(const float *__restrict__ U, const float *__restrict__ T2,
                                    const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
@@ -125,12 +145,12 @@ __device__ __forceinline__ void rk(const float *__restrict__ U, const float *__r
     if constexpr (STAGE == 0) {
         O[i] = k;
     } else if constexpr (STAGE == 1) {
-        O[i] = s + ct * k;
+        store_out(O + i, s + ct * k);
     } else if constexpr (STAGE == 4) {
         const float u = U[i];
-        O[i] = u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k;
+        store_out(O + i, u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k);
     } else {
-        O[i] = U[i] + ct * k;
+        store_out(O + i, U[i] + ct * k);
     }
 }
 
@@ -151,20 +171,32 @@ __constant__ float FILT_W[7] = {   // GAUSS_W7_PML, src/elements.cu:30-33
     1.1e-13f, 1.678e-06f, 0.03419f, 0.93163f, 0.03419f, 1.678e-06f, 1.1e-13f};
 
 // Load NF fields of the stage state (rows z0-3 .. z0+TZ+2, columns x0-3 .. x0+66)
-// into LDS; cells outside [0,nrows) x [0,P) read as 0.
+// into LDS (tile column x0 at LDS column XOFF); cells outside [0,nrows) x [0,P)
+// read as 0.  The 64 interior columns move as 16-byte vectors (x0, the pitch P
+// and every field offset are multiples of 64 floats), the 2 x 3 halo columns as
+// single floats.
 template <int NF, int TZ>
 __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const float *__restrict__ S,
                                           const long long *off, int f0, int z0, int x0, int nrows, int P) {
-    constexpr int N = (TZ + 2 * HALO) * LXW;
+    constexpr int LZ = TZ + 2 * HALO;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const float *src = S + off[f0 + f];
-        for (int e = threadIdx.x; e < N; e += NTHR) {
-            const int r = e / LXW, c = e - r * LXW;
-            const int gz = z0 - HALO + r, gx = x0 - HALO + c;
+        for (int e = threadIdx.x; e < LZ * (TX / 4); e += NTHR) {
+            const int r = e / (TX / 4), q = e - r * (TX / 4);
+            const int gz = z0 - HALO + r;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gz >= 0 && gz < nrows) v = *reinterpret_cast<const float4 *>(src + (long long)gz * P + x0 + 4 * q);
+            *reinterpret_cast<float4 *>(&sh[f][r][XOFF + 4 * q]) = v;
+        }
+        for (int e = threadIdx.x; e < LZ * 2 * HALO; e += NTHR) {
+            const int r = e / (2 * HALO), h = e - r * (2 * HALO);
+            const int gz = z0 - HALO + r;
+            const int dx = h < HALO ? h - HALO : TX + (h - HALO);
+            const int gx = x0 + dx;
             float v = 0.0f;
             if (gz >= 0 && gz < nrows && gx >= 0 && gx < P) v = src[(long long)gz * P + gx];
-            sh[f][r][c] = v;
+            sh[f][r][XOFF + dx] = v;
         }
     }
 }
@@ -188,7 +220,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     if (ix < 3 || ix > nx - 4) return;
     const float amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
     const int scol = side_col(ix, nx, g.npd);
-    const int cc = tx + HALO;
+    const int cc = tx + XOFF;
     const float (*shp)[LXP] = sh[2];
     const float (*shvx)[LXP] = sh[0];
     const float (*shvz)[LXP] = sh[1];
@@ -201,7 +233,9 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         if (iz > nzf - 1) break;
         const int r = lz + HALO;
         const long long gi = (long long)iz * P + ix;
-        const float xx = c.xix[gi], xz = c.xiz[gi], zx = c.zex[gi], zz = c.zez[gi];
+        float xx, xz;
+        xi_metrics(c, gi, xx, xz);
+        const float zx = c.zex[gi], zz = c.zez[gi];
         const float gxixi = xx * xx + xz * xz;
         const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
 
@@ -223,8 +257,10 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 const int m = o + q;
                 const long long gq = gi + m;
                 const float mJp = -c.J[gq] * shp[r][cc + m];
-                ax[q] = mJp * c.xix[gq];
-                az[q] = mJp * c.xiz[gq];
+                float xxq, xzq;
+                xi_metrics(c, gq, xxq, xzq);
+                ax[q] = mJp * xxq;
+                az[q] = mJp * xzq;
                 const int rr = iz + m;
                 if (at_if) {
                     if (rr <= nzf - 1) fze_row(rr, bx[q], bz[q]);
@@ -335,7 +371,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
     const int ix = x0 + tx;
     if (ix < 3 || ix > nx - 4) return;
     const int scol = side_col(ix, nx, npd);
-    const int cc = tx + HALO;
+    const int cc = tx + XOFF;
     const float (*svx)[LXP] = sh[0];
     const float (*svz)[LXP] = sh[1];
     const float (*ssx)[LXP] = sh[2];
@@ -353,7 +389,9 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
         const int r = lz + HALO;
         const long long si = (long long)iz * P + ix;
         const long long gi = moff + si;
-        const float xx = c.xix[gi], xz = c.xiz[gi], zx = c.zex[gi], zz = c.zez[gi];
+        float xx, xz;
+        xi_metrics(c, gi, xx, xz);
+        const float zx = c.zex[gi], zz = c.zez[gi];
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
         const bool side = scol >= 0, bottom = iz >= zb0;
 
@@ -385,7 +423,9 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             for (int q = 0; q < 5; ++q) {
                 const int m = o + q;
                 const long long gq = gi + m;
-                const float Jq = c.J[gq], xxq = c.xix[gq], xzq = c.xiz[gq];
+                const float Jq = c.J[gq];
+                float xxq, xzq;
+                xi_metrics(c, gq, xxq, xzq);
                 ax[q] = Jq * (xxq * ssx[r][cc + m] + xzq * ssxz[r][cc + m]);
                 az[q] = Jq * (xxq * ssxz[r][cc + m] + xzq * ssz[r][cc + m]);
                 const int rr = iz + m;

@@ -55,6 +55,45 @@ def test_rhs_matches_oracle_on_random_state(preset, direction, tile_rows):
         assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
 
 
+def _general_metrics(m, seed=1):
+    """Copy of m with xi_x, xi_z varying in space (exercises the general-metric
+    path; the oracle reads the same arrays)."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    a = dict(m.arrays)
+    nz, nx = a["xi_x"].shape
+    zz, xx = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    a["xi_x"] = (a["xi_x"] * (1.0 + 0.03 * np.sin(xx / 9.0) * np.cos(zz / 7.0))).astype(np.float32)
+    a["xi_z"] = (0.02 * a["xi_x"] * np.sin(zz / 11.0 + xx / 13.0)).astype(np.float32)
+    return dataclasses.replace(m, arrays=a)
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_rhs_general_metrics(direction):
+    m = _general_metrics(modelgen.preset("tiny_curvi"))
+    o = pyoracle.Oracle(m)
+    s = msfdm.Solver(m)
+    st = _random_state(m, 21 + direction)
+    for name, a in st.items():
+        a32 = a.astype(np.float32)
+        o.set(name, a32.astype(np.float64))
+        s.set(name, a32)
+    o.rhs(direction, 0.3)
+    s.eval_rhs(direction, 0.3)
+    for name in msfdm.FIELDS:
+        ko = o.get_k(name)
+        kg = s.get_rhs(name).astype(np.float64)
+        scale = np.abs(ko).max()
+        assert np.abs(kg - ko).max() <= 2e-5 * max(scale, 1e-30), name
+
+
+def test_records_general_metrics():
+    m = _general_metrics(modelgen.preset("tiny_curvi"))
+    o, s = _run_both(m, 200)
+    for a, b in zip(s.records(), o.records()):
+        assert rel_l2(a, b) <= RECORD_TOL
+
+
 def _run_both(m, nsteps, isrc=0, graph=True, tile_rows=None):
     o = pyoracle.Oracle(m)
     o.reset(isrc)
