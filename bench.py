@@ -46,6 +46,18 @@ def cpu_baseline(model, nsteps: int):
                       f"(after 2 untimed steps), {dt:.2f} s wall"}
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a plain device copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload}.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else round(k["traffic_bytes"])
+
+
 def bench_batch(args):
     """64 shot windows of uneven size: one batched run vs the same shots one at a
     time with the single-shot solver, both on `args.batch_steps` steps per shot
@@ -169,6 +181,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5) to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots shard across GPUs with one gradient all-reduce; not built here)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 4
@@ -185,10 +201,6 @@ def main():
     if args.mode == "gradient":
         bench_gradient(args)
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(shots shard across GPUs with one gradient all-reduce; not built here)", file=sys.stderr)
-        sys.exit(2)
 
     import msfdm
     from msfdm import modelgen
@@ -248,8 +260,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
-            "kernel": "stage_kernel<F, stage 2> (fluid + solid tiles)",
+            "traffic": measured_traffic(args.workload, f"stage_kernel<0, 2, {s.tile_rows}>"),
+            "kernel": f"stage_kernel<0, 2, {s.tile_rows}> (RK stage 2, F direction, fluid + solid tiles)",
             "alg_bytes_per_launch": b_stage,
         },
         "cpu_baseline": cpu,
