@@ -207,6 +207,7 @@ def main():
 
     model = modelgen.preset(args.workload)
     s = msfdm.Solver(model)
+    s.autotune(64)   # tile height from the library's autotuner (8 or 16 rows)
     s.reset(0)
     s.run(args.warmup)
     s.sync()

@@ -183,7 +183,7 @@ void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, fl
     if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, T2, T3, O, j, ct, amp);
     else launch_media<DIR, STAGE, 16>(s, S, U, T2, T3, O, j, ct, amp);
     if constexpr (STAGE == 4) {
-        D::iface_final<DIR><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, T2, T3,
+        D::iface_final<DIR><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, T2, T3,
                                                                                     O, s->kif, ct, j);
     } else {
         D::iface_stage<DIR, STAGE><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, S, U, O, s->kif, ct);
@@ -428,7 +428,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
 
     // Tile heights: small grids need more blocks to fill 256 CUs.
     const long long cells = (long long)nx * nz;
-    s->tz = cells < 1000000 ? 8 : 16;
+    s->tz = 8;
 
     // Algorithmic bytes per step: 13 state floats per field (stage 1: read U,
     // write T2; stages 2-3: read stage state + U, write next; stage 4: read
@@ -625,7 +625,7 @@ extern "C" int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms
             default: {
                 D::Ctl quiet = s->ctl;   // no record slot is written while timing
                 quiet.gap = 0x7fffffff;
-                D::iface_final<0><<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, T4, T2, T3, U, s->kif, dt / 6, 1);
+                D::iface_final<0><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, T4, T2, T3, U, s->kif, dt / 6, 1);
                 break;
             }
         }
@@ -748,7 +748,7 @@ void batch_launch_stage(msfdm_batch *b, int rS, int rO, int j, float ct) {
     else batch_launch_media<DIR, STAGE, 16>(b, rS, rO, j, ct);
     const int ns = (int)b->shots.size();
     if constexpr (STAGE == 4) {
-        D::batch_iface_final<DIR><<<dim3((b->maxnx + D::FB - 1) / D::FB, ns), D::FB, 0, b->stream>>>(b->d_shots, ct, j);
+        D::batch_iface_final<DIR><<<dim3((b->maxnx + D::FO - 1) / D::FO, ns), D::FB, 0, b->stream>>>(b->d_shots, ct, j);
     } else {
         D::batch_iface_stage<DIR, STAGE><<<dim3((b->maxnx + 127) / 128, ns), 128, 0, b->stream>>>(b->d_shots, rS, rO, ct);
     }
@@ -824,7 +824,7 @@ extern "C" int msfdm_batch_create(const msfdm_model *const *models, const int *s
 #undef BCK
     long long cells = 0;
     for (auto *s : b->shots) cells += (long long)s->d.nx * (s->d.nz_fluid + s->d.nz_solid - 1);
-    b->tz = cells < 1000000 ? 8 : 16;
+    b->tz = 8;
     if (int rc = batch_upload(b)) return fail(rc);
     *out = b;
     return MSFDM_OK;

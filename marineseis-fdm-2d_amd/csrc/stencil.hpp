@@ -154,6 +154,52 @@ __device__ __forceinline__ void rk(const float *__restrict__ U, const float *__r
     }
 }
 
+// rk with U[i] already in a register (loaded ahead of the row's arithmetic);
+// stage 4 loads it here unless MSFDM_PREFETCH_U4.
+#ifndef MSFDM_PREFETCH_U4
+#define MSFDM_PREFETCH_U4 0
+#endif
+template <int STAGE>
+__device__ __forceinline__ constexpr bool prefetch_u() { return STAGE == 2 || STAGE == 3 || (STAGE == 4 && MSFDM_PREFETCH_U4); }
+template <int STAGE>
+__device__ __forceinline__ void rk_u(const float *__restrict__ U, const float *__restrict__ T2,
+                                     const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
+                                     float k, float ct, float u) {
+    if constexpr (STAGE == 0) {
+        O[i] = k;
+    } else if constexpr (STAGE == 1) {
+        store_out(O + i, s + ct * k);
+    } else if constexpr (STAGE == 4) {
+        if constexpr (!prefetch_u<STAGE>()) u = U[i];
+        store_out(O + i, u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k);
+    } else {
+        store_out(O + i, u + ct * k);
+    }
+}
+
+// Pointwise inputs of one row of a thread, loaded one row ahead of their use so
+// that their latency overlaps the previous row's arithmetic (and, for the first
+// row, the tile load).  u: U^n of the row's state fields (stages 2-4).
+template <int NF, int NC>
+struct RowPre {
+    float xx, xz, zx, zz;
+    float cf[NC];
+    float u[NF];
+};
+
+template <int STAGE, int NF, int NC>
+__device__ __forceinline__ void prefetch_row(RowPre<NF, NC> &p, const Coef &c, const float *const (&cp)[NC],
+                                             const float *__restrict__ U, const long long *off, int f0,
+                                             long long ai, long long gi) {
+    xi_metrics(c, gi, p.xx, p.xz);
+    p.zx = c.zex[gi];
+    p.zz = c.zez[gi];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) p.cf[q] = cp[q][ai];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p.u[f] = prefetch_u<STAGE>() ? U[off[f0 + f] + ai] : 0.0f;
+}
+
 // Compact PML-strip column of ix (left strip first), or -1.
 __device__ __forceinline__ int side_col(int ix, int nx, int npd) {
     if (ix - 3 < npd) return ix - 3;
@@ -179,24 +225,49 @@ template <int NF, int TZ>
 __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const float *__restrict__ S,
                                           const long long *off, int f0, int z0, int x0, int nrows, int P) {
     constexpr int LZ = TZ + 2 * HALO;
+    constexpr int NV = LZ * (TX / 4);              // float4 interior elements per field
+    constexpr int IV = (NV + NTHR - 1) / NTHR;     // ... per thread
+    constexpr int NH = LZ * 2 * HALO;              // halo floats per field
+    constexpr int IH = (NH + NTHR - 1) / NTHR;
+    // every global load of the tile is issued before the first LDS store
+    float4 v[NF][IV];
+    float hv[NF][IH];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const float *src = S + off[f0 + f];
-        for (int e = threadIdx.x; e < LZ * (TX / 4); e += NTHR) {
+#pragma unroll
+        for (int i = 0; i < IV; ++i) {
+            const int e = threadIdx.x + i * NTHR;
             const int r = e / (TX / 4), q = e - r * (TX / 4);
             const int gz = z0 - HALO + r;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gz >= 0 && gz < nrows) v = *reinterpret_cast<const float4 *>(src + (long long)gz * P + x0 + 4 * q);
-            *reinterpret_cast<float4 *>(&sh[f][r][XOFF + 4 * q]) = v;
+            v[f][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (e < NV && gz >= 0 && gz < nrows)
+                v[f][i] = *reinterpret_cast<const float4 *>(src + (long long)gz * P + x0 + 4 * q);
         }
-        for (int e = threadIdx.x; e < LZ * 2 * HALO; e += NTHR) {
+#pragma unroll
+        for (int i = 0; i < IH; ++i) {
+            const int e = threadIdx.x + i * NTHR;
             const int r = e / (2 * HALO), h = e - r * (2 * HALO);
             const int gz = z0 - HALO + r;
+            const int gx = x0 + (h < HALO ? h - HALO : TX + (h - HALO));
+            hv[f][i] = 0.0f;
+            if (e < NH && gz >= 0 && gz < nrows && gx >= 0 && gx < P) hv[f][i] = src[(long long)gz * P + gx];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int i = 0; i < IV; ++i) {
+            const int e = threadIdx.x + i * NTHR;
+            const int r = e / (TX / 4), q = e - r * (TX / 4);
+            if (e < NV) *reinterpret_cast<float4 *>(&sh[f][r][XOFF + 4 * q]) = v[f][i];
+        }
+#pragma unroll
+        for (int i = 0; i < IH; ++i) {
+            const int e = threadIdx.x + i * NTHR;
+            const int r = e / (2 * HALO), h = e - r * (2 * HALO);
             const int dx = h < HALO ? h - HALO : TX + (h - HALO);
-            const int gx = x0 + dx;
-            float v = 0.0f;
-            if (gz >= 0 && gz < nrows && gx >= 0 && gx < P) v = src[(long long)gz * P + gx];
-            sh[f][r][XOFF + dx] = v;
+            if (e < NH) sh[f][r][XOFF + dx] = hv[f][i];
         }
     }
 }
@@ -214,28 +285,39 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = 3 + bz * TZ;
     const int P = g.P, nx = g.nx, nzf = g.nzf;
+    const int ix = x0 + tx;
+    const bool active = ix >= 3 && ix <= nx - 4;
+    const int scol = side_col(ix, nx, g.npd);
+    float amp = 0.0f, pxa = 0.f, pxb = 0.f, pxc = 0.f;
+    const float *const cp[2] = {c.rhof_inv, c.Kf};
+    RowPre<3, 2> nxt;
+    if (active) {
+        amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
+        if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
+        const long long a0 = (long long)min(z0 + ty, nzf - 1) * P + ix;
+        prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_FVX, a0, a0);
+    }
     load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
     __syncthreads();
-    const int ix = x0 + tx;
-    if (ix < 3 || ix > nx - 4) return;
-    const float amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
-    const int scol = side_col(ix, nx, g.npd);
+    if (!active) return;
     const int cc = tx + XOFF;
     const float (*shp)[LXP] = sh[2];
     const float (*shvx)[LXP] = sh[0];
     const float (*shvz)[LXP] = sh[1];
-    float pxa = 0.f, pxb = 0.f, pxc = 0.f;
-    if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
 
+#pragma unroll
     for (int j = 0; j < TZ / 4; ++j) {
         const int lz = ty + 4 * j;
         const int iz = z0 + lz;
         if (iz > nzf - 1) break;
+        const RowPre<3, 2> cur = nxt;
+        if (j + 1 < TZ / 4) {
+            const long long an = (long long)min(iz + 4, nzf - 1) * P + ix;
+            prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_FVX, an, an);
+        }
         const int r = lz + HALO;
         const long long gi = (long long)iz * P + ix;
-        float xx, xz;
-        xi_metrics(c, gi, xx, xz);
-        const float zx = c.zex[gi], zz = c.zez[gi];
+        const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz;
         const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
 
@@ -299,12 +381,12 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             Dx = -xx * t;
             Dz = -xz * t;
         }
-        const float rinv = c.rhof_inv[gi];
+        const float rinv = cur.cf[0];
         float kvx = rvx * rinv, kvz = rvz * rinv;
 
         // ---- pressure: dp/dt = -K div v (include/pml_fs_kernels.cuh:452-507)
         float kp = 0.0f, kEx = 0.0f, kEz = 0.0f;
-        const float Km = c.Kf[gi];
+        const float Km = cur.cf[1];
         float dvxx = 0.f, dvzx = 0.f, dvxz = 0.f, dvzz = 0.f;
         if (dist_fs != 0) {
             dvxx = dx_lds<DIR>(shvx[r], cc);
@@ -348,9 +430,9 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             kif[2 * (long long)P + ix] = kp;
             if constexpr (STAGE != 0) continue;
         }
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct);
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct);
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct, cur.u[0]);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct, cur.u[1]);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct, cur.u[2]);
     }
 }
 
@@ -366,32 +448,43 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
+    const int ix = x0 + tx;
+    const bool active = ix >= 3 && ix <= nx - 4;
+    const int scol = side_col(ix, nx, npd);
+    const long long moff = (long long)(g.nzf - 1) * P;   // solid row -> global metric row
+    float pxa = 0.f, pxb = 0.f, pxc = 0.f;
+    const float *const cp[5] = {c.C11, c.C13, c.C33, c.C55, c.rhos_inv};
+    RowPre<5, 5> nxt;
+    if (active) {
+        if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
+        const long long a0 = (long long)min(z0 + ty, nzs - 4) * P + ix;
+        prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_SVX, a0, moff + a0);
+    }
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
-    const int ix = x0 + tx;
-    if (ix < 3 || ix > nx - 4) return;
-    const int scol = side_col(ix, nx, npd);
+    if (!active) return;
     const int cc = tx + XOFF;
     const float (*svx)[LXP] = sh[0];
     const float (*svz)[LXP] = sh[1];
     const float (*ssx)[LXP] = sh[2];
     const float (*ssz)[LXP] = sh[3];
     const float (*ssxz)[LXP] = sh[4];
-    const long long moff = (long long)(g.nzf - 1) * P;   // solid row -> global metric row
-    float pxa = 0.f, pxb = 0.f, pxc = 0.f;
-    if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
     const int zb0 = nzs - 3 - npd;   // first bottom-PML row
 
+#pragma unroll
     for (int j = 0; j < TZ / 4; ++j) {
         const int lz = ty + 4 * j;
         const int iz = z0 + lz;
         if (iz > nzs - 4) break;
+        const RowPre<5, 5> cur = nxt;
+        if (j + 1 < TZ / 4) {
+            const long long an = (long long)min(iz + 4, nzs - 4) * P + ix;
+            prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_SVX, an, moff + an);
+        }
         const int r = lz + HALO;
         const long long si = (long long)iz * P + ix;
         const long long gi = moff + si;
-        float xx, xz;
-        xi_metrics(c, gi, xx, xz);
-        const float zx = c.zex[gi], zz = c.zez[gi];
+        const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
         const bool side = scol >= 0, bottom = iz >= zb0;
 
@@ -400,7 +493,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
         const float dvxz = mac_dz<DIR>(svx, r, cc, iz, true);
         const float dvzx = dx_lds<DIR>(svz[r], cc);
         const float dvzz = mac_dz<DIR>(svz, r, cc, iz, true);
-        const float c11 = c.C11[si], c13 = c.C13[si], c33 = c.C33[si], c55 = c.C55[si];
+        const float c11 = cur.cf[0], c13 = cur.cf[1], c33 = cur.cf[2], c55 = cur.cf[3];
         const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
         float ksx = c11 * exx + c13 * ezz;
         float ksz = c13 * exx + c33 * ezz;
@@ -459,7 +552,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             Dzx = zx * (b + gz * a) + zz * (gz * e + f);
             Dzz = zx * (gz * e + f) + zz * (gz * cz + d);
         }
-        const float rinv = c.rhos_inv[si];
+        const float rinv = cur.cf[4];
         float kvx = rvx * rinv, kvz = rvz * rinv;
 
         // ---- xi-PML memory (side strips)
@@ -515,11 +608,11 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             kif[7 * (long long)P + ix] = ksxz;
             if constexpr (STAGE != 0) continue;
         }
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct);
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct);
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct);
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct);
-        rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cur.u[0]);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cur.u[1]);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct, cur.u[2]);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cur.u[3]);
+        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cur.u[4]);
     }
 }
 
@@ -607,24 +700,32 @@ iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict_
     iface_stage_body<DIR, STAGE>(g, c, S, U, O, kif, ct, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Stage-4 interface rows + fl_so_filter (out of place) + receiver records for
-// the FB columns starting at x0.
+// Stage-4 interface rows + fl_so_filter (out of place) + receiver records.
+// A block of FB threads loads FB consecutive interface columns starting at
+// x0 - 3 (one per thread, all loads issued at once) and filters the FO = FB - 6
+// columns x0 .. x0 + FO - 1 that have their full 7-tap neighbourhood in LDS.
 constexpr int FB = 128;
+constexpr int FO = FB - 2 * HALO;
 template <int DIR>
-__device__ __forceinline__ void iface_final_body(float (*pre)[FB + 2 * HALO], const Geo &g, const Coef &c,
+__device__ __forceinline__ void iface_final_body(float (*pre)[FB], const Geo &g, const Coef &c,
                                                  const Ctl &ctl, const float *__restrict__ T4,
                                                  const float *__restrict__ T2, const float *__restrict__ T3,
                                                  float *__restrict__ Uo, const float *__restrict__ kif, float cb,
                                                  int jstep, int x0) {
     const int nx = g.nx;
-    for (int t = threadIdx.x; t < FB + 2 * HALO; t += FB) {
-        const int ix = x0 - HALO + t;
+    const int t = threadIdx.x;
+    const int ix = x0 - HALO + t;
+    const bool inner = ix >= 3 && ix <= nx - 4;
+    const int it = ctl.base[0] + jstep;
+    float2 n = make_float2(0.f, 0.f);
+    {
         float v[8];
-        if (ix >= 3 && ix <= nx - 4) {
+        if (inner) {
+            n = c.nif[ix];
             float k[8];
 #pragma unroll
             for (int f = 0; f < 8; ++f) k[f] = kif[(long long)f * g.P + ix];
-            interface_bc<DIR>(k, c.nif[ix]);
+            interface_bc<DIR>(k, n);
 #pragma unroll
             for (int f = 0; f < 8; ++f) {
                 const long long i = iface_index(g, f, ix);
@@ -639,9 +740,7 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB + 2 * HALO], co
         for (int f = 0; f < 8; ++f) pre[f][t] = v[f];
     }
     __syncthreads();
-    const int ix = x0 + threadIdx.x;
-    if (ix < 3 || ix > nx - 4) return;
-    const int t = threadIdx.x + HALO;
+    if (!inner || t < HALO || t >= FB - HALO) return;
     const int klo = max(-3, 3 - ix), khi = min(3, nx - 4 - ix);
     float wsum = 0.0f;
     for (int kk = klo; kk <= khi; ++kk) wsum += FILT_W[kk + 3];
@@ -652,7 +751,6 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB + 2 * HALO], co
         for (int kk = klo; kk <= khi; ++kk) s += FILT_W[kk + 3] * pre[f][t + kk];
         v[f] = s / wsum;
     }
-    const float2 n = c.nif[ix];
     const float n1 = n.x, n2 = n.y;
     const float vnf = n1 * v[0] + n2 * v[1], vtf = n2 * v[0] - n1 * v[1];
     const float vns = n1 * v[3] + n2 * v[4], vts = n2 * v[3] - n1 * v[4];
@@ -670,7 +768,6 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB + 2 * HALO], co
     o[7] = n1 * n2 * (sn - stt);
 #pragma unroll
     for (int f = 0; f < 8; ++f) Uo[iface_index(g, f, ix)] = o[f];
-    const int it = ctl.base[0] + jstep;
     if (it % ctl.gap == 0) {
         const int slot = it / ctl.gap;
         if (slot < ctl.rec_cap) {
@@ -688,8 +785,8 @@ __global__ void __launch_bounds__(FB)
 iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ T4, const float *__restrict__ T2,
             const float *__restrict__ T3, float *__restrict__ Uo, const float *__restrict__ kif, float cb,
             int jstep) {
-    __shared__ float pre[8][FB + 2 * HALO];
-    iface_final_body<DIR>(pre, g, c, ctl, T4, T2, T3, Uo, kif, cb, jstep, blockIdx.x * FB);
+    __shared__ float pre[8][FB];
+    iface_final_body<DIR>(pre, g, c, ctl, T4, T2, T3, Uo, kif, cb, jstep, blockIdx.x * FO);
 }
 
 // ---------------------------------------------------------------------------
@@ -736,11 +833,11 @@ batch_iface_stage(const Shot *__restrict__ shots, int rS, int rO, float ct) {
 template <int DIR>
 __global__ void __launch_bounds__(FB)
 batch_iface_final(const Shot *__restrict__ shots, float cb, int jstep) {
-    __shared__ float pre[8][FB + 2 * HALO];
+    __shared__ float pre[8][FB];
     const Shot &s = shots[blockIdx.y];
-    if ((int)blockIdx.x * FB > s.g.nx) return;   // whole block beyond this shot's grid
+    if ((int)blockIdx.x * FO > s.g.nx) return;   // whole block beyond this shot's grid
     iface_final_body<DIR>(pre, s.g, s.c, s.ctl, s.buf[3], s.buf[1], s.buf[2], s.buf[0], s.kif, cb, jstep,
-                          blockIdx.x * FB);
+                          blockIdx.x * FO);
 }
 
 __global__ void advance_base(int *base, int n) {
