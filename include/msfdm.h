@@ -166,6 +166,18 @@ int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
 int msfdm_update_media(msfdm_solver *s, const float *K, const float *rho_f, const float *C11, const float *C13,
                        const float *C33, const float *C55, const float *rho_s);
 
+/* --- Grid metrics (model generation) ------------------------------------ */
+/* FP64 curvilinear metric terms of the grid (x, z), each [nz][nx] row-major
+ * doubles in metres, computed on `device`: numpy.gradient(edge_order=2)
+ * derivatives in unit computational spacing and the inverse metric
+ * J = x_xi z_ze - x_ze z_xi, xi_x = z_ze/J, xi_z = -x_ze/J, ze_x = -z_xi/J,
+ * ze_z = x_xi/J.  Replaces the metric block of the reference's model
+ * generators (tools/run_models/model_ortho_classic_scholte.py:188-195);
+ * the outputs are what they write as jacob/{J,xi_x,xi_z,ze_x,ze_z}.bin
+ * after rounding to float32.  No solver is needed. */
+int msfdm_grid_metrics(const double *x, const double *z, int nz, int nx, int device, double *J, double *xi_x,
+                       double *xi_z, double *ze_x, double *ze_z);
+
 /* --- Discrete adjoint (no reference counterpart) ------------------------- */
 /* Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
  * arrays for solid vx, solid vz, fluid p as returned by msfdm_get_records) with

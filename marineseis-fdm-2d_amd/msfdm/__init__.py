@@ -4,7 +4,7 @@
 its C ABI; ``msfdm.modelgen`` builds and writes solver input models.
 """
 from ._lib import (ARRAYS, BIN_PATH, FID, FIELDS, LIB_PATH, NFIELD, Batch, HostModel, MsfdmError,
-                   Solver, last_error, lib, run_cli)
+                   Solver, grid_metrics, last_error, lib, run_cli)
 
 __all__ = ["ARRAYS", "BIN_PATH", "FID", "FIELDS", "LIB_PATH", "NFIELD", "Batch", "HostModel", "MsfdmError",
-           "Solver", "last_error", "lib", "run_cli"]
+           "Solver", "grid_metrics", "last_error", "lib", "run_cli"]

@@ -39,7 +39,7 @@ EXPORTS = [
     "msfdm_batch_set_graph", "msfdm_batch_time_run", "msfdm_batch_autotune",
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
-    "msfdm_update_media",
+    "msfdm_update_media", "msfdm_grid_metrics",
 ]
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
 
@@ -119,6 +119,7 @@ def lib():
         "msfdm_adjoint_apply": ([vp, i, i], i),
         "msfdm_adjoint_get_output": ([vp, i, fp], i),
         "msfdm_update_media": ([vp] + [fp] * 7, i),
+        "msfdm_grid_metrics": ([C.POINTER(C.c_double)] * 2 + [i, i, i] + [C.POINTER(C.c_double)] * 5, i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
@@ -143,6 +144,22 @@ def _f32(a):
 
 def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def grid_metrics(x, z, device: int = 0) -> dict:
+    """FP64 metric terms of the grid (x, z) ([nz][nx] metres) computed on the GPU:
+    numpy.gradient(edge_order=2) derivatives and the inverse metric.  Returns
+    float64 arrays J, xi_x, xi_z, ze_x, ze_z."""
+    xd = np.ascontiguousarray(x, dtype=np.float64)
+    zd = np.ascontiguousarray(z, dtype=np.float64)
+    if xd.ndim != 2 or xd.shape != zd.shape:
+        raise ValueError("x and z must be 2-D arrays of one shape")
+    nz, nx = xd.shape
+    out = {k: np.empty((nz, nx)) for k in ("J", "xi_x", "xi_z", "ze_x", "ze_z")}
+    dp = C.POINTER(C.c_double)
+    check(lib().msfdm_grid_metrics(xd.ctypes.data_as(dp), zd.ctypes.data_as(dp), nz, nx, int(device),
+                                   *[out[k].ctypes.data_as(dp) for k in ("J", "xi_x", "xi_z", "ze_x", "ze_z")]))
+    return out
 
 
 class HostModel:

@@ -17,8 +17,12 @@ Grid conventions (``include/elements.h:77-87``, ``docs/io.md:53-66``):
   undulation and ``w`` blends it in: 0 on the surface row, 1 on the interface
   row, 0 from the first bottom-PML row down (equal PML thickness everywhere,
   ``tools/run_models/model_ortho_classic_scholte.py:104-133``);
-* metrics are analytic: ``J = h z_zeta``, ``xi_x = 1/h``, ``xi_z = 0``,
-  ``ze_x = -z_xi / J``, ``ze_z = h / J`` (``model_FANDS_PANDF_wavy.py:130-144``).
+* metrics: ``metrics="analytic"`` (default) takes the closed-form derivatives of
+  ``z``: ``J = h z_zeta``, ``xi_x = 1/h``, ``xi_z = 0``, ``ze_x = -z_xi / J``,
+  ``ze_z = h / J`` (``model_FANDS_PANDF_wavy.py:130-144``); ``metrics="gpu"``
+  differentiates the grid coordinates on the GPU in FP64 with the reference's
+  ``numpy.gradient(edge_order=2)`` stencil (``model_ortho_classic_scholte.py:188-195``)
+  through ``msfdm.grid_metrics``.
 
 The solid may carry a seeded Gaussian random perturbation ``(1 + delta)`` applied to
 vp, vs and rho with vs/vp fixed; ``delta`` comes from a counter-based splitmix64
@@ -125,12 +129,28 @@ class Model:
         return self.nz_fluid + self.nz_solid - 1
 
 
+def grid_coordinates(nx: int, nz: int, h: float, water_depth: float, *, npd: int, seafloor=()):
+    """FP64 grid coordinates (x, z)[nz][nx] in metres of the body-fitted grid:
+    x = h xi, z = h (zeta - 3) + d(x) w(zeta) (the grid of ``build_model``)."""
+    zeta0 = int(round(water_depth / h)) + 3
+    w, _ = _blend(nz, zeta0, float(nz - 4 - npd))
+    x = h * np.arange(nx, dtype=np.float64)
+    d = np.zeros(nx)
+    for amp, lam in seafloor:
+        d += amp * np.sin(2 * np.pi * x / lam)
+    zeta = np.arange(nz, dtype=np.float64)
+    Z = h * (zeta[:, None] - 3.0) + d[None, :] * w[:, None]
+    X = np.broadcast_to(x[None, :], (nz, nx)).copy()
+    return X, Z
+
+
 def build_model(nx: int, nz: int, h: float, water_depth: float, *, npd: int, fpeak: float,
                 dt: float, nt: int, src_x, src_z=None, src_height: float | None = None,
                 seafloor=(), solid=(3000.0, 1700.0, 2200.0), fluid=(1500.0, 1000.0),
                 random_std: float = 0.0, corr_len: float = 50.0, seed: int = 0,
                 snapshot: int | None = None, ppw0: float = 8.0, rc: float = 1e-3,
-                npow: int = 2, vp_max: float | None = None) -> Model:
+                npow: int = 2, vp_max: float | None = None, metrics: str = "analytic",
+                device: int = 0) -> Model:
     """Build a fluid-over-solid model.
 
     ``seafloor`` is a list of ``(amplitude, wavelength)`` terms of the interface
@@ -155,11 +175,19 @@ def build_model(nx: int, nz: int, h: float, water_depth: float, *, npd: int, fpe
     dz_dze = h + d[None, :] * dw[:, None]
     if np.any(dz_dze <= 0.2 * h):
         raise ValueError("seafloor undulation folds the grid")
-    J = h * dz_dze
-    xi_x = np.full((nz, nx), 1.0 / h)
-    xi_z = np.zeros((nz, nx))
-    ze_x = -dz_dxi / J
-    ze_z = h / J
+    if metrics == "analytic":
+        J = h * dz_dze
+        xi_x = np.full((nz, nx), 1.0 / h)
+        xi_z = np.zeros((nz, nx))
+        ze_x = -dz_dxi / J
+        ze_z = h / J
+    elif metrics == "gpu":
+        from ._lib import grid_metrics
+        X, Z = grid_coordinates(nx, nz, h, water_depth, npd=npd, seafloor=seafloor)
+        gm = grid_metrics(X, Z, device)
+        J, xi_x, xi_z, ze_x, ze_z = (gm[k] for k in ("J", "xi_x", "xi_z", "ze_x", "ze_z"))
+    else:
+        raise ValueError(f"metrics must be 'analytic' or 'gpu', not {metrics!r}")
 
     vp_s, vs_s, rho_s = solid
     vp_f, rho_f = fluid
