@@ -197,6 +197,23 @@ int msfdm_adjoint_forward(msfdm_solver *s, int isrc, int nsteps, int every);
 int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, const float *res_vz, const float *res_p,
                            double *gK, double *grho_f, double *gC11, double *gC13, double *gC33, double *gC55,
                            double *grho_s);
+/* Multi-shot gradient: the sum over shots isrc[0..nshot) of the gradients of
+ * msfdm_gradient, with res = [nshot][3][nrec][nx] floats (per shot the three
+ * record planes, nrec = (nsteps - 1) / gap + 1).  The shots run in waves; the
+ * shots of a wave step together, forward (checkpointed every `every` steps;
+ * 0 picks the interval that needs the least memory) and adjoint, one launch
+ * per RK stage for the whole wave.  The wave size is the number of shots whose
+ * checkpoints and segment storage fit mem_bytes of device memory (0: 90% of
+ * the free memory) and is returned in *wave_shots (may be NULL).  Per-shot
+ * gradients are FP64 and are added in shot order, so the result equals the
+ * in-order sum of the single-shot gradients.  The solver's wavefield and step
+ * counter are left as they were. */
+int msfdm_gradient_shots(msfdm_solver *s, int nshot, const int *isrc, int nsteps, int every, double mem_bytes,
+                         const float *res, double *gK, double *grho_f, double *gC11, double *gC13, double *gC33,
+                         double *gC55, double *grho_s, int *wave_shots);
+/* Device bytes one shot of a msfdm_gradient_shots wave needs, and the
+ * checkpoint interval it would use. */
+int msfdm_gradient_shots_plan(const msfdm_solver *s, int nsteps, int every, double *bytes_per_shot, int *every_used);
 /* Operator probes: set an input field, apply op (0 = L_dir^T B_dir^T at the
  * current wavefield, 1 = Filter^T), read an output field. */
 int msfdm_adjoint_set_input(msfdm_solver *s, int field, const float *in);

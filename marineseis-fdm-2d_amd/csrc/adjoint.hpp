@@ -26,7 +26,15 @@ namespace dev {
 struct Grad {
     double *K, *rho_f;                 // [nzf][P]
     double *C11, *C13, *C33, *C55, *rho_s;   // [nzs][P]
+    long long gstride;                 // doubles between the gradients of consecutive shots
 };
+
+// Gradients of shot `shot` of a multi-shot launch (blockIdx.y).
+__device__ __forceinline__ Grad grad_of_shot(Grad g, int shot) {
+    const long long o = (long long)shot * g.gstride;
+    g.K += o; g.rho_f += o; g.C11 += o; g.C13 += o; g.C33 += o; g.C55 += o; g.rho_s += o;
+    return g;
+}
 
 // weights of the one-sided operators at offsets (ascending)
 template <int DIR> __device__ __forceinline__ float drp_w(int q) {   // q = 0..4 for offsets lo_off..lo_off+4
@@ -83,7 +91,19 @@ struct AdjIn {
     const float *qif;    // [8][P] interface qbar = B^T kbar
     float *TB;           // output Tbar (zeroed)
     float clam, cprev;
+    long long bstride;   // floats between the state buffers of consecutive shots
+    long long qstride;   // floats between the qif buffers of consecutive shots
 };
+
+// Inputs of shot `shot` of a multi-shot launch (blockIdx.y); every shot's
+// buffers sit at a fixed stride from shot 0's.
+__device__ __forceinline__ AdjIn adj_of_shot(AdjIn a, int shot) {
+    const long long o = (long long)shot * a.bstride;
+    a.S += o; a.LP += o; a.TB += o;
+    if (a.TBp) a.TBp += o;
+    a.qif += (long long)shot * a.qstride;
+    return a;
+}
 
 __device__ __forceinline__ float kbar(const AdjIn &a, long long i) {
     float v = a.clam * a.LP[i];
@@ -490,8 +510,10 @@ __device__ __forceinline__ void acc_flush(float (*acc)[TZ + 2 * HALO][LXP], floa
 
 template <int DIR, int TZ>
 __global__ void __launch_bounds__(NTHR)
-adj_stage_kernel(Geo g, Coef c, AdjIn a, Grad gr, int nbx, int nbz_f, int ntiles) {
+adj_stage_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int ntiles) {
     __shared__ float acc[5][TZ + 2 * HALO][LXP];
+    const AdjIn a = adj_of_shot(a0, blockIdx.y);
+    const Grad gr = grad_of_shot(gr0, blockIdx.y);
     const int per_xcd = gridDim.x >> 3;
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
@@ -527,7 +549,9 @@ __device__ __forceinline__ void bc_matrix(float2 n, float M[8][8]) {
 
 // qif = B^T kbar on the interface rows, kbar = clam * LP + cprev * TBp.
 template <int DIR>
-__global__ void __launch_bounds__(128) adj_iface_q(Geo g, Coef c, AdjIn a, float *qif) {
+__global__ void __launch_bounds__(128) adj_iface_q(Geo g, Coef c, AdjIn a0, float *qif) {
+    const AdjIn a = adj_of_shot(a0, blockIdx.y);
+    qif += (long long)blockIdx.y * a0.qstride;
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     if (ix < 3 || ix > g.nx - 4) return;
     float kb[8];
@@ -558,9 +582,12 @@ __device__ __forceinline__ void filter_map(float2 n, const float v[8], float o[8
 }
 
 // LP interface rows (columns 3..nx-4) = Filter^T(LAM interface rows).
-__global__ void __launch_bounds__(FB) adj_filter(Geo g, Coef c, const float *__restrict__ LAM, float *__restrict__ LP) {
+__global__ void __launch_bounds__(FB) adj_filter(Geo g, Coef c, const float *__restrict__ LAM, float *__restrict__ LP,
+                                                  long long bstride) {
     __shared__ float vb[8][FB + 2 * HALO];
     __shared__ float ws[FB + 2 * HALO];
+    LAM += (long long)blockIdx.y * bstride;
+    LP += (long long)blockIdx.y * bstride;
     const int nx = g.nx, x0 = blockIdx.x * FB;
     for (int t = threadIdx.x; t < FB + 2 * HALO; t += FB) {
         const int ix = x0 - HALO + t;
@@ -606,16 +633,31 @@ __global__ void __launch_bounds__(FB) adj_filter(Geo g, Coef c, const float *__r
     }
 }
 
+// total += sum over nshot per-shot gradient arrays of n doubles each (stride
+// gstride), added in shot order.
+__global__ void grad_sum_kernel(double *__restrict__ total, const double *__restrict__ g, long long n, long long gstride,
+                                int nshot) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double t = total[i];
+        for (int k = 0; k < nshot; ++k) t += g[(long long)k * gstride + i];
+        total[i] = t;
+    }
+}
+
 // ACC += TB over n floats.
 __global__ void axpy_kernel(float *__restrict__ acc, const float *__restrict__ tb, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         acc[i] += tb[i];
 }
 
-// Record residual injection into lambda at record slot r.
-__global__ void adj_inject(Geo g, float *__restrict__ LAM, const float *__restrict__ res, int nrec, int r) {
+// Record residual injection into lambda at record slot r (shot blockIdx.y:
+// lambda at bstride floats and residuals [3][nrec][nx] at rstride floats per shot).
+__global__ void adj_inject(Geo g, float *__restrict__ LAM, const float *__restrict__ res, int nrec, int r,
+                           long long bstride, long long rstride) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     if (ix >= g.nx) return;
+    LAM += (long long)blockIdx.y * bstride;
+    res += (long long)blockIdx.y * rstride;
     const long long plane = (long long)nrec * g.nx, o = (long long)r * g.nx + ix;
     LAM[g.off[MSFDM_SVX] + ix] += res[o];
     LAM[g.off[MSFDM_SVZ] + ix] += res[plane + o];

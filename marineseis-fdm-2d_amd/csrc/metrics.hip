@@ -11,9 +11,11 @@
 //   ze_x = -z_xi / J,           ze_z = x_xi / J.
 // For vertical grid columns (x = h xi) this is the reference's J = h z_ze,
 // xi_x = 1/h, xi_z = 0, ze_x = -z_xi / J, ze_z = h / J.  Every operation is
-// an explicitly rounded FP64 multiply / add / divide (no contraction), in
-// numpy's evaluation order, so the results are those of the same formulas
-// evaluated in numpy float64.
+// a separately rounded FP64 multiply / add / divide in numpy's evaluation
+// order, so the results are those of the same formulas evaluated in numpy
+// float64.  Contraction into FMAs is switched off for this file (pragma and
+// -ffp-contract=off): a fused multiply-add rounds once where numpy rounds twice,
+// which moves the cancelling one-sided end stencil by up to an ulp of z.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -26,32 +28,35 @@ namespace {
 
 __device__ __forceinline__ double grad_at(const double *__restrict__ f, long long stride, int i, int n) {
     // numpy.gradient(f, edge_order=2) along one axis with unit spacing, n >= 3
+#pragma clang fp contract(off)
     if (i == 0) {
-        const double a = __dmul_rn(-1.5, f[0]), b = __dmul_rn(2.0, f[stride]), c = __dmul_rn(-0.5, f[2 * stride]);
-        return __dadd_rn(__dadd_rn(a, b), c);
+        const double a = -1.5 * f[0], b = 2.0 * f[stride], c = -0.5 * f[2 * stride];
+        return (a + b) + c;
     }
     if (i == n - 1) {
-        const double a = __dmul_rn(0.5, f[-2 * stride]), b = __dmul_rn(-2.0, f[-stride]), c = __dmul_rn(1.5, f[0]);
-        return __dadd_rn(__dadd_rn(a, b), c);
+        const double a = 0.5 * f[-2 * stride], b = -2.0 * f[-stride], c = 1.5 * f[0];
+        return (a + b) + c;
     }
-    return __ddiv_rn(__dsub_rn(f[stride], f[-stride]), 2.0);
+    return (f[stride] - f[-stride]) / 2.0;
 }
 
 __global__ void __launch_bounds__(256)
 metric_kernel(const double *__restrict__ X, const double *__restrict__ Z, int nz, int nx, double *__restrict__ J,
               double *__restrict__ xix, double *__restrict__ xiz, double *__restrict__ zex, double *__restrict__ zez) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (i >= nx || k >= nz) return;
     const long long q = (long long)k * nx + i;
     const double x_xi = grad_at(X + q, 1, i, nx), z_xi = grad_at(Z + q, 1, i, nx);
     const double x_ze = grad_at(X + q, nx, k, nz), z_ze = grad_at(Z + q, nx, k, nz);
-    const double j = __dsub_rn(__dmul_rn(x_xi, z_ze), __dmul_rn(x_ze, z_xi));
+    const double p1 = x_xi * z_ze, p2 = x_ze * z_xi;
+    const double j = p1 - p2;
     J[q] = j;
-    xix[q] = __ddiv_rn(z_ze, j);
-    xiz[q] = __ddiv_rn(__dsub_rn(0.0, x_ze), j);
-    zex[q] = __ddiv_rn(__dsub_rn(0.0, z_xi), j);
-    zez[q] = __ddiv_rn(x_xi, j);
+    xix[q] = z_ze / j;
+    xiz[q] = (0.0 - x_ze) / j;
+    zex[q] = (0.0 - z_xi) / j;
+    zez[q] = x_xi / j;
 }
 
 #define MCHK(expr)                                                                                      \

@@ -735,39 +735,54 @@ int batch_upload(msfdm_batch *b) {
     return MSFDM_OK;
 }
 
+// What a batched launch sequence needs: the device shot table, the tile tables
+// (8- and 16-row tiles), the tile height in use and the widest grid.
+struct BatchView {
+    const D::Shot *shots;
+    const int4 *tiles[2];
+    int ntiles[2];
+    int tz, ns, maxnx;
+    int *base;
+    float dt;
+    hipStream_t stream;
+};
+BatchView view_of(const msfdm_batch *b) {
+    return BatchView{b->d_shots, {b->d_tiles[0], b->d_tiles[1]}, {b->ntiles[0], b->ntiles[1]}, b->tz,
+                     (int)b->shots.size(), b->maxnx, b->base, b->dt, b->stream};
+}
+
 template <int DIR, int STAGE, int TZ>
-void batch_launch_media(msfdm_batch *b, int rS, int rO, int j, float ct) {
+void batch_launch_media(const BatchView &b, int rS, int rO, int j, float ct) {
     const int k = TZ == 8 ? 0 : 1;
-    const int grid = 8 * ((b->ntiles[k] + 7) / 8);
-    D::batch_stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, b->stream>>>(b->d_shots, b->d_tiles[k], b->ntiles[k], rS,
-                                                                            rO, j, ct);
+    const int grid = 8 * ((b.ntiles[k] + 7) / 8);
+    D::batch_stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k], rS, rO, j,
+                                                                           ct);
 }
 template <int DIR, int STAGE>
-void batch_launch_stage(msfdm_batch *b, int rS, int rO, int j, float ct) {
-    if (b->tz == 8) batch_launch_media<DIR, STAGE, 8>(b, rS, rO, j, ct);
+void batch_launch_stage(const BatchView &b, int rS, int rO, int j, float ct) {
+    if (b.tz == 8) batch_launch_media<DIR, STAGE, 8>(b, rS, rO, j, ct);
     else batch_launch_media<DIR, STAGE, 16>(b, rS, rO, j, ct);
-    const int ns = (int)b->shots.size();
     if constexpr (STAGE == 4) {
-        D::batch_iface_final<DIR><<<dim3((b->maxnx + D::FO - 1) / D::FO, ns), D::FB, 0, b->stream>>>(b->d_shots, ct, j);
+        D::batch_iface_final<DIR><<<dim3((b.maxnx + D::FO - 1) / D::FO, b.ns), D::FB, 0, b.stream>>>(b.shots, ct, j);
     } else {
-        D::batch_iface_stage<DIR, STAGE><<<dim3((b->maxnx + 127) / 128, ns), 128, 0, b->stream>>>(b->d_shots, rS, rO, ct);
+        D::batch_iface_stage<DIR, STAGE><<<dim3((b.maxnx + 127) / 128, b.ns), 128, 0, b.stream>>>(b.shots, rS, rO, ct);
     }
 }
 template <int D1>
-void batch_step_dir(msfdm_batch *b, int j) {
+void batch_step_dir(const BatchView &b, int j) {
     constexpr int D2 = 1 - D1;
-    const float dt = b->dt;
+    const float dt = b.dt;
     batch_launch_stage<D1, 1>(b, R_U, R_T2, j, dt / 2);
     batch_launch_stage<D2, 2>(b, R_T2, R_T3, j, dt / 2);
     batch_launch_stage<D1, 3>(b, R_T3, R_T4, j, dt);
     batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6);
 }
-void batch_enqueue(msfdm_batch *b, int it0, int n) {
+void batch_enqueue(const BatchView &b, int it0, int n) {
     for (int j = 0; j < n; ++j) {
         if ((it0 + j) & 1) batch_step_dir<1>(b, j);
         else batch_step_dir<0>(b, j);
     }
-    D::advance_base<<<1, 64, 0, b->stream>>>(b->base, n);
+    D::advance_base<<<1, 64, 0, b.stream>>>(b.base, n);
 }
 
 void batch_free(msfdm_batch *b) {
@@ -869,7 +884,7 @@ extern "C" int msfdm_batch_run(msfdm_batch *b, int nsteps) {
         if (b->use_graph && left >= GRAPH_STEPS && (b->host_it & 1) == 0) {
             if (!b->gexec) {
                 HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                batch_enqueue(b, 0, GRAPH_STEPS);
+                batch_enqueue(view_of(b), 0, GRAPH_STEPS);
                 hipGraph_t g = nullptr;
                 HIPCHK(hipStreamEndCapture(b->stream, &g));
                 b->graph = g;
@@ -880,7 +895,7 @@ extern "C" int msfdm_batch_run(msfdm_batch *b, int nsteps) {
             left -= GRAPH_STEPS;
         } else {
             const int n = (b->use_graph && (b->host_it & 1)) ? 1 : std::min(left, GRAPH_STEPS);
-            batch_enqueue(b, b->host_it, n);
+            batch_enqueue(view_of(b), b->host_it, n);
             HIPCHK(hipGetLastError());
             b->host_it += n;
             left -= n;
@@ -1027,6 +1042,7 @@ int ensure_adjoint(msfdm_solver *s) {
         s->grad.C33 = p; p += nS;
         s->grad.C55 = p; p += nS;
         s->grad.rho_s = p;
+        s->grad.gstride = 0;
     }
     return MSFDM_OK;
 }
@@ -1046,6 +1062,7 @@ int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, floa
     D::AdjIn a;
     a.S = S; a.LP = s->adj[A_LP]; a.TBp = TBp; a.qif = s->qif; a.TB = TB; a.clam = clam; a.cprev = cprev;
     HIPCHK(hipMemsetAsync(TB, 0, s->buf_floats * sizeof(float), s->stream));
+    a.bstride = 0; a.qstride = 0;
     D::adj_iface_q<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, a, s->qif);
     if (s->tz == 8) launch_adj_media<DIR, 8>(s, a);
     else launch_adj_media<DIR, 16>(s, a);
@@ -1059,7 +1076,7 @@ int adj_step(msfdm_solver *s, int it, const float *Un, const float *T2, const fl
     const size_t bytes = s->buf_floats * sizeof(float);
     float *LAM = s->adj[A_LAM], *LP = s->adj[A_LP], *ACC = s->adj[A_ACC], *TA = s->adj[A_TBA], *TBb = s->adj[A_TBB];
     HIPCHK(hipMemcpyAsync(LP, LAM, bytes, hipMemcpyDeviceToDevice, s->stream));
-    D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, LAM, LP);
+    D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, LAM, LP, 0);
     HIPCHK(hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream));
     const float dt = s->d.dt;
     const bool odd = it & 1;                 // forward stage directions: d1 d2 d1 d2
@@ -1167,7 +1184,7 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
         // reverse sweep over the segment
         for (int it = it1 - 1; it >= it0 && rc == MSFDM_OK; --it) {
             if (it % gap == 0)
-                D::adj_inject<<<((int)nx + 255) / 256, 256, 0, s->stream>>>(s->geo, s->adj[A_LAM], res, nrec, it / gap);
+                D::adj_inject<<<((int)nx + 255) / 256, 256, 0, s->stream>>>(s->geo, s->adj[A_LAM], res, nrec, it / gap, 0, 0);
             const int j = it - it0;
             rc = adj_step(s, it, s->seg[4 * j], s->seg[4 * j + 1], s->seg[4 * j + 2], s->seg[4 * j + 3]);
         }
@@ -1218,7 +1235,7 @@ extern "C" int msfdm_adjoint_apply(msfdm_solver *s, int op, int dir) {
         if (rc) return rc;
     } else {
         HIPCHK(hipMemcpyAsync(out, s->adj[A_LP], s->buf_floats * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
-        D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->adj[A_LP], out);
+        D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, s->adj[A_LP], out, 0);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1256,5 +1273,303 @@ extern "C" int msfdm_update_media(msfdm_solver *s, const float *K, const float *
             }
         HIPCHK(hipMemcpy(const_cast<float *>(it.dst), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    return MSFDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Multi-shot gradient: the shots of one model are stepped together, forward
+// (checkpointed) and adjoint, one launch per RK stage for all shots of a
+// wave.  Every per-shot buffer of a wave is a slab with a fixed stride per
+// shot; the forward recomputation runs the batched stage kernels from per-step
+// shot tables pointing into the segment storage, the adjoint kernels take the
+// shot from blockIdx.y.  Each shot accumulates its own FP64 gradient (the
+// single-shot arithmetic), and the per-shot gradients are added in shot order,
+// so the result is deterministic.  The wave size follows from a memory budget.
+// No reference counterpart.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ShotsPlan {
+    long long bf = 0;     // floats per state buffer
+    long long gfl = 0;    // doubles per gradient set
+    long long P = 0;
+    long long rfl = 0;    // residual floats per shot
+    int every = 0, nseg = 0, nrec = 0;
+    long long bufs = 0;   // state buffers per shot: U, 5 adjoint, checkpoints, 4 per segment step
+    size_t per_shot = 0;  // device bytes per shot of a wave
+    size_t shared = 0;    // device bytes independent of the wave size
+};
+
+ShotsPlan plan_shots(const msfdm_solver *s, int nsteps, int every) {
+    ShotsPlan p;
+    p.P = s->geo.P;
+    p.bf = s->buf_floats;
+    p.gfl = 2 * (long long)s->d.nz_fluid * p.P + 5 * (long long)s->d.nz_solid * p.P;
+    p.nrec = (nsteps - 1) / s->gap + 1;
+    p.rfl = 3 * (long long)p.nrec * s->d.nx;
+    if (every <= 0) every = std::max(2, (int)std::lround(std::sqrt(nsteps / 4.0)));
+    if (every & 1) ++every;
+    p.every = every;
+    p.nseg = (nsteps + every - 1) / every;
+    p.bufs = 6 + p.nseg + 4LL * every;
+    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + 24 * p.P + p.rfl) + sizeof(double) * (size_t)p.gfl +
+                 (size_t)every * sizeof(D::Shot);
+    p.shared = sizeof(double) * (size_t)p.gfl + ((size_t)64 << 20);
+    return p;
+}
+
+enum { W_LAM = 0, W_LP = 1, W_ACC = 2, W_TA = 3, W_TB = 4 };
+
+struct ShotsWave {
+    msfdm_solver *s = nullptr;
+    ShotsPlan p;
+    int kc = 0;                        // shots per wave (slab capacity)
+    float *fmem = nullptr;             // all float slabs
+    double *gmem = nullptr;            // kc per-shot gradient sets, then the total
+    float *U = nullptr, *ckpt = nullptr, *seg = nullptr, *kif = nullptr, *qif = nullptr, *res = nullptr;
+    float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *grad = nullptr, *total = nullptr;
+    D::Shot *tables = nullptr;         // p.every tables of kc shots
+    int4 *tiles = nullptr;             // tiles of kc shots (tile height s->tz)
+    int nt_shot = 0, nbx = 0, nbf = 0;
+    // slab idx of a region holding one state buffer per shot
+    float *slab(float *base, long long idx) const { return base + idx * kc * p.bf; }
+    ~ShotsWave() {
+        if (s) (void)hipStreamSynchronize(s->stream);   // nothing queued may still use the slabs
+        for (void *q : {(void *)fmem, (void *)gmem, (void *)tables, (void *)tiles})
+            if (q) (void)hipFree(q);
+    }
+};
+
+int wave_alloc(ShotsWave &w) {
+    msfdm_solver *s = w.s;
+    const ShotsPlan &p = w.p;
+    const long long kc = w.kc;
+    const long long nfl = kc * (p.bufs * p.bf + 24 * p.P + p.rfl);
+    if (hipMalloc(&w.fmem, nfl * sizeof(float)) != hipSuccess)
+        return set_error(MSFDM_EDEVICE, "out of device memory for the multi-shot gradient slabs");
+    if (hipMalloc(&w.gmem, (kc + 1) * p.gfl * sizeof(double)) != hipSuccess)
+        return set_error(MSFDM_EDEVICE, "out of device memory for the multi-shot gradients");
+    HIPCHK(hipMemsetAsync(w.fmem, 0, nfl * sizeof(float), s->stream));
+    HIPCHK(hipMemsetAsync(w.gmem, 0, (kc + 1) * p.gfl * sizeof(double), s->stream));
+    float *f = w.fmem;
+    w.U = f; f += kc * p.bf;
+    for (auto &a : w.adj) { a = f; f += kc * p.bf; }
+    w.ckpt = f; f += (long long)p.nseg * kc * p.bf;
+    w.seg = f; f += 4LL * p.every * kc * p.bf;
+    w.kif = f; f += kc * 16 * p.P;
+    w.qif = f; f += kc * 8 * p.P;
+    w.res = f;
+    w.grad = w.gmem;
+    w.total = w.gmem + kc * p.gfl;
+    HIPCHK(hipMalloc(&w.tables, (size_t)p.every * kc * sizeof(D::Shot)));
+    const int tz = s->tz;
+    w.nbx = (s->d.nx - 4) / D::TX + 1;
+    w.nbf = (s->d.nz_fluid - 3 + tz - 1) / tz;
+    const int nbs = (s->d.nz_solid - 3 + tz - 1) / tz;
+    w.nt_shot = w.nbx * (w.nbf + nbs);
+    std::vector<int4> t;
+    for (int i = 0; i < w.kc; ++i)
+        for (int band = 0; band < w.nbf + nbs; ++band)
+            for (int bx = 0; bx < w.nbx; ++bx) t.push_back(make_int4(i, band, bx, w.nbf));
+    HIPCHK(hipMalloc(&w.tiles, t.size() * sizeof(int4)));
+    HIPCHK(hipMemcpy(w.tiles, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return MSFDM_OK;
+}
+
+BatchView wave_view(const ShotsWave &w, int kk, int slot) {
+    BatchView v{};
+    v.shots = w.tables + (size_t)slot * w.kc;
+    const int k = w.s->tz == 8 ? 0 : 1;
+    v.tiles[k] = w.tiles;
+    v.ntiles[k] = kk * w.nt_shot;
+    v.tz = w.s->tz;
+    v.ns = kk;
+    v.maxnx = w.s->d.nx;
+    v.base = w.s->base;
+    v.dt = w.s->d.dt;
+    v.stream = w.s->stream;
+    return v;
+}
+
+void wave_step(const ShotsWave &w, int kk, int slot, int it) {
+    const BatchView v = wave_view(w, kk, slot);
+    if (it & 1) batch_step_dir<1>(v, 0);
+    else batch_step_dir<0>(v, 0);
+    D::advance_base<<<1, 64, 0, v.stream>>>(v.base, 1);
+}
+
+D::Grad wave_grad(const ShotsWave &w) {
+    const long long nF = (long long)w.s->d.nz_fluid * w.p.P, nS = (long long)w.s->d.nz_solid * w.p.P;
+    D::Grad g;
+    g.K = w.grad; g.rho_f = w.grad + nF;
+    g.C11 = w.grad + 2 * nF; g.C13 = g.C11 + nS; g.C33 = g.C13 + nS; g.C55 = g.C33 + nS; g.rho_s = g.C55 + nS;
+    g.gstride = w.p.gfl;
+    return g;
+}
+
+template <int DIR>
+void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, float *TB, float clam, float cprev) {
+    msfdm_solver *s = w.s;
+    D::AdjIn a;
+    a.S = S; a.LP = w.adj[W_LP]; a.TBp = TBp; a.qif = w.qif; a.TB = TB; a.clam = clam; a.cprev = cprev;
+    a.bstride = w.p.bf;
+    a.qstride = 8 * w.p.P;
+    (void)hipMemsetAsync(TB, 0, (size_t)kk * w.p.bf * sizeof(float), s->stream);
+    D::adj_iface_q<DIR><<<dim3((s->d.nx + 127) / 128, kk), 128, 0, s->stream>>>(s->geo, s->coef, a, w.qif);
+    const int grid = 8 * ((w.nt_shot + 7) / 8);
+    if (s->tz == 8)
+        D::adj_stage_kernel<DIR, 8><<<dim3(grid, kk), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, wave_grad(w), w.nbx,
+                                                                                 w.nbf, w.nt_shot);
+    else
+        D::adj_stage_kernel<DIR, 16><<<dim3(grid, kk), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, wave_grad(w), w.nbx,
+                                                                                  w.nbf, w.nt_shot);
+}
+
+// Reverse of step `it` for the kk shots of the wave (segment slot j holds the
+// forward stage states of that step).
+void wave_adj_step(ShotsWave &w, int kk, int it, int j) {
+    msfdm_solver *s = w.s;
+    const long long n = (long long)kk * w.p.bf;
+    const size_t bytes = (size_t)n * sizeof(float);
+    float *LAM = w.adj[W_LAM], *LP = w.adj[W_LP], *ACC = w.adj[W_ACC], *TA = w.adj[W_TA], *TBb = w.adj[W_TB];
+    (void)hipMemcpyAsync(LP, LAM, bytes, hipMemcpyDeviceToDevice, s->stream);
+    D::adj_filter<<<dim3((s->d.nx + D::FB - 1) / D::FB, kk), D::FB, 0, s->stream>>>(s->geo, s->coef, LAM, LP, w.p.bf);
+    (void)hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream);
+    const float dt = s->d.dt;
+    const bool odd = it & 1;
+    const int ag = (int)std::min<long long>((n + 255) / 256, 8192);
+    auto acc = [&](const float *tb) { D::axpy_kernel<<<ag, 256, 0, s->stream>>>(ACC, tb, n); };
+    const float *Un = w.slab(w.seg, 4LL * j), *T2 = w.slab(w.seg, 4LL * j + 1), *T3 = w.slab(w.seg, 4LL * j + 2),
+                *T4 = w.slab(w.seg, 4LL * j + 3);
+    if (odd) wave_adj_stage<0>(w, kk, T4, nullptr, TA, dt / 6, 0.f); else wave_adj_stage<1>(w, kk, T4, nullptr, TA, dt / 6, 0.f);
+    acc(TA);
+    if (odd) wave_adj_stage<1>(w, kk, T3, TA, TBb, dt / 3, dt); else wave_adj_stage<0>(w, kk, T3, TA, TBb, dt / 3, dt);
+    acc(TBb);
+    if (odd) wave_adj_stage<0>(w, kk, T2, TBb, TA, dt / 3, dt / 2); else wave_adj_stage<1>(w, kk, T2, TBb, TA, dt / 3, dt / 2);
+    acc(TA);
+    if (odd) wave_adj_stage<1>(w, kk, Un, TA, TBb, dt / 6, dt / 2); else wave_adj_stage<0>(w, kk, Un, TA, TBb, dt / 6, dt / 2);
+    acc(TBb);
+    std::swap(w.adj[W_LAM], w.adj[W_ACC]);
+}
+
+// One wave: shots isrc[0..kk) with residuals res_host ([kk][3][nrec][nx]).
+int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res_host) {
+    msfdm_solver *s = w.s;
+    const ShotsPlan &p = w.p;
+    const int every = p.every, gap = s->gap;
+    const size_t sbytes = (size_t)kk * p.bf * sizeof(float);
+    std::vector<D::Shot> h((size_t)every * w.kc);
+    for (int j = 0; j < every; ++j)
+        for (int i = 0; i < kk; ++i) {
+            D::Shot &t = h[(size_t)j * w.kc + i];
+            t.g = s->geo;
+            t.c = s->coef;
+            t.ctl = s->ctl;
+            t.ctl.base = s->base;
+            t.ctl.rec = nullptr;        // records are not kept in the gradient run
+            t.ctl.rec_cap = 0;
+            t.ctl.gap = 0x7fffffff;
+            t.ctl.srcx = s->srcx.empty() ? -100 : s->srcx[isrc[i]];
+            t.ctl.srcz = s->srcz.empty() ? -100 : s->srcz[isrc[i]];
+            t.buf[0] = w.U + (long long)i * p.bf;
+            for (int r = 1; r < 4; ++r) t.buf[r] = w.slab(w.seg, 4LL * j + r) + (long long)i * p.bf;
+            t.kif = w.kif + (long long)i * 16 * p.P;
+        }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(w.tables, h.data(), h.size() * sizeof(D::Shot), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w.res, res_host, (size_t)kk * p.rfl * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(w.U, 0, sbytes, s->stream));
+    HIPCHK(hipMemsetAsync(w.adj[W_LAM], 0, sbytes, s->stream));
+    HIPCHK(hipMemsetAsync(w.kif, 0, (size_t)kk * 16 * p.P * sizeof(float), s->stream));
+    HIPCHK(hipMemsetAsync(w.grad, 0, (size_t)kk * p.gfl * sizeof(double), s->stream));
+    // checkpointed forward
+    D::set_base<<<1, 64, 0, s->stream>>>(s->base, 0);
+    for (int it = 0; it < nsteps; ++it) {
+        if (it % every == 0)
+            HIPCHK(hipMemcpyAsync(w.slab(w.ckpt, it / every), w.U, sbytes, hipMemcpyDeviceToDevice, s->stream));
+        wave_step(w, kk, 0, it);
+    }
+    HIPCHK(hipGetLastError());
+    // reverse sweep, segment by segment
+    for (int sg = p.nseg - 1; sg >= 0; --sg) {
+        const int it0 = sg * every, it1 = std::min(nsteps, it0 + every);
+        HIPCHK(hipMemcpyAsync(w.U, w.slab(w.ckpt, sg), sbytes, hipMemcpyDeviceToDevice, s->stream));
+        D::set_base<<<1, 64, 0, s->stream>>>(s->base, it0);
+        for (int it = it0; it < it1; ++it) {
+            const int j = it - it0;
+            HIPCHK(hipMemcpyAsync(w.slab(w.seg, 4LL * j), w.U, sbytes, hipMemcpyDeviceToDevice, s->stream));
+            wave_step(w, kk, j, it);
+        }
+        for (int it = it1 - 1; it >= it0; --it) {
+            if (it % gap == 0)
+                D::adj_inject<<<dim3((s->d.nx + 255) / 256, kk), 256, 0, s->stream>>>(s->geo, w.adj[W_LAM], w.res, p.nrec,
+                                                                                       it / gap, p.bf, p.rfl);
+            wave_adj_step(w, kk, it, it - it0);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    D::grad_sum_kernel<<<(int)std::min<long long>((p.gfl + 255) / 256, 8192), 256, 0, s->stream>>>(w.total, w.grad, p.gfl,
+                                                                                                  p.gfl, kk);
+    HIPCHK(hipGetLastError());
+    return MSFDM_OK;
+}
+
+}  // namespace
+
+extern "C" int msfdm_gradient_shots(msfdm_solver *s, int nshot, const int *isrc, int nsteps, int every,
+                                    double mem_bytes, const float *res, double *gK, double *grho_f, double *gC11,
+                                    double *gC13, double *gC33, double *gC55, double *grho_s, int *wave_shots) {
+    clear_error();
+    if (!s || !isrc || !res || nshot <= 0 || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    const int nsrc = s->srcx.empty() ? 1 : (int)s->srcx.size();
+    for (int i = 0; i < nshot; ++i)
+        if (isrc[i] < 0 || isrc[i] >= nsrc) return set_error(MSFDM_EINVAL, "shot index out of range");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = grow_tables(s, nsteps)) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    ShotsWave w;
+    w.s = s;
+    w.p = plan_shots(s, nsteps, every);
+    size_t budget = 0;
+    if (mem_bytes > 0) {
+        budget = (size_t)mem_bytes;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.9 * (double)fr);
+    }
+    if (budget <= w.p.shared + w.p.per_shot)
+        return set_error(MSFDM_EDEVICE, "device memory budget too small for one shot (" +
+                                            std::to_string((w.p.shared + w.p.per_shot) >> 20) + " MiB needed)");
+    w.kc = (int)std::min<size_t>((size_t)nshot, (budget - w.p.shared) / w.p.per_shot);
+    if (wave_shots) *wave_shots = w.kc;
+    if (int rc = wave_alloc(w)) return rc;
+    for (int w0 = 0; w0 < nshot; w0 += w.kc) {
+        const int kk = std::min(w.kc, nshot - w0);
+        if (int rc = wave_run(w, kk, isrc + w0, nsteps, res + (size_t)w0 * w.p.rfl)) return rc;
+    }
+    D::set_base<<<1, 64, 0, s->stream>>>(s->base, s->host_it);   // the solver's own run is left where it was
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const long long nF = (long long)s->d.nz_fluid * w.p.P, nS = (long long)s->d.nz_solid * w.p.P;
+    const int nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
+    const double *t = w.total;
+    int rc;
+    if ((rc = copy_grad_out(s, t, nzf, gK)) || (rc = copy_grad_out(s, t + nF, nzf, grho_f)) ||
+        (rc = copy_grad_out(s, t + 2 * nF, nzs, gC11)) || (rc = copy_grad_out(s, t + 2 * nF + nS, nzs, gC13)) ||
+        (rc = copy_grad_out(s, t + 2 * nF + 2 * nS, nzs, gC33)) ||
+        (rc = copy_grad_out(s, t + 2 * nF + 3 * nS, nzs, gC55)) ||
+        (rc = copy_grad_out(s, t + 2 * nF + 4 * nS, nzs, grho_s)))
+        return rc;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_gradient_shots_plan(const msfdm_solver *s, int nsteps, int every, double *bytes_per_shot,
+                                         int *every_used) {
+    clear_error();
+    if (!s || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    const ShotsPlan p = plan_shots(s, nsteps, every);
+    if (bytes_per_shot) *bytes_per_shot = (double)p.per_shot;
+    if (every_used) *every_used = p.every;
     return MSFDM_OK;
 }

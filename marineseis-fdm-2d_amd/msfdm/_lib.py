@@ -39,7 +39,7 @@ EXPORTS = [
     "msfdm_batch_set_graph", "msfdm_batch_time_run", "msfdm_batch_autotune",
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
-    "msfdm_update_media", "msfdm_grid_metrics",
+    "msfdm_update_media", "msfdm_grid_metrics", "msfdm_gradient_shots", "msfdm_gradient_shots_plan",
 ]
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
 
@@ -120,6 +120,8 @@ def lib():
         "msfdm_adjoint_get_output": ([vp, i, fp], i),
         "msfdm_update_media": ([vp] + [fp] * 7, i),
         "msfdm_grid_metrics": ([C.POINTER(C.c_double)] * 2 + [i, i, i] + [C.POINTER(C.c_double)] * 5, i),
+        "msfdm_gradient_shots": ([vp, i, ip, i, i, C.c_double, fp] + [C.POINTER(C.c_double)] * 7 + [ip], i),
+        "msfdm_gradient_shots_plan": ([vp, i, i, C.POINTER(C.c_double), ip], i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
@@ -339,6 +341,28 @@ class Solver:
         check(lib().msfdm_gradient(self.h, int(isrc), int(nsteps), int(every), *[_fptr(r) for r in rs],
                                    *[g[n].ctypes.data_as(dp) for n in GRADS]))
         return g
+
+    def gradient_shots(self, shots, nsteps: int, res, every: int = 0, mem_bytes: float = 0.0):
+        """Sum over the shot indices `shots` of the single-shot gradients, all shots
+        of a wave stepped together.  res[i] = (res_vx, res_vz, res_p) of shot i, each
+        [nrec][nx].  Returns (dict of float64 gradients, shots per wave)."""
+        isrc = np.ascontiguousarray(shots, dtype=np.int32)
+        r = np.ascontiguousarray(np.stack([np.stack([_f32(a) for a in rr]) for rr in res]), dtype=np.float32)
+        if r.shape[0] != isrc.size:
+            raise ValueError("one residual triple per shot")
+        g = self._grad_arrays()
+        dp = C.POINTER(C.c_double)
+        wave = C.c_int()
+        check(lib().msfdm_gradient_shots(self.h, int(isrc.size), isrc.ctypes.data_as(C.POINTER(C.c_int)), int(nsteps),
+                                         int(every), float(mem_bytes), _fptr(r),
+                                         *[g[n].ctypes.data_as(dp) for n in GRADS], C.byref(wave)))
+        return g, wave.value
+
+    def gradient_shots_plan(self, nsteps: int, every: int = 0):
+        """(device bytes per shot of a multi-shot gradient wave, checkpoint interval)."""
+        b, e = C.c_double(), C.c_int()
+        check(lib().msfdm_gradient_shots_plan(self.h, int(nsteps), int(every), C.byref(b), C.byref(e)))
+        return b.value, e.value
 
     def adjoint_set_input(self, f, arr):
         f = FID[f] if isinstance(f, str) else f

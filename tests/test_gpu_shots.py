@@ -1,0 +1,86 @@
+"""Multi-shot gradient (msfdm_gradient_shots): all shots of a wave stepped together.
+
+* equals the in-order sum of the single-shot gradients (msfdm_gradient) to
+  1e-6 relative L2 per array (the adjoint's LDS / global atomic adds make
+  repeated runs of either path differ in the last bits), with the memory budget
+  forcing waves of 2 + 1 shots and with all shots in one wave;
+* matches the sum of the FP64 oracle's per-shot gradients to 1e-4 relative L2;
+* a budget below one shot is refused with MSFDM_EDEVICE, a bad shot index
+  with MSFDM_EINVAL.
+"""
+import numpy as np
+import pytest
+
+import msfdm
+from msfdm import modelgen
+import pyoracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 60
+
+
+def _model():
+    return modelgen.build_model(nx=72, nz=56, h=5.0, water_depth=100.0, npd=8, fpeak=25.0, dt=4e-4, nt=NSTEPS,
+                                src_x=[20, 36, 52], src_height=10.0, seafloor=[(6.0, 150.0)], random_std=0.05,
+                                corr_len=20.0, seed=3)
+
+
+def _residuals(m, nrec, seed=21):
+    rng = np.random.default_rng(seed)
+    return [[rng.standard_normal((nrec, m.nx)).astype(np.float32) for _ in range(3)] for _ in m.src_x]
+
+
+def test_shots_equal_sum_of_single_shot_gradients():
+    m = _model()
+    s = msfdm.Solver(m)
+    s.reset(0); s.run(NSTEPS); s.sync()
+    nrec = s.num_records()
+    res = _residuals(m, nrec)
+    single = [s.gradient(i, NSTEPS, res[i], every=8) for i in range(3)]
+    ref = {n: single[0][n] + single[1][n] + single[2][n] for n in msfdm._lib.GRADS}
+    before = [a.copy() for a in s.records()]
+    per_shot, every = s.gradient_shots_plan(NSTEPS, 8)
+    assert every == 8 and per_shot > 0
+    for budget, want in ((64 * 2 ** 20 + 2.9 * per_shot, 2), (0.0, 3)):
+        print(f"[shots] budget {budget:.3e} B", flush=True)
+        g, wave = s.gradient_shots([0, 1, 2], NSTEPS, res, every=8, mem_bytes=budget)
+        assert wave == want
+        for n in msfdm._lib.GRADS:
+            e = rel_l2(g[n], ref[n])
+            print(f"  {n}: rel L2 vs sum of single shots {e:.2e}", flush=True)
+            assert e <= 1e-6, (n, e)
+    # the solver's own records (of its last forward run) are untouched
+    for a, b in zip(before, s.records()):
+        assert np.array_equal(a, b)
+
+
+def test_shots_match_oracle():
+    m = _model()
+    o = pyoracle.Oracle(m)
+    nrec = (NSTEPS - 1) // o.gap + 1
+    res = _residuals(m, nrec, seed=5)
+    s = msfdm.Solver(m)
+    g, wave = s.gradient_shots([2, 0, 1], NSTEPS, [res[2], res[0], res[1]])
+    assert wave == 3
+    go = None
+    for i in (2, 0, 1):
+        gi = o.gradient(i, NSTEPS, [r.astype(np.float64) for r in res[i]], every=10)
+        go = gi if go is None else {n: go[n] + gi[n] for n in go}
+    for n in msfdm._lib.GRADS:
+        e = rel_l2(g[n], go[n])
+        print(f"  {n}: rel L2 vs FP64 oracle {e:.2e}", flush=True)
+        assert e <= 1e-4, (n, e)
+
+
+def test_shots_errors():
+    m = _model()
+    s = msfdm.Solver(m)
+    res = _residuals(m, 30)[:1]   # not read: both calls fail before the run
+    with pytest.raises(msfdm.MsfdmError) as e:
+        s.gradient_shots([0], NSTEPS, res, mem_bytes=1e6)
+    assert e.value.code == 4   # MSFDM_EDEVICE
+    with pytest.raises(msfdm.MsfdmError) as e:
+        s.gradient_shots([7], NSTEPS, res)
+    assert e.value.code == 1   # MSFDM_EINVAL
