@@ -166,6 +166,61 @@ def bench_gradient(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_shots(args):
+    """Multi-shot gradient: n shots of one model (sources spread evenly over the
+    middle 80 % of the line), all shots of a wave stepped together
+    (msfdm_gradient_shots) against the same shots one at a time (msfdm_gradient),
+    seeded synthetic residuals.  When --shots-steps is below the model's record
+    length the timed run covers that many steps and shots/s is scaled to the
+    full record."""
+    import numpy as np
+    import msfdm
+    from msfdm import modelgen
+    n = args.shots
+    nx = modelgen.preset(args.shots_workload, nt=1).nx
+    src = [int(round(x)) for x in np.linspace(0.1 * (nx - 1), 0.9 * (nx - 1), n)]
+    model = modelgen.preset(args.shots_workload, src_x=src, **({"nt": args.shots_record} if args.shots_record else {}))
+    full = model.nt
+    steps = args.shots_steps or full
+    s = msfdm.Solver(model)
+    s.reset(0); s.run(steps); s.sync()
+    nrec = s.num_records()
+    rng = np.random.default_rng(5)
+    res = [[rng.standard_normal((nrec, model.nx)).astype(np.float32) for _ in range(3)] for _ in range(n)]
+    per_shot, every = s.gradient_shots_plan(steps)
+    per_shot_full, every_full = s.gradient_shots_plan(full)
+    s.gradient_shots([0], steps, res[:1])                      # warm-up (allocation, code objects)
+    t0 = time.perf_counter()
+    g, wave = s.gradient_shots(list(range(n)), steps, res)
+    t_batch = time.perf_counter() - t0
+    k = min(n, args.shots_seq)
+    idx = [int(i) for i in np.linspace(0, n - 1, k)]
+    t_seq, ref = 0.0, None
+    for i in idx:
+        t0 = time.perf_counter()
+        gi = s.gradient(i, steps, res[i])
+        t_seq += time.perf_counter() - t0
+        ref = gi if ref is None else {a: ref[a] + gi[a] for a in ref}
+    t_seq *= n / k
+    gsub, _ = s.gradient_shots(idx, steps, [res[i] for i in idx])
+    par = {a: float(np.linalg.norm(gsub[a] - ref[a]) / np.linalg.norm(ref[a])) for a in msfdm._lib.GRADS}
+    scale = full / steps
+    hbm = 288e9
+    out = {"metric": f"multi-shot fwd+adjoint gradient shots/s ({args.shots_workload}, {full}-step records)",
+           "value": round(n / (t_batch * scale), 4), "unit": "shots/s",
+           "sequential_value": round(n / (t_seq * scale), 4),
+           "speedup_vs_sequential": round(t_seq / t_batch, 3),
+           "shots": n, "wave_shots": wave, "steps_timed": steps, "every": every,
+           "grid": [model.nx, model.nz], "batched_s": round(t_batch, 3), "sequential_s_est": round(t_seq, 3),
+           "sequential_shots_timed": k,
+           "bytes_per_shot_full_record": per_shot_full, "every_full_record": every_full,
+           "shots_per_wave_in_288GB": int(0.9 * hbm // per_shot_full),
+           "rel_l2_batched_vs_sum_of_single": par,
+           "data": "synthetic residuals (seeded normal), generated model",
+           "note": (f"timed {steps} of {full} steps; shots/s scaled x{scale:g}" if scale != 1 else "full records")}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,7 +229,12 @@ def main():
     ap.add_argument("--workload", default="seafloor2001")
     ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--kernel-reps", type=int, default=100)
-    ap.add_argument("--mode", default="solver", choices=["solver", "batch", "gradient"])
+    ap.add_argument("--mode", default="solver", choices=["solver", "batch", "gradient", "shots"])
+    ap.add_argument("--shots", type=int, default=32)
+    ap.add_argument("--shots-workload", default="flat401")
+    ap.add_argument("--shots-steps", type=int, default=0, help="steps per shot timed (0 = the full record)")
+    ap.add_argument("--shots-record", type=int, default=0, help="record length in steps (0 = the workload's)")
+    ap.add_argument("--shots-seq", type=int, default=4, help="shots timed one at a time for the comparison")
     ap.add_argument("--fd-dirs", type=int, default=16)
     ap.add_argument("--batch-shots", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=400)
@@ -200,6 +260,9 @@ def main():
         return
     if args.mode == "gradient":
         bench_gradient(args)
+        return
+    if args.mode == "shots":
+        bench_shots(args)
         return
 
     import msfdm
