@@ -9,7 +9,8 @@
 //                     Sbar[j] += a_ij qbar[i]; stencil contributions are
 //                     gathered in an LDS accumulator covering the tile and its
 //                     3-cell halo (ghost-flux reflections included) and flushed
-//                     with one global atomic add per cell; PML memory terms
+//                     with one global atomic add per cell (one LDS copy per
+//                     wave, so no LDS atomics); PML memory terms
 //                     and the model-gradient terms dk/dm . qbar (evaluated on
 //                     the forward stage state S) are pointwise and need no
 //                     atomics.  Gradients accumulate in FP64.
@@ -84,6 +85,13 @@ __device__ __forceinline__ float mac_g(const float *f, long long i, long long P,
     return fwd ? (-f[i] + f[i + P]) : -(-f[i] + f[i - P]);
 }
 
+// LDS accumulator of one wave: 5 fields x (its RW rows + 3-row halos) x the
+// tile's 70 columns (64 + 3-column halos).
+template <int TZ>
+struct WaveAcc {
+    float v[5][TZ / 4 + 2 * HALO][LXP];
+};
+
 struct AdjIn {
     const float *S;      // forward stage state
     const float *LP;     // lambda'
@@ -115,9 +123,10 @@ __device__ __forceinline__ float kbar(const AdjIn &a, long long i) {
 // Fluid tile: rows 3..nzf-1.  acc[0..2] = vx, vz, p adjoint accumulators.
 // ---------------------------------------------------------------------------
 template <int DIR, int TZ>
-__device__ __forceinline__ void fluid_adj_body(float (*acc)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+__device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, const Coef &c,
                                                const AdjIn &a, const Grad &gr, int bx, int bz) {
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    constexpr int RW = TZ / 4;   // consecutive tile rows per wave
     const int x0 = bx * TX, z0 = 3 + bz * TZ;
     const int P = g.P, nx = g.nx, nzf = g.nzf;
     const float *S = a.S;
@@ -126,11 +135,17 @@ __device__ __forceinline__ void fluid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
     const bool active_col = ix >= 3 && ix <= nx - 4;
     const int scol = active_col ? side_col(ix, nx, g.npd) : -1;
     const int cc = tx + HALO;
+    // this wave's accumulator; its rows start at tile row ty * RW - HALO
+    float (*wa)[RW + 2 * HALO][LXP] = acc[ty].v;
+    // The x-direction taps of neighbouring lanes hit the same cells, so each
+    // read-modify-write completes before the next is issued (volatile keeps
+    // program order; one wave's LDS operations execute in order).
     auto add = [&](int fld, int iz_to, int ix_to, float v) {
-        atomicAdd(&acc[fld][iz_to - z0 + HALO][ix_to - x0 + HALO], v);
+        volatile float *p = &wa[fld][iz_to - z0 - ty * RW + HALO][ix_to - x0 + HALO];
+        *p = *p + v;
     };
-    for (int j = 0; j < TZ / 4; ++j) {
-        const int iz = z0 + ty + 4 * j;
+    for (int j = 0; j < RW; ++j) {
+        const int iz = z0 + ty * RW + j;
         if (!active_col || iz > nzf - 1) continue;
         const long long gi = (long long)iz * P + ix;
         const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
@@ -276,9 +291,10 @@ __device__ __forceinline__ void fluid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
 // Solid tile: rows 0..nzs-4.  acc[0..4] = vx, vz, sx, sz, sxz accumulators.
 // ---------------------------------------------------------------------------
 template <int DIR, int TZ>
-__device__ __forceinline__ void solid_adj_body(float (*acc)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+__device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, const Coef &c,
                                                const AdjIn &a, const Grad &gr, int bx, int bz) {
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    constexpr int RW = TZ / 4;   // consecutive tile rows per wave
     const int x0 = bx * TX, z0 = bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     const float *S = a.S;
@@ -289,11 +305,17 @@ __device__ __forceinline__ void solid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
     const int scol = active_col ? side_col(ix, nx, npd) : -1;
     const long long moff = (long long)(g.nzf - 1) * P;
     const int zb0 = nzs - 3 - npd;
+    // this wave's accumulator; its rows start at tile row ty * RW - HALO
+    float (*wa)[RW + 2 * HALO][LXP] = acc[ty].v;
+    // The x-direction taps of neighbouring lanes hit the same cells, so each
+    // read-modify-write completes before the next is issued (volatile keeps
+    // program order; one wave's LDS operations execute in order).
     auto add = [&](int fld, int iz_to, int ix_to, float v) {
-        atomicAdd(&acc[fld][iz_to - z0 + HALO][ix_to - x0 + HALO], v);
+        volatile float *p = &wa[fld][iz_to - z0 - ty * RW + HALO][ix_to - x0 + HALO];
+        *p = *p + v;
     };
-    for (int j = 0; j < TZ / 4; ++j) {
-        const int iz = z0 + ty + 4 * j;
+    for (int j = 0; j < RW; ++j) {
+        const int iz = z0 + ty * RW + j;
         if (!active_col || iz > nzs - 4) continue;
         const long long si = (long long)iz * P + ix, gi = moff + si;
         const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
@@ -483,26 +505,38 @@ __device__ __forceinline__ void solid_adj_body(float (*acc)[TZ + 2 * HALO][LXP],
     }
 }
 
-// Zero the LDS accumulator, run the medium body, flush with global atomics.
+// Zero the per-wave LDS accumulators, run the medium body, flush the tile and
+// its halo with one global atomic add per cell.  Wave w owns tile rows
+// w*RW .. w*RW+RW-1 and accumulates into its own copy covering rows
+// w*RW-3 .. w*RW+RW+2, so every LDS update is a plain read-modify-write: no
+// other wave touches the copy, the lanes of one instruction touch distinct
+// cells, and the updates are kept in program order; the flush adds the
+// overlapping copies in wave order.
 template <int NF, int TZ>
-__device__ __forceinline__ void acc_zero(float (*acc)[TZ + 2 * HALO][LXP]) {
-    constexpr int N = (TZ + 2 * HALO) * LXW;
-    for (int f = 0; f < NF; ++f)
-        for (int e = threadIdx.x; e < N; e += NTHR) {
-            const int r = e / LXW, cc = e - r * LXW;
-            acc[f][r][cc] = 0.0f;
-        }
+__device__ __forceinline__ void acc_zero(WaveAcc<TZ> *acc) {
+    constexpr int RW = TZ / 4, N = (RW + 2 * HALO) * LXW;
+    for (int w = 0; w < 4; ++w)
+        for (int f = 0; f < NF; ++f)
+            for (int e = threadIdx.x; e < N; e += NTHR) {
+                const int r = e / LXW, cc = e - r * LXW;
+                acc[w].v[f][r][cc] = 0.0f;
+            }
 }
 template <int NF, int TZ>
-__device__ __forceinline__ void acc_flush(float (*acc)[TZ + 2 * HALO][LXP], float *TB, const long long *off, int f0,
-                                          int z0, int x0, int nrows, int P) {
-    constexpr int N = (TZ + 2 * HALO) * LXW;
+__device__ __forceinline__ void acc_flush(WaveAcc<TZ> *acc, float *TB, const long long *off, int f0, int z0, int x0,
+                                          int nrows, int P) {
+    constexpr int RW = TZ / 4, N = (TZ + 2 * HALO) * LXW;
     for (int f = 0; f < NF; ++f) {
         float *dst = TB + off[f0 + f];
         for (int e = threadIdx.x; e < N; e += NTHR) {
-            const int r = e / LXW, cc = e - r * LXW;
+            const int r = e / LXW, cc = e - r * LXW;   // tile row r - HALO
+            float v = 0.0f;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int lr = r - w * RW;
+                if (lr >= 0 && lr < RW + 2 * HALO) v += acc[w].v[f][lr][cc];
+            }
             const int gz = z0 - HALO + r, gx = x0 - HALO + cc;
-            const float v = acc[f][r][cc];
             if (v != 0.0f && gz >= 0 && gz < nrows && gx >= 0 && gx < P) atomicAdd(dst + (long long)gz * P + gx, v);
         }
     }
@@ -511,7 +545,7 @@ __device__ __forceinline__ void acc_flush(float (*acc)[TZ + 2 * HALO][LXP], floa
 template <int DIR, int TZ>
 __global__ void __launch_bounds__(NTHR)
 adj_stage_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int ntiles) {
-    __shared__ float acc[5][TZ + 2 * HALO][LXP];
+    __shared__ WaveAcc<TZ> acc[4];
     const AdjIn a = adj_of_shot(a0, blockIdx.y);
     const Grad gr = grad_of_shot(gr0, blockIdx.y);
     const int per_xcd = gridDim.x >> 3;

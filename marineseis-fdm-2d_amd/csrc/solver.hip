@@ -1047,8 +1047,12 @@ int ensure_adjoint(msfdm_solver *s) {
     return MSFDM_OK;
 }
 
-template <int DIR, int TZ>
+// The adjoint stage kernel always runs 8-row tiles (one LDS accumulator copy
+// per wave fits three blocks per CU; 16-row tiles would fit two).
+constexpr int ADJ_TZ = 8;
+template <int DIR>
 void launch_adj_media(msfdm_solver *s, const D::AdjIn &a) {
+    constexpr int TZ = ADJ_TZ;
     const int nbx = (s->d.nx - 4) / D::TX + 1;
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
@@ -1064,8 +1068,7 @@ int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, floa
     HIPCHK(hipMemsetAsync(TB, 0, s->buf_floats * sizeof(float), s->stream));
     a.bstride = 0; a.qstride = 0;
     D::adj_iface_q<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, a, s->qif);
-    if (s->tz == 8) launch_adj_media<DIR, 8>(s, a);
-    else launch_adj_media<DIR, 16>(s, a);
+    launch_adj_media<DIR>(s, a);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
 }
@@ -1332,6 +1335,7 @@ struct ShotsWave {
     D::Shot *tables = nullptr;         // p.every tables of kc shots
     int4 *tiles = nullptr;             // tiles of kc shots (tile height s->tz)
     int nt_shot = 0, nbx = 0, nbf = 0;
+    int nta_shot = 0, nbfa = 0;        // adjoint tiles (ADJ_TZ rows) per shot
     // slab idx of a region holding one state buffer per shot
     float *slab(float *base, long long idx) const { return base + idx * kc * p.bf; }
     ~ShotsWave() {
@@ -1368,6 +1372,8 @@ int wave_alloc(ShotsWave &w) {
     w.nbf = (s->d.nz_fluid - 3 + tz - 1) / tz;
     const int nbs = (s->d.nz_solid - 3 + tz - 1) / tz;
     w.nt_shot = w.nbx * (w.nbf + nbs);
+    w.nbfa = (s->d.nz_fluid - 3 + ADJ_TZ - 1) / ADJ_TZ;
+    w.nta_shot = w.nbx * (w.nbfa + (s->d.nz_solid - 3 + ADJ_TZ - 1) / ADJ_TZ);
     std::vector<int4> t;
     for (int i = 0; i < w.kc; ++i)
         for (int band = 0; band < w.nbf + nbs; ++band)
@@ -1417,13 +1423,9 @@ void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, floa
     a.qstride = 8 * w.p.P;
     (void)hipMemsetAsync(TB, 0, (size_t)kk * w.p.bf * sizeof(float), s->stream);
     D::adj_iface_q<DIR><<<dim3((s->d.nx + 127) / 128, kk), 128, 0, s->stream>>>(s->geo, s->coef, a, w.qif);
-    const int grid = 8 * ((w.nt_shot + 7) / 8);
-    if (s->tz == 8)
-        D::adj_stage_kernel<DIR, 8><<<dim3(grid, kk), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, wave_grad(w), w.nbx,
-                                                                                 w.nbf, w.nt_shot);
-    else
-        D::adj_stage_kernel<DIR, 16><<<dim3(grid, kk), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, wave_grad(w), w.nbx,
-                                                                                  w.nbf, w.nt_shot);
+    const int grid = 8 * ((w.nta_shot + 7) / 8);
+    D::adj_stage_kernel<DIR, ADJ_TZ><<<dim3(grid, kk), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, wave_grad(w), w.nbx,
+                                                                                 w.nbfa, w.nta_shot);
 }
 
 // Reverse of step `it` for the kk shots of the wave (segment slot j holds the
