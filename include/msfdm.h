@@ -143,9 +143,10 @@ int msfdm_get_tile_rows(const msfdm_solver *s, int *rows);
 /* Enable (1) / disable (0) hipGraph replay of the time loop (default 1). */
 int msfdm_set_graph(msfdm_solver *s, int enable);
 /* Time `reps` launches of one kernel on the solver's stream with HIP events
- * (which: 0 = RK stage-2 kernel over both media, 1 = interface-row stage,
+ * (which: 0 = RK stage-2 kernel over both media, 1 = RK stage-4 kernel,
  * 2 = stage-4 interface rows + filter + record); returns the mean ms per
- * launch.  Overwrites the stage buffers: reset the shot afterwards. */
+ * launch.  The kernels write a scratch buffer; the interface rows of the stage
+ * buffers may be rewritten: reset the shot afterwards. */
 int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms);
 /* Time nsteps of msfdm_run between HIP events on the solver's stream. */
 int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
@@ -153,7 +154,9 @@ int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
 int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
 /* Algorithmic HBM bytes of one launch timed by msfdm_time_kernel(which): an
  * RK stage-2 launch reads the stage state and U^n and writes the next stage
- * state (3 floats per field) plus the coefficients once, per updated cell. */
+ * state (3 floats per field), a stage-4 launch reads T4, U^n, T2, T3 and
+ * writes U' (5 floats per field), each plus the coefficients once, per
+ * updated cell. */
 int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
 
 /* Tile-shape autotuner: times nsteps steps for each tile height, keeps the

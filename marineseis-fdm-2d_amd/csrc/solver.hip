@@ -48,7 +48,7 @@ struct msfdm_solver {
     long long buf_floats = 0;
     float *buf[4] = {nullptr, nullptr, nullptr, nullptr};
     float *kbuf = nullptr;       // raw-k output of msfdm_eval_rhs
-    float *kif = nullptr;        // [8][P] interface-row k + [8][P] interface rows of U^n
+    float *kif = nullptr;        // [KIF_ROWS][P]: raw interface-row k of the 4 stages + U^n interface rows
     float *coefmem = nullptr;
     float *amp = nullptr;
     int *base = nullptr;
@@ -169,24 +169,28 @@ int grow_tables(msfdm_solver *s, int steps_needed) {
 // ---------------------------------------------------------------------------
 template <int DIR, int STAGE, int TZ>
 void launch_media(msfdm_solver *s, const float *S, const float *U, const float *T2, const float *T3, float *O, int j,
-                  float ct, float amp) {
+                  float ct, float amp, float cprev) {
     const int nbx = (s->d.nx - 4) / D::TX + 1;
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
     const int grid = 8 * ((ntiles + 7) / 8);
     D::stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif,
-                                                                      j, ct, amp, nbx, nbf, ntiles);
+                                                                      j, ct, amp, cprev, nbx, nbf, ntiles);
 }
+// One RK stage; cprev is the previous stage's weight (its interface rows are
+// rebuilt by this stage's tiles).  Stage 4 adds the interface / filter / record
+// kernel, the raw-k mode (STAGE 0) the interface condition on the raw k.
 template <int DIR, int STAGE>
-void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, float amp) {
+void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, float amp, float cprev) {
     const float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3];
-    if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, T2, T3, O, j, ct, amp);
-    else launch_media<DIR, STAGE, 16>(s, S, U, T2, T3, O, j, ct, amp);
+    if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, T2, T3, O, j, ct, amp, cprev);
+    else launch_media<DIR, STAGE, 16>(s, S, U, T2, T3, O, j, ct, amp, cprev);
     if constexpr (STAGE == 4) {
-        D::iface_final<DIR><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, T2, T3,
-                                                                                    O, s->kif, ct, j);
-    } else {
-        D::iface_stage<DIR, STAGE><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, S, U, O, s->kif, ct);
+        const float dt = s->d.dt;
+        D::iface_final<DIR><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, O, s->kif,
+                                                                                    dt / 2, dt, ct, j);
+    } else if constexpr (STAGE == 0) {
+        D::iface_rhs<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, O, s->kif);
     }
 }
 
@@ -197,10 +201,10 @@ void enqueue_step_dir(msfdm_solver *s, int j) {
     constexpr int D2 = 1 - D1;
     const float dt = s->d.dt;
     float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3], *T4 = s->buf[R_T4];
-    launch_stage<D1, 1>(s, U, T2, j, dt / 2, 0.f);
-    launch_stage<D2, 2>(s, T2, T3, j, dt / 2, 0.f);
-    launch_stage<D1, 3>(s, T3, T4, j, dt, 0.f);
-    launch_stage<D2, 4>(s, T4, U, j, dt / 6, 0.f);
+    launch_stage<D1, 1>(s, U, T2, j, dt / 2, 0.f, 0.f);
+    launch_stage<D2, 2>(s, T2, T3, j, dt / 2, 0.f, dt / 2);
+    launch_stage<D1, 3>(s, T3, T4, j, dt, 0.f, dt / 2);
+    launch_stage<D2, 4>(s, T4, U, j, dt / 6, 0.f, dt);
 }
 void enqueue_steps(msfdm_solver *s, int it0, int n) {
     for (int j = 0; j < n; ++j) {
@@ -341,8 +345,8 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
         CK(hipMalloc(&s->buf[r], off * sizeof(float)));
         CK(hipMemset(s->buf[r], 0, off * sizeof(float)));
     }
-    CK(hipMalloc(&s->kif, 16 * P * sizeof(float)));
-    CK(hipMemset(s->kif, 0, 16 * P * sizeof(float)));
+    CK(hipMalloc(&s->kif, D::KIF_ROWS * P * sizeof(float)));
+    CK(hipMemset(s->kif, 0, D::KIF_ROWS * P * sizeof(float)));
     CK(hipMalloc(&s->base, 64 * sizeof(int)));
     CK(hipMemset(s->base, 0, 64 * sizeof(int)));
 
@@ -442,8 +446,10 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     s->bytes_step = 4.0 * (cf * (13 * 3 + 4 * ncf) + cs * (13 * 5 + 4 * ncs) + (pmlf + pmls) * 13 * 4);
     // stage-2 launch: 3 state floats per field + coefficients (+3 per PML memory field)
     s->bytes_kernel[0] = 4.0 * (cf * (3 * 3 + ncf) + cs * (3 * 5 + ncs) + (pmlf + pmls) * 3 * 4);
-    s->bytes_kernel[1] = 4.0 * (double)(nx - 6) * (8 * 3 + 2);          // k, U, out, normal
-    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 6 + 2);          // k, U^n, T2, T3, T4, out, normal
+    // stage-4 launch: 5 state floats per field (read T4, U^n, T2, T3, write U) + coefficients
+    s->bytes_kernel[1] = 4.0 * (cf * (5 * 3 + ncf) + cs * (5 * 5 + ncs) + (pmlf + pmls) * 5 * 4);
+    // interface / filter / record launch: 4 k sets + U^n, normal, output rows
+    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 5 + 2 + 8);
     CK(hipStreamSynchronize(s->stream));
 #undef CK
     *out = s;
@@ -566,8 +572,8 @@ extern "C" int msfdm_eval_rhs(msfdm_solver *s, int dir, float amp) {
     }
     HIPCHK(hipMemsetAsync(s->kbuf, 0, s->buf_floats * sizeof(float), s->stream));
     float *U = s->buf[R_U];
-    if (dir == 0) launch_stage<0, 0>(s, U, s->kbuf, 0, 0.f, amp);
-    else launch_stage<1, 0>(s, U, s->kbuf, 0, 0.f, amp);
+    if (dir == 0) launch_stage<0, 0>(s, U, s->kbuf, 0, 0.f, amp, 0.f);
+    else launch_stage<1, 0>(s, U, s->kbuf, 0, 0.f, amp, 0.f);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     return MSFDM_OK;
@@ -613,19 +619,26 @@ extern "C" int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms
     if (int rc = grow_tables(s, s->host_it + 1)) return rc;   // the stage reads amplitude row host_it
     float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3], *T4 = s->buf[R_T4];
     const float dt = s->d.dt;
+    if (!s->kbuf) {   // timing output (the state buffers are left as they are)
+        HIPCHK(hipMalloc(&s->kbuf, s->buf_floats * sizeof(float)));
+        HIPCHK(hipMemset(s->kbuf, 0, s->buf_floats * sizeof(float)));
+    }
+    float *X = s->kbuf;
     auto one = [&]() {
         switch (which) {
-            case 0:
-                if (s->tz == 8) launch_media<0, 2, 8>(s, T2, U, T2, T3, T3, 0, dt / 2, 0.f);
-                else launch_media<0, 2, 16>(s, T2, U, T2, T3, T3, 0, dt / 2, 0.f);
+            case 0:   // RK stage 2: reads T2 (+ U^n), writes the next stage state
+                if (s->tz == 8) launch_media<0, 2, 8>(s, T2, U, T2, T3, X, 0, dt / 2, 0.f, dt / 2);
+                else launch_media<0, 2, 16>(s, T2, U, T2, T3, X, 0, dt / 2, 0.f, dt / 2);
                 break;
-            case 1:
-                D::iface_stage<0, 2><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, T2, U, T3, s->kif, dt / 2);
+            case 1:   // RK stage 4: reads T4, U^n, T2, T3, writes U'
+                if (s->tz == 8) launch_media<1, 4, 8>(s, T4, U, T2, T3, X, 0, dt / 6, 0.f, dt);
+                else launch_media<1, 4, 16>(s, T4, U, T2, T3, X, 0, dt / 6, 0.f, dt);
                 break;
             default: {
                 D::Ctl quiet = s->ctl;   // no record slot is written while timing
                 quiet.gap = 0x7fffffff;
-                D::iface_final<0><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, T4, T2, T3, U, s->kif, dt / 6, 1);
+                D::iface_final<1><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, quiet, X,
+                                                                                           s->kif, dt / 2, dt, dt / 6, 0);
                 break;
             }
         }
@@ -752,30 +765,29 @@ BatchView view_of(const msfdm_batch *b) {
 }
 
 template <int DIR, int STAGE, int TZ>
-void batch_launch_media(const BatchView &b, int rS, int rO, int j, float ct) {
+void batch_launch_media(const BatchView &b, int rS, int rO, int j, float ct, float cprev) {
     const int k = TZ == 8 ? 0 : 1;
     const int grid = 8 * ((b.ntiles[k] + 7) / 8);
     D::batch_stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k], rS, rO, j,
-                                                                           ct);
+                                                                           ct, cprev);
 }
 template <int DIR, int STAGE>
-void batch_launch_stage(const BatchView &b, int rS, int rO, int j, float ct) {
-    if (b.tz == 8) batch_launch_media<DIR, STAGE, 8>(b, rS, rO, j, ct);
-    else batch_launch_media<DIR, STAGE, 16>(b, rS, rO, j, ct);
+void batch_launch_stage(const BatchView &b, int rS, int rO, int j, float ct, float cprev) {
+    if (b.tz == 8) batch_launch_media<DIR, STAGE, 8>(b, rS, rO, j, ct, cprev);
+    else batch_launch_media<DIR, STAGE, 16>(b, rS, rO, j, ct, cprev);
     if constexpr (STAGE == 4) {
-        D::batch_iface_final<DIR><<<dim3((b.maxnx + D::FO - 1) / D::FO, b.ns), D::FB, 0, b.stream>>>(b.shots, ct, j);
-    } else {
-        D::batch_iface_stage<DIR, STAGE><<<dim3((b.maxnx + 127) / 128, b.ns), 128, 0, b.stream>>>(b.shots, rS, rO, ct);
+        D::batch_iface_final<DIR><<<dim3((b.maxnx + D::FO - 1) / D::FO, b.ns), D::FB, 0, b.stream>>>(b.shots, b.dt / 2,
+                                                                                                   b.dt, ct, j);
     }
 }
 template <int D1>
 void batch_step_dir(const BatchView &b, int j) {
     constexpr int D2 = 1 - D1;
     const float dt = b.dt;
-    batch_launch_stage<D1, 1>(b, R_U, R_T2, j, dt / 2);
-    batch_launch_stage<D2, 2>(b, R_T2, R_T3, j, dt / 2);
-    batch_launch_stage<D1, 3>(b, R_T3, R_T4, j, dt);
-    batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6);
+    batch_launch_stage<D1, 1>(b, R_U, R_T2, j, dt / 2, 0.f);
+    batch_launch_stage<D2, 2>(b, R_T2, R_T3, j, dt / 2, dt / 2);
+    batch_launch_stage<D1, 3>(b, R_T3, R_T4, j, dt, dt / 2);
+    batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6, dt);
 }
 void batch_enqueue(const BatchView &b, int it0, int n) {
     for (int j = 0; j < n; ++j) {
@@ -1315,7 +1327,7 @@ ShotsPlan plan_shots(const msfdm_solver *s, int nsteps, int every) {
     p.every = every;
     p.nseg = (nsteps + every - 1) / every;
     p.bufs = 6 + p.nseg + 4LL * every;
-    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + 24 * p.P + p.rfl) + sizeof(double) * (size_t)p.gfl +
+    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl) + sizeof(double) * (size_t)p.gfl +
                  (size_t)every * sizeof(D::Shot);
     p.shared = sizeof(double) * (size_t)p.gfl + ((size_t)64 << 20);
     return p;
@@ -1349,7 +1361,7 @@ int wave_alloc(ShotsWave &w) {
     msfdm_solver *s = w.s;
     const ShotsPlan &p = w.p;
     const long long kc = w.kc;
-    const long long nfl = kc * (p.bufs * p.bf + 24 * p.P + p.rfl);
+    const long long nfl = kc * (p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl);
     if (hipMalloc(&w.fmem, nfl * sizeof(float)) != hipSuccess)
         return set_error(MSFDM_EDEVICE, "out of device memory for the multi-shot gradient slabs");
     if (hipMalloc(&w.gmem, (kc + 1) * p.gfl * sizeof(double)) != hipSuccess)
@@ -1361,7 +1373,7 @@ int wave_alloc(ShotsWave &w) {
     for (auto &a : w.adj) { a = f; f += kc * p.bf; }
     w.ckpt = f; f += (long long)p.nseg * kc * p.bf;
     w.seg = f; f += 4LL * p.every * kc * p.bf;
-    w.kif = f; f += kc * 16 * p.P;
+    w.kif = f; f += kc * D::KIF_ROWS * p.P;
     w.qif = f; f += kc * 8 * p.P;
     w.res = f;
     w.grad = w.gmem;
@@ -1476,14 +1488,14 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
             t.ctl.srcz = s->srcz.empty() ? -100 : s->srcz[isrc[i]];
             t.buf[0] = w.U + (long long)i * p.bf;
             for (int r = 1; r < 4; ++r) t.buf[r] = w.slab(w.seg, 4LL * j + r) + (long long)i * p.bf;
-            t.kif = w.kif + (long long)i * 16 * p.P;
+            t.kif = w.kif + (long long)i * D::KIF_ROWS * p.P;
         }
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy(w.tables, h.data(), h.size() * sizeof(D::Shot), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(w.res, res_host, (size_t)kk * p.rfl * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(w.U, 0, sbytes, s->stream));
     HIPCHK(hipMemsetAsync(w.adj[W_LAM], 0, sbytes, s->stream));
-    HIPCHK(hipMemsetAsync(w.kif, 0, (size_t)kk * 16 * p.P * sizeof(float), s->stream));
+    HIPCHK(hipMemsetAsync(w.kif, 0, (size_t)kk * D::KIF_ROWS * p.P * sizeof(float), s->stream));
     HIPCHK(hipMemsetAsync(w.grad, 0, (size_t)kk * p.gfl * sizeof(double), s->stream));
     // checkpointed forward
     D::set_base<<<1, 64, 0, s->stream>>>(s->base, 0);

@@ -1,6 +1,6 @@
 // Device code of the fluid-solid RK4 stepper for gfx950.
 //
-// One RK stage = two launches:
+// One RK stage = one launch (stage 4: two):
 //   stage_kernel<DIR,STAGE,TZ> runs, per 64 x TZ tile,
 //   fluid_body                 fluid RHS (update_velocity_fluid + update_stress_fluid,
 //                              include/pml_fs_kernels.cuh:333-507) + xi-PML memory
@@ -9,16 +9,18 @@
 //   solid_body                 solid RHS (update_stress_solid + update_velocity_solid,
 //                              include/pml_fs_kernels.cuh:509-752) + xi/zeta-PML memory
 //                              + the RK4 combination
-//   iface_stage<DIR,STAGE>     fl_so_boundary (include/pml_fs_kernels.cuh:754-827)
-//                              on the interface rows + their RK4 combination;
-//                              in stage 4 also fl_so_filter (src/pml_fs_kernels.cu:428-525,
+//                              + the previous stage's interface rows, rebuilt from
+//                              its raw interface k with fl_so_boundary
+//                              (include/pml_fs_kernels.cuh:754-827)
+//   iface_final<DIR>           stage 4: the interface rows of U' (fl_so_boundary of
+//                              all four stages), fl_so_filter (src/pml_fs_kernels.cu:428-525,
 //                              out of place) and the receiver-row record.
 //
 // The stage kernels read the stage state once through an LDS tile with a 3-cell
 // halo on every side, build the interface / free-surface ghost fluxes
 // (src/pml_fs_kernels.cu:385-426) from that tile, and never materialise k: the
 // classical-RK4 accumulator and the next stage state are written directly.
-// Interface-row k values (fluid row nzf-1, solid row 0) go to a small [8][P]
+// Interface-row k values (fluid row nzf-1, solid row 0) go to a small side
 // buffer because the boundary condition couples both media.
 //
 // STAGE 1..4 are the RK stages; STAGE 0 writes the raw k (used to check the
@@ -272,6 +274,78 @@ __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const
     }
 }
 
+// fl_so_boundary<DIR> on the eight interface k values (include/pml_fs_kernels.cuh:754-827).
+// The sxz reconstruction keeps the reference's expression (no nx*nz*snn term).
+template <int DIR>
+__device__ __forceinline__ void interface_bc(float k[8], float2 n) {
+    const float n1 = n.x, n2 = n.y;
+    const float vxf = k[0], vzf = k[1];
+    float pf = k[2];
+    const float vxs = k[3], vzs = k[4], sxx = k[5], szz = k[6], sxz = k[7];
+    float vnf = n1 * vxf + n2 * vzf;
+    const float vtf = n2 * vxf - n1 * vzf;
+    float vns = n1 * vxs + n2 * vzs;
+    const float vts = n2 * vxs - n1 * vzs;
+    float snn = n1 * n1 * sxx + 2.0f * n1 * n2 * sxz + n2 * n2 * szz;
+    float snt = n1 * n2 * (sxx - szz) + (n2 * n2 - n1 * n1) * sxz;
+    const float stt = n2 * n2 * sxx - 2.0f * n1 * n2 * sxz + n1 * n1 * szz;
+    if constexpr (DIR == 0) { vnf = vns; pf = -snn; snt = 0.0f; }
+    else { vns = vnf; snn = -pf; snt = 0.0f; }
+    k[0] = n1 * vnf + n2 * vtf;
+    k[1] = n2 * vnf - n1 * vtf;
+    k[2] = pf;
+    k[3] = n1 * vns + n2 * vts;
+    k[4] = n2 * vns - n1 * vts;
+    k[5] = n1 * n1 * snn + 2.0f * n1 * n2 * snt + n2 * n2 * stt;
+    k[6] = n2 * n2 * snn - 2.0f * n1 * n2 * snt + n1 * n1 * stt;
+    k[7] = -n1 * n2 * stt + (n2 * n2 - n1 * n1) * snt;
+}
+
+__device__ __forceinline__ long long iface_index(const Geo &g, int f, int ix) {
+    // f 0..2: fluid vx, vz, p at row nzf-1; 3..7: solid vx, vz, sx, sz, sxz at row 0
+    return f < 3 ? g.off[MSFDM_FVX + f] + (long long)(g.nzf - 1) * g.P + ix : g.off[MSFDM_SVX + f - 3] + ix;
+}
+
+// Interface side buffer kif ([KIF_ROWS][P]).  Rows 8(s-1) .. 8(s-1)+7 hold the
+// raw interface-row k of RK stage s (the raw-k mode, stage 0, uses rows 0..7);
+// rows KIF_U .. KIF_U+7 hold U^n on the interface rows, saved by stage 1.  Row f
+// of a set: 0..2 = fluid vx, vz, p on fluid row nzf-1; 3..7 = solid vx, vz, sxx,
+// szz, sxz on solid row 0.
+constexpr int KIF_U = 32, KIF_ROWS = 40;
+__device__ __forceinline__ constexpr int kif_set(int stage) { return stage <= 1 ? 0 : 8 * (stage - 1); }
+
+// The interface rows of stage state S (RK stage STAGE >= 2) are not written by
+// the previous stage's launch, whose k there still lacks the interface
+// condition; every tile that holds an interface row rebuilds it in its LDS tile
+// from that stage's raw k (fl_so_boundary of the previous stage's direction
+// 1 - DIR) and U^n:  S_if = U^n_if + cprev * B(k_prev).  The tile whose own 64
+// columns hold the row (wb) also stores the rebuilt values to S, so the stage
+// buffers are complete for later readers (the RK4 combination of stage 4 uses
+// the same values through iface_final, the adjoint reads the stored states).
+// Other blocks of the same launch may load that row before or after the store;
+// they overwrite it in LDS either way.
+template <int DIR, int STAGE, int NF, int TZ>
+__device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+                                                      const float *__restrict__ S, const float *__restrict__ kif,
+                                                      int lrow, int f0, int x0, float cprev, bool wb) {
+    const long long P = g.P;
+    const float *kp = kif + (long long)kif_set(STAGE - 1) * P;
+    for (int t = threadIdx.x; t < LXW; t += NTHR) {
+        const int dx = t - HALO, ix = x0 + dx;
+        if (ix < 3 || ix > g.nx - 4) continue;
+        float k[8];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) k[f] = kp[f * P + ix];
+        interface_bc<1 - DIR>(k, c.nif[ix]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const float v = kif[(KIF_U + f0 + f) * P + ix] + cprev * k[f0 + f];
+            sh[f][lrow][XOFF + dx] = v;
+            if (wb && dx >= 0 && dx < TX) const_cast<float *>(S)[iface_index(g, f0 + f, ix)] = v;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Fluid stage: rows 3..nzf-1, columns 3..nx-4.
 // ---------------------------------------------------------------------------
@@ -280,7 +354,7 @@ __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
            const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
            const float *__restrict__ T3, float *__restrict__ O, float *__restrict__ kif, int jstep,
-           float ct, float amp_direct, int bx, int bz) {
+           float ct, float amp_direct, float cprev, int bx, int bz) {
     // sh[0..2] = vx, vz, p of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = 3 + bz * TZ;
@@ -299,6 +373,13 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     }
     load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
     __syncthreads();
+    if constexpr (STAGE >= 2) {
+        const int lrow = (nzf - 1) - (z0 - HALO);   // tile row of the interface (the tile starts inside the fluid)
+        if (lrow < TZ + 2 * HALO) {
+            rebuild_interface_row<DIR, STAGE, 3, TZ>(sh, g, c, S, kif, lrow, 0, x0, cprev, lrow - HALO < TZ);
+            __syncthreads();
+        }
+    }
     if (!active) return;
     const int cc = tx + XOFF;
     const float (*shp)[LXP] = sh[2];
@@ -425,9 +506,15 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 kp += amp * SRC_W[szo + 2][sxo + 2];
         }
         if (dist_flso == 0) {
-            kif[0 * (long long)P + ix] = kvx;
-            kif[1 * (long long)P + ix] = kvz;
-            kif[2 * (long long)P + ix] = kp;
+            float *kk = kif + (long long)kif_set(STAGE) * P;
+            kk[0 * (long long)P + ix] = kvx;
+            kk[1 * (long long)P + ix] = kvz;
+            kk[2 * (long long)P + ix] = kp;
+            if constexpr (STAGE == 1) {   // S = U^n here
+                kif[(long long)(KIF_U + 0) * P + ix] = shvx[r][cc];
+                kif[(long long)(KIF_U + 1) * P + ix] = shvz[r][cc];
+                kif[(long long)(KIF_U + 2) * P + ix] = shp[r][cc];
+            }
             if constexpr (STAGE != 0) continue;
         }
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct, cur.u[0]);
@@ -443,7 +530,7 @@ template <int DIR, int STAGE, int TZ>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const float *__restrict__ S,
            const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
-           float *__restrict__ O, float *__restrict__ kif, float ct, int bx, int bz) {
+           float *__restrict__ O, float *__restrict__ kif, float ct, float cprev, int bx, int bz) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = bz * TZ;
@@ -462,6 +549,12 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
     }
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
+    if constexpr (STAGE >= 2) {
+        if (z0 == 0) {   // the band holding solid row 0 (tile row HALO)
+            rebuild_interface_row<DIR, STAGE, 5, TZ>(sh, g, c, S, kif, HALO, 3, x0, cprev, true);
+            __syncthreads();
+        }
+    }
     if (!active) return;
     const int cc = tx + XOFF;
     const float (*svx)[LXP] = sh[0];
@@ -601,11 +694,19 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZEZ] + ai, Ez, kEz, ct);
         }
         if (iz == 0) {
-            kif[3 * (long long)P + ix] = kvx;
-            kif[4 * (long long)P + ix] = kvz;
-            kif[5 * (long long)P + ix] = ksx;
-            kif[6 * (long long)P + ix] = ksz;
-            kif[7 * (long long)P + ix] = ksxz;
+            float *kk = kif + (long long)kif_set(STAGE) * P;
+            kk[3 * (long long)P + ix] = kvx;
+            kk[4 * (long long)P + ix] = kvz;
+            kk[5 * (long long)P + ix] = ksx;
+            kk[6 * (long long)P + ix] = ksz;
+            kk[7 * (long long)P + ix] = ksxz;
+            if constexpr (STAGE == 1) {   // S = U^n here
+                kif[(long long)(KIF_U + 3) * P + ix] = svx[r][cc];
+                kif[(long long)(KIF_U + 4) * P + ix] = svz[r][cc];
+                kif[(long long)(KIF_U + 5) * P + ix] = ssx[r][cc];
+                kif[(long long)(KIF_U + 6) * P + ix] = ssz[r][cc];
+                kif[(long long)(KIF_U + 7) * P + ix] = ssxz[r][cc];
+            }
             if constexpr (STAGE != 0) continue;
         }
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cur.u[0]);
@@ -628,79 +729,45 @@ template <int DIR, int STAGE, int TZ>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
              const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
-             float *__restrict__ kif, int jstep, float ct, float amp_direct, int nbx, int nbz_f, int ntiles) {
+             float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
+             int ntiles) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
     const int per_xcd = gridDim.x >> 3;
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
     if (band < nbz_f)
-        fluid_body<DIR, STAGE, TZ>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, bx, band);
+        fluid_body<DIR, STAGE, TZ>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx, band);
     else
-        solid_body<DIR, STAGE, TZ>(sh, g, c, S, U, T2, T3, O, kif, ct, bx, band - nbz_f);
+        solid_body<DIR, STAGE, TZ>(sh, g, c, S, U, T2, T3, O, kif, ct, cprev, bx, band - nbz_f);
 }
 
-// fl_so_boundary<DIR> on the eight interface k values (include/pml_fs_kernels.cuh:754-827).
-// The sxz reconstruction keeps the reference's expression (no nx*nz*snn term).
+// Raw-k mode (STAGE 0, msfdm_eval_rhs): the interface condition applied to the
+// raw k of the interface rows at column ix.
 template <int DIR>
-__device__ __forceinline__ void interface_bc(float k[8], float2 n) {
-    const float n1 = n.x, n2 = n.y;
-    const float vxf = k[0], vzf = k[1];
-    float pf = k[2];
-    const float vxs = k[3], vzs = k[4], sxx = k[5], szz = k[6], sxz = k[7];
-    float vnf = n1 * vxf + n2 * vzf;
-    const float vtf = n2 * vxf - n1 * vzf;
-    float vns = n1 * vxs + n2 * vzs;
-    const float vts = n2 * vxs - n1 * vzs;
-    float snn = n1 * n1 * sxx + 2.0f * n1 * n2 * sxz + n2 * n2 * szz;
-    float snt = n1 * n2 * (sxx - szz) + (n2 * n2 - n1 * n1) * sxz;
-    const float stt = n2 * n2 * sxx - 2.0f * n1 * n2 * sxz + n1 * n1 * szz;
-    if constexpr (DIR == 0) { vnf = vns; pf = -snn; snt = 0.0f; }
-    else { vns = vnf; snn = -pf; snt = 0.0f; }
-    k[0] = n1 * vnf + n2 * vtf;
-    k[1] = n2 * vnf - n1 * vtf;
-    k[2] = pf;
-    k[3] = n1 * vns + n2 * vts;
-    k[4] = n2 * vns - n1 * vts;
-    k[5] = n1 * n1 * snn + 2.0f * n1 * n2 * snt + n2 * n2 * stt;
-    k[6] = n2 * n2 * snn - 2.0f * n1 * n2 * snt + n1 * n1 * stt;
-    k[7] = -n1 * n2 * stt + (n2 * n2 - n1 * n1) * snt;
-}
-
-__device__ __forceinline__ long long iface_index(const Geo &g, int f, int ix) {
-    // f 0..2: fluid vx, vz, p at row nzf-1; 3..7: solid vx, vz, sx, sz, sxz at row 0
-    return f < 3 ? g.off[MSFDM_FVX + f] + (long long)(g.nzf - 1) * g.P + ix : g.off[MSFDM_SVX + f - 3] + ix;
-}
-
-// Interface rows of stages 1-3 (and of the raw-k mode, STAGE 0) at column ix.
-// kif holds [8][P] interface k values followed by [8][P] copies of U^n on the
-// interface rows, saved in stage 1 for the stage-4 kernel (which overwrites U there).
-template <int DIR, int STAGE>
-__device__ __forceinline__ void iface_stage_body(const Geo &g, const Coef &c, const float *__restrict__ S,
-                                                 const float *__restrict__ U, float *__restrict__ O,
-                                                 float *__restrict__ kif, float ct, int ix) {
+__device__ __forceinline__ void iface_rhs_body(const Geo &g, const Coef &c, float *__restrict__ O,
+                                               const float *__restrict__ kif, int ix) {
     if (ix < 3 || ix > g.nx - 4) return;
     float k[8];
 #pragma unroll
     for (int f = 0; f < 8; ++f) k[f] = kif[(long long)f * g.P + ix];
     interface_bc<DIR>(k, c.nif[ix]);
 #pragma unroll
-    for (int f = 0; f < 8; ++f) {
-        const long long i = iface_index(g, f, ix);
-        const float sv = STAGE == 1 ? S[i] : 0.0f;
-        if constexpr (STAGE == 1) kif[(long long)(8 + f) * g.P + ix] = sv;
-        rk<STAGE>(U, nullptr, nullptr, O, i, sv, k[f], ct);
-    }
+    for (int f = 0; f < 8; ++f) O[iface_index(g, f, ix)] = k[f];
 }
 
-template <int DIR, int STAGE>
+template <int DIR>
 __global__ void __launch_bounds__(128)
-iface_stage(Geo g, Coef c, const float *__restrict__ S, const float *__restrict__ U,
-            float *__restrict__ O, float *__restrict__ kif, float ct) {
-    iface_stage_body<DIR, STAGE>(g, c, S, U, O, kif, ct, blockIdx.x * blockDim.x + threadIdx.x);
+iface_rhs(Geo g, Coef c, float *__restrict__ O, const float *__restrict__ kif) {
+    iface_rhs_body<DIR>(g, c, O, kif, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Stage-4 interface rows + fl_so_filter (out of place) + receiver records.
+// The interface rows of T2, T3, T4 and the stage-4 k are rebuilt from the four
+// raw k sets and U^n (the interface condition of each stage's direction: the
+// stages run d1 d2 d1 d2 with d2 = DIR), with the same expressions the stage
+// kernels use: T2 = U + dt/2 B(k1), T3 = U + dt/2 B(k2), T4 = U + dt B(k3),
+// U' = U + ((T2 - U) + 2 (T3 - U) + (T4 - U)) / 3 + dt/6 B(k4).
 // A block of FB threads loads FB consecutive interface columns starting at
 // x0 - 3 (one per thread, all loads issued at once) and filters the FO = FB - 6
 // columns x0 .. x0 + FO - 1 that have their full 7-tap neighbourhood in LDS.
@@ -708,9 +775,8 @@ constexpr int FB = 128;
 constexpr int FO = FB - 2 * HALO;
 template <int DIR>
 __device__ __forceinline__ void iface_final_body(float (*pre)[FB], const Geo &g, const Coef &c,
-                                                 const Ctl &ctl, const float *__restrict__ T4,
-                                                 const float *__restrict__ T2, const float *__restrict__ T3,
-                                                 float *__restrict__ Uo, const float *__restrict__ kif, float cb,
+                                                 const Ctl &ctl, float *__restrict__ Uo,
+                                                 const float *__restrict__ kif, float c12, float c3, float cb,
                                                  int jstep, int x0) {
     const int nx = g.nx;
     const int t = threadIdx.x;
@@ -722,15 +788,24 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB], const Geo &g,
         float v[8];
         if (inner) {
             n = c.nif[ix];
-            float k[8];
-#pragma unroll
-            for (int f = 0; f < 8; ++f) k[f] = kif[(long long)f * g.P + ix];
-            interface_bc<DIR>(k, n);
+            const long long P = g.P;
+            float k1[8], k2[8], k3[8], k4[8];
 #pragma unroll
             for (int f = 0; f < 8; ++f) {
-                const long long i = iface_index(g, f, ix);
-                const float u = kif[(long long)(8 + f) * g.P + ix];   // U^n saved in stage 1
-                v[f] = u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (T4[i] - u)) * (1.0f / 3.0f) + cb * k[f];
+                k1[f] = kif[(long long)(kif_set(1) + f) * P + ix];
+                k2[f] = kif[(long long)(kif_set(2) + f) * P + ix];
+                k3[f] = kif[(long long)(kif_set(3) + f) * P + ix];
+                k4[f] = kif[(long long)(kif_set(4) + f) * P + ix];
+            }
+            interface_bc<1 - DIR>(k1, n);
+            interface_bc<DIR>(k2, n);
+            interface_bc<1 - DIR>(k3, n);
+            interface_bc<DIR>(k4, n);
+#pragma unroll
+            for (int f = 0; f < 8; ++f) {
+                const float u = kif[(long long)(KIF_U + f) * P + ix];   // U^n saved in stage 1
+                const float t2 = u + c12 * k1[f], t3 = u + c12 * k2[f], t4 = u + c3 * k3[f];
+                v[f] = u + ((t2 - u) + 2.0f * (t3 - u) + (t4 - u)) * (1.0f / 3.0f) + cb * k4[f];
             }
         } else {
 #pragma unroll
@@ -782,11 +857,10 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB], const Geo &g,
 
 template <int DIR>
 __global__ void __launch_bounds__(FB)
-iface_final(Geo g, Coef c, Ctl ctl, const float *__restrict__ T4, const float *__restrict__ T2,
-            const float *__restrict__ T3, float *__restrict__ Uo, const float *__restrict__ kif, float cb,
-            int jstep) {
+iface_final(Geo g, Coef c, Ctl ctl, float *__restrict__ Uo, const float *__restrict__ kif, float c12, float c3,
+            float cb, int jstep) {
     __shared__ float pre[8][FB];
-    iface_final_body<DIR>(pre, g, c, ctl, T4, T2, T3, Uo, kif, cb, jstep, blockIdx.x * FO);
+    iface_final_body<DIR>(pre, g, c, ctl, Uo, kif, c12, c3, cb, jstep, blockIdx.x * FO);
 }
 
 // ---------------------------------------------------------------------------
@@ -805,7 +879,7 @@ struct Shot {
 template <int DIR, int STAGE, int TZ>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
-                   int jstep, float ct) {
+                   int jstep, float ct, float cprev) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
     const int per_xcd = gridDim.x >> 3;
     const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -816,28 +890,20 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     float *O = s.buf[rO];
     if (tl.y < tl.w)
         fluid_body<DIR, STAGE, TZ>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
-                                   tl.z, tl.y);
+                                   cprev, tl.z, tl.y);
     else
-        solid_body<DIR, STAGE, TZ>(sh, s.g, s.c, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, ct, tl.z, tl.y - tl.w);
+        solid_body<DIR, STAGE, TZ>(sh, s.g, s.c, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, ct, cprev, tl.z,
+                                   tl.y - tl.w);
 }
 
 // grid (column blocks of the widest shot, shots)
-template <int DIR, int STAGE>
-__global__ void __launch_bounds__(128)
-batch_iface_stage(const Shot *__restrict__ shots, int rS, int rO, float ct) {
-    const Shot &s = shots[blockIdx.y];
-    iface_stage_body<DIR, STAGE>(s.g, s.c, s.buf[rS], s.buf[0], s.buf[rO], s.kif, ct,
-                                 blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 template <int DIR>
 __global__ void __launch_bounds__(FB)
-batch_iface_final(const Shot *__restrict__ shots, float cb, int jstep) {
+batch_iface_final(const Shot *__restrict__ shots, float c12, float c3, float cb, int jstep) {
     __shared__ float pre[8][FB];
     const Shot &s = shots[blockIdx.y];
     if ((int)blockIdx.x * FO > s.g.nx) return;   // whole block beyond this shot's grid
-    iface_final_body<DIR>(pre, s.g, s.c, s.ctl, s.buf[3], s.buf[1], s.buf[2], s.buf[0], s.kif, cb, jstep,
-                          blockIdx.x * FO);
+    iface_final_body<DIR>(pre, s.g, s.c, s.ctl, s.buf[0], s.kif, c12, c3, cb, jstep, blockIdx.x * FO);
 }
 
 __global__ void advance_base(int *base, int n) {
