@@ -10,7 +10,8 @@
 //                     gathered in an LDS accumulator covering the tile and its
 //                     3-cell halo (ghost-flux reflections included) and flushed
 //                     with one global atomic add per cell (one LDS copy per
-//                     wave, so no LDS atomics); PML memory terms
+//                     wave, x-direction taps gathered across lanes: no LDS
+//                     atomics); PML memory terms
 //                     and the model-gradient terms dk/dm . qbar (evaluated on
 //                     the forward stage state S) are pointwise and need no
 //                     atomics.  Gradients accumulate in FP64.
@@ -119,6 +120,30 @@ __device__ __forceinline__ float kbar(const AdjIn &a, long long i) {
     return v;
 }
 
+// x-direction scatter of one field on one row: tap q of lane t lands on tile
+// column t + o + q (o = lo_off<DIR>).  It runs as a gather over the wave: lane u
+// sums the taps landing on column u and lanes 0..3 the taps landing on the
+// four halo columns (F: -1, 64, 65, 66; B: -3, -2, -1, 64), so every
+// accumulator cell has a single writing lane and the read-modify-writes need
+// no ordering.  v[] is zero on lanes that contribute nothing; all 64 lanes
+// must reach the call (the wave's row is uniform).
+template <int DIR>
+__device__ __forceinline__ void xgather(float (*row)[LXP], int lr, int tx, const float v[5]) {
+    constexpr int o = lo_off<DIR>();
+    const int hc = DIR == 0 ? (tx == 0 ? -1 : TX - 1 + tx) : (tx < 3 ? tx - 3 : TX);
+    float s = 0.0f, sh = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const int src = tx - o - q, hsrc = hc - o - q;
+        const float x = __shfl(v[q], src & (TX - 1));
+        const float y = __shfl(v[q], hsrc & (TX - 1));
+        s += (src >= 0 && src < TX) ? x : 0.0f;
+        sh += (hsrc >= 0 && hsrc < TX) ? y : 0.0f;
+    }
+    row[lr][tx + HALO] += s;
+    if (tx < 4) row[lr][hc + HALO] += sh;
+}
+
 // ---------------------------------------------------------------------------
 // Fluid tile: rows 3..nzf-1.  acc[0..2] = vx, vz, p adjoint accumulators.
 // ---------------------------------------------------------------------------
@@ -137,20 +162,27 @@ __device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
     const int cc = tx + HALO;
     // this wave's accumulator; its rows start at tile row ty * RW - HALO
     float (*wa)[RW + 2 * HALO][LXP] = acc[ty].v;
-    // The x-direction taps of neighbouring lanes hit the same cells, so each
-    // read-modify-write completes before the next is issued (volatile keeps
-    // program order; one wave's LDS operations execute in order).
-    auto add = [&](int fld, int iz_to, int ix_to, float v) {
-        volatile float *p = &wa[fld][iz_to - z0 - ty * RW + HALO][ix_to - x0 + HALO];
-        *p = *p + v;
+    // Every accumulator cell has one writing lane: single-cell updates go to the
+    // lane's own column (z-direction taps, ghost reflections), the x-direction
+    // taps go through xgather.  Lanes outside columns 3..nx-4 run the row with
+    // in-bounds loads but contribute nothing.
+    const bool act = active_col;
+    auto add = [&](int fld, int iz_to, float v) {
+        if (act) wa[fld][iz_to - z0 - ty * RW + HALO][tx + HALO] += v;
+    };
+    auto addx = [&](int fld, int iz_to, const float (&v)[5]) {
+        float w[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) w[q] = act ? v[q] : 0.0f;
+        xgather<DIR>(wa[fld], iz_to - z0 - ty * RW + HALO, tx, w);
     };
     for (int j = 0; j < RW; ++j) {
         const int iz = z0 + ty * RW + j;
-        if (!active_col || iz > nzf - 1) continue;
+        if (iz > nzf - 1) continue;   // uniform over the wave
         const long long gi = (long long)iz * P + ix;
         const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
         float xx, xz;
-        xi_metrics(c, gi, xx, xz);
+        xi_metrics<2>(c, gi, xx, xz);
         const float gxixi = xx * xx + xz * xz;
         const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
         const bool at_row = dist_flso == 0;
@@ -193,7 +225,7 @@ __device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                 const long long gq = gi + m;
                 const float mJp = -c.J[gq] * Sp[gq];
                 float xxq, xzq;
-                xi_metrics(c, gq, xxq, xzq);
+                xi_metrics<2>(c, gq, xxq, xzq);
                 sx += w * mJp * xxq;
                 sz += w * mJp * xzq;
                 const int rr = iz + m;
@@ -216,16 +248,17 @@ __device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
             const float szb = rvzb / J + xz * projb / (gxixi * J);
             auto fze_add = [&](int rr, float fbx, float fbz) {
                 const long long q = (long long)rr * P + ix;
-                add(2, rr, ix, -c.J[q] * (c.zex[q] * fbx + c.zez[q] * fbz));
+                add(2, rr, -c.J[q] * (c.zex[q] * fbx + c.zez[q] * fbz));
             };
+            float xv[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
                 const int m = o + q;
                 const float w = drp_w<DIR>(q);
                 const long long gq = gi + m;
                 float xxq, xzq;
-                xi_metrics(c, gq, xxq, xzq);
-                add(2, iz, ix + m, -c.J[gq] * (xxq * w * sxb + xzq * w * szb));
+                xi_metrics<2>(c, gq, xxq, xzq);
+                xv[q] = -c.J[gq] * (xxq * w * sxb + xzq * w * szb);
                 const int rr = iz + m;
                 const float fbx = w * sxb, fbz = w * szb;
                 if (at_if) {
@@ -236,6 +269,7 @@ __device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                     else fze_add(6 - rr, -fbx, -fbz);
                 }
             }
+            addx(2, iz, xv);
         } else {
             const float dpx = drp_g<DIR>(Sp, gi, 1), dpz = drp_g<DIR>(Sp, gi, P);
             const float gxize = xx * zx + xz * zz;
@@ -246,15 +280,17 @@ __device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
             const float dpxb = -xx * rvxb - xz * rvzb + tb;
             const float dpzb = -zx * rvxb - zz * rvzb + tb * gxize / gxixi;
             constexpr int o = lo_off<DIR>();
+            float xv[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                add(2, iz, ix + o + q, drp_w<DIR>(q) * dpxb);
-                add(2, iz + o + q, ix, drp_w<DIR>(q) * dpzb);
+                xv[q] = drp_w<DIR>(q) * dpxb;
+                add(2, iz + o + q, drp_w<DIR>(q) * dpzb);
             }
+            addx(2, iz, xv);
         }
         float kvx = rvx, kvz = rvz;
         if (scol >= 0) { kvx += (pxc - 1.0f) * Dx - pxc * Vx; kvz += (pxc - 1.0f) * Dz - pxc * Vz; }
-        gr.rho_f[gi] += -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
+        if (act) gr.rho_f[gi] += -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
         // ---- pressure equation (row 3: p_k = 0, no dependence)
         if (dist_fs != 0) {
             const float Km = c.Kf[gi];
@@ -274,15 +310,17 @@ __device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                 const float Dvx = dvxx + gg * dvxz, Dvz = dvzx + gg * dvzz;
                 bracket += -(pxc - 1.0f) * xx * Dvx - (pxc - 1.0f) * xz * Dvz + pxc * xx * Ex + pxc * xz * Ez;
             }
-            constexpr int o = lo_off<DIR>();
+            float xa[5], xb[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                add(0, iz, ix + o + q, drp_w<DIR>(q) * bxx);
-                add(1, iz, ix + o + q, drp_w<DIR>(q) * bzx);
+                xa[q] = drp_w<DIR>(q) * bxx;
+                xb[q] = drp_w<DIR>(q) * bzx;
             }
-            mac_dz_adj<DIR>(dzn, false, bxz, [&](int m, float v) { add(0, iz + m, ix, v); });
-            mac_dz_adj<DIR>(dzn, false, bzz, [&](int m, float v) { add(1, iz + m, ix, v); });
-            gr.K[gi] += (double)qp * (double)bracket;
+            addx(0, iz, xa);
+            addx(1, iz, xb);
+            mac_dz_adj<DIR>(dzn, false, bxz, [&](int m, float v) { add(0, iz + m, v); });
+            mac_dz_adj<DIR>(dzn, false, bzz, [&](int m, float v) { add(1, iz + m, v); });
+            if (act) gr.K[gi] += (double)qp * (double)bracket;
         }
     }
 }
@@ -307,22 +345,29 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
     const int zb0 = nzs - 3 - npd;
     // this wave's accumulator; its rows start at tile row ty * RW - HALO
     float (*wa)[RW + 2 * HALO][LXP] = acc[ty].v;
-    // The x-direction taps of neighbouring lanes hit the same cells, so each
-    // read-modify-write completes before the next is issued (volatile keeps
-    // program order; one wave's LDS operations execute in order).
-    auto add = [&](int fld, int iz_to, int ix_to, float v) {
-        volatile float *p = &wa[fld][iz_to - z0 - ty * RW + HALO][ix_to - x0 + HALO];
-        *p = *p + v;
+    // Every accumulator cell has one writing lane: single-cell updates go to the
+    // lane's own column (z-direction taps, ghost reflections), the x-direction
+    // taps go through xgather.  Lanes outside columns 3..nx-4 run the row with
+    // in-bounds loads but contribute nothing.
+    const bool act = active_col;
+    auto add = [&](int fld, int iz_to, float v) {
+        if (act) wa[fld][iz_to - z0 - ty * RW + HALO][tx + HALO] += v;
+    };
+    auto addx = [&](int fld, int iz_to, const float (&v)[5]) {
+        float w[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) w[q] = act ? v[q] : 0.0f;
+        xgather<DIR>(wa[fld], iz_to - z0 - ty * RW + HALO, tx, w);
     };
     for (int j = 0; j < RW; ++j) {
         const int iz = z0 + ty * RW + j;
-        if (!active_col || iz > nzs - 4) continue;
+        if (iz > nzs - 4) continue;   // uniform over the wave
         const long long si = (long long)iz * P + ix, gi = moff + si;
         const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
         float xx, xz;
-        xi_metrics(c, gi, xx, xz);
+        xi_metrics<2>(c, gi, xx, xz);
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
-        const bool side = scol >= 0, bottom = iz >= zb0;
+        const bool side = scol >= 0, bottom = act && iz >= zb0;   // PML memory only in updated columns
         const bool at_row = iz == 0;
         const float qvx = at_row ? a.qif[3 * (long long)P + ix] : kbar(a, g.off[MSFDM_SVX] + si);
         const float qvz = at_row ? a.qif[4 * (long long)P + ix] : kbar(a, g.off[MSFDM_SVZ] + si);
@@ -372,18 +417,22 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                 A13 += q1 * zz * Dvz - pzc * zz * Ez;
                 A55 += q1 * (zz * Dvx + zx * Dvz) - pzc * zz * Ex - pzc * zx * Ez;
             }
-            constexpr int o = lo_off<DIR>();
+            float xa[5], xb[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                add(0, iz, ix + o + q, drp_w<DIR>(q) * bxx);
-                add(1, iz, ix + o + q, drp_w<DIR>(q) * bzx);
+                xa[q] = drp_w<DIR>(q) * bxx;
+                xb[q] = drp_w<DIR>(q) * bzx;
             }
-            mac_dz_adj<DIR>(iz, true, bxz, [&](int m, float v) { add(0, iz + m, ix, v); });
-            mac_dz_adj<DIR>(iz, true, bzz, [&](int m, float v) { add(1, iz + m, ix, v); });
-            gr.C11[si] += (double)qsx * (double)A11;
-            gr.C13[si] += (double)qsx * (double)A13 + (double)qsz * (double)A11;
-            gr.C33[si] += (double)qsz * (double)A13;
-            gr.C55[si] += (double)qsxz * (double)A55;
+            addx(0, iz, xa);
+            addx(1, iz, xb);
+            mac_dz_adj<DIR>(iz, true, bxz, [&](int m, float v) { add(0, iz + m, v); });
+            mac_dz_adj<DIR>(iz, true, bzz, [&](int m, float v) { add(1, iz + m, v); });
+            if (act) {
+                gr.C11[si] += (double)qsx * (double)A11;
+                gr.C13[si] += (double)qsx * (double)A13 + (double)qsz * (double)A11;
+                gr.C33[si] += (double)qsz * (double)A13;
+                gr.C55[si] += (double)qsxz * (double)A55;
+            }
         }
         // ---- velocity equations
         {
@@ -430,7 +479,7 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                     const long long gq = gi + m, sq = si + m;
                     const float Jq = c.J[gq];
                     float xxq, xzq;
-                    xi_metrics(c, gq, xxq, xzq);
+                    xi_metrics<2>(c, gq, xxq, xzq);
                     sx += w * Jq * (xxq * Ssx[sq] + xzq * Ssxz[sq]);
                     sz += w * Jq * (xxq * Ssxz[sq] + xzq * Ssz[sq]);
                     const int rr = iz + m;
@@ -449,10 +498,11 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                 auto fze_add = [&](int rr, float Fx, float Fz) {
                     const long long q = moff + (long long)rr * P + ix;
                     const float Jq = c.J[q], zxq = c.zex[q], zzq = c.zez[q];
-                    add(2, rr, ix, Jq * zxq * Fx);
-                    add(4, rr, ix, Jq * (zzq * Fx + zxq * Fz));
-                    add(3, rr, ix, Jq * zzq * Fz);
+                    add(2, rr, Jq * zxq * Fx);
+                    add(4, rr, Jq * (zzq * Fx + zxq * Fz));
+                    add(3, rr, Jq * zzq * Fz);
                 };
+                float x2[5], x3[5], x4[5];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) {
                     const int m = o + q;
@@ -460,15 +510,18 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                     const long long gq = gi + m;
                     const float Jq = c.J[gq];
                     float xxq, xzq;
-                    xi_metrics(c, gq, xxq, xzq);
+                    xi_metrics<2>(c, gq, xxq, xzq);
                     const float Fx = w * sxb, Fz = w * szb;
-                    add(2, iz, ix + m, Jq * xxq * Fx);
-                    add(4, iz, ix + m, Jq * (xzq * Fx + xxq * Fz));
-                    add(3, iz, ix + m, Jq * xzq * Fz);
+                    x2[q] = Jq * xxq * Fx;
+                    x4[q] = Jq * (xzq * Fx + xxq * Fz);
+                    x3[q] = Jq * xzq * Fz;
                     const int rr = iz + m;
                     if (rr >= 0) fze_add(rr, Fx, Fz);
                     else { fze_add(0, 2.f * Fx, 2.f * Fz); fze_add(-rr, -Fx, -Fz); }
                 }
+                addx(2, iz, x2);
+                addx(4, iz, x4);
+                addx(3, iz, x3);
             } else {
                 const float A = drp_g<DIR>(Ssx, si, 1), Bv = drp_g<DIR>(Ssx, si, P);
                 const float Cv = drp_g<DIR>(Ssz, si, 1), Dv = drp_g<DIR>(Ssz, si, P);
@@ -486,21 +539,25 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
                 const float db = zz * rvzb + xz * gx * Dxzb + zz * Dzzb;
                 const float eb = xz * rvxb + xx * rvzb + xz * Dxxb + xx * Dxzb + zz * gz * Dzxb + zx * gz * Dzzb;
                 const float fb = zz * rvxb + zx * rvzb + xz * gx * Dxxb + xx * gx * Dxzb + zz * Dzxb + zx * Dzzb;
+                float x2[5], x3[5], x4[5];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) {
                     const float w = drp_w<DIR>(q);
-                    add(2, iz, ix + o + q, w * ab);
-                    add(2, iz + o + q, ix, w * bb);
-                    add(3, iz, ix + o + q, w * cb);
-                    add(3, iz + o + q, ix, w * db);
-                    add(4, iz, ix + o + q, w * eb);
-                    add(4, iz + o + q, ix, w * fb);
+                    x2[q] = w * ab;
+                    add(2, iz + o + q, w * bb);
+                    x3[q] = w * cb;
+                    add(3, iz + o + q, w * db);
+                    x4[q] = w * eb;
+                    add(4, iz + o + q, w * fb);
                 }
+                addx(2, iz, x2);
+                addx(3, iz, x3);
+                addx(4, iz, x4);
             }
             float kvx = rvx, kvz = rvz;
             if (side) { kvx += q1x * Dxx - pxc * Vxs; kvz += q1x * Dxz - pxc * Vzs; }
             if (bottom) { kvx += q1z * Dzx - pzc * Vxb; kvz += q1z * Dzz - pzc * Vzb; }
-            gr.rho_s[si] += -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
+            if (act) gr.rho_s[si] += -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
         }
     }
 }
@@ -508,10 +565,9 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
 // Zero the per-wave LDS accumulators, run the medium body, flush the tile and
 // its halo with one global atomic add per cell.  Wave w owns tile rows
 // w*RW .. w*RW+RW-1 and accumulates into its own copy covering rows
-// w*RW-3 .. w*RW+RW+2, so every LDS update is a plain read-modify-write: no
-// other wave touches the copy, the lanes of one instruction touch distinct
-// cells, and the updates are kept in program order; the flush adds the
-// overlapping copies in wave order.
+// w*RW-3 .. w*RW+RW+2; no other wave touches the copy and each cell of it has
+// one writing lane, so every LDS update is a plain read-modify-write.  The
+// flush adds the overlapping copies in wave order.
 template <int NF, int TZ>
 __device__ __forceinline__ void acc_zero(WaveAcc<TZ> *acc) {
     constexpr int RW = TZ / 4, N = (RW + 2 * HALO) * LXW;

@@ -174,8 +174,12 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
     const int grid = 8 * ((ntiles + 7) / 8);
-    D::stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif,
-                                                                      j, ct, amp, cprev, nbx, nbf, ntiles);
+    if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
+        D::stage_kernel<DIR, STAGE, TZ, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
+                                                                             s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+    else
+        D::stage_kernel<DIR, STAGE, TZ, 0><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
+                                                                             s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
 }
 // One RK stage; cprev is the previous stage's weight (its interface rows are
 // rebuilt by this stage's tiles).  Stage 4 adds the interface / filter / record

@@ -1,7 +1,7 @@
 // Device code of the fluid-solid RK4 stepper for gfx950.
 //
 // One RK stage = one launch (stage 4: two):
-//   stage_kernel<DIR,STAGE,TZ> runs, per 64 x TZ tile,
+//   stage_kernel<DIR,STAGE,TZ,MET> runs, per 64 x TZ tile,
 //   fluid_body                 fluid RHS (update_velocity_fluid + update_stress_fluid,
 //                              include/pml_fs_kernels.cuh:333-507) + xi-PML memory
 //                              + source (src/pml_fs_kernels.cu:527-531, 618-621)
@@ -70,8 +70,11 @@ struct Coef {
     float xix0;
 };
 
+// MET: 0 = general metrics (arrays read), 1 = vertical columns (compile-time
+// constants), 2 = decided at run time from c.vert.
+template <int MET>
 __device__ __forceinline__ void xi_metrics(const Coef &c, long long gi, float &xx, float &xz) {
-    if (c.vert) { xx = c.xix0; xz = 0.0f; }
+    if (MET == 1 || (MET == 2 && c.vert)) { xx = c.xix0; xz = 0.0f; }
     else { xx = c.xix[gi]; xz = c.xiz[gi]; }
 }
 
@@ -189,11 +192,11 @@ struct RowPre {
     float u[NF];
 };
 
-template <int STAGE, int NF, int NC>
+template <int STAGE, int MET, int NF, int NC>
 __device__ __forceinline__ void prefetch_row(RowPre<NF, NC> &p, const Coef &c, const float *const (&cp)[NC],
                                              const float *__restrict__ U, const long long *off, int f0,
                                              long long ai, long long gi) {
-    xi_metrics(c, gi, p.xx, p.xz);
+    xi_metrics<MET>(c, gi, p.xx, p.xz);
     p.zx = c.zex[gi];
     p.zz = c.zez[gi];
 #pragma unroll
@@ -349,7 +352,7 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
 // ---------------------------------------------------------------------------
 // Fluid stage: rows 3..nzf-1, columns 3..nx-4.
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ>
+template <int DIR, int STAGE, int TZ, int MET>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
            const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
@@ -369,7 +372,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
         const long long a0 = (long long)min(z0 + ty, nzf - 1) * P + ix;
-        prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_FVX, a0, a0);
+        prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_FVX, a0, a0);
     }
     load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
     __syncthreads();
@@ -394,7 +397,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const RowPre<3, 2> cur = nxt;
         if (j + 1 < TZ / 4) {
             const long long an = (long long)min(iz + 4, nzf - 1) * P + ix;
-            prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_FVX, an, an);
+            prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_FVX, an, an);
         }
         const int r = lz + HALO;
         const long long gi = (long long)iz * P + ix;
@@ -421,7 +424,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 const long long gq = gi + m;
                 const float mJp = -c.J[gq] * shp[r][cc + m];
                 float xxq, xzq;
-                xi_metrics(c, gq, xxq, xzq);
+                xi_metrics<MET>(c, gq, xxq, xzq);
                 ax[q] = mJp * xxq;
                 az[q] = mJp * xzq;
                 const int rr = iz + m;
@@ -526,7 +529,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 // ---------------------------------------------------------------------------
 // Solid stage: rows 0..nzs-4, columns 3..nx-4.  Metrics at global row iz+nzf-1.
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ>
+template <int DIR, int STAGE, int TZ, int MET>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const float *__restrict__ S,
            const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
@@ -545,7 +548,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
     if (active) {
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
         const long long a0 = (long long)min(z0 + ty, nzs - 4) * P + ix;
-        prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_SVX, a0, moff + a0);
+        prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_SVX, a0, moff + a0);
     }
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
@@ -572,7 +575,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
         const RowPre<5, 5> cur = nxt;
         if (j + 1 < TZ / 4) {
             const long long an = (long long)min(iz + 4, nzs - 4) * P + ix;
-            prefetch_row<STAGE>(nxt, c, cp, U, g.off, MSFDM_SVX, an, moff + an);
+            prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_SVX, an, moff + an);
         }
         const int r = lz + HALO;
         const long long si = (long long)iz * P + ix;
@@ -611,7 +614,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
                 const long long gq = gi + m;
                 const float Jq = c.J[gq];
                 float xxq, xzq;
-                xi_metrics(c, gq, xxq, xzq);
+                xi_metrics<MET>(c, gq, xxq, xzq);
                 ax[q] = Jq * (xxq * ssx[r][cc + m] + xzq * ssxz[r][cc + m]);
                 az[q] = Jq * (xxq * ssxz[r][cc + m] + xzq * ssz[r][cc + m]);
                 const int rr = iz + m;
@@ -725,7 +728,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
 #ifndef MSFDM_STAGE_WAVES
 #define MSFDM_STAGE_WAVES 0   // minimum waves per SIMD requested for stage_kernel (0 = compiler choice)
 #endif
-template <int DIR, int STAGE, int TZ>
+template <int DIR, int STAGE, int TZ, int MET>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
              const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
@@ -737,9 +740,9 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
     if (band < nbz_f)
-        fluid_body<DIR, STAGE, TZ>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx, band);
+        fluid_body<DIR, STAGE, TZ, MET>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx, band);
     else
-        solid_body<DIR, STAGE, TZ>(sh, g, c, S, U, T2, T3, O, kif, ct, cprev, bx, band - nbz_f);
+        solid_body<DIR, STAGE, TZ, MET>(sh, g, c, S, U, T2, T3, O, kif, ct, cprev, bx, band - nbz_f);
 }
 
 // Raw-k mode (STAGE 0, msfdm_eval_rhs): the interface condition applied to the
@@ -889,10 +892,10 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     const float *S = s.buf[rS];
     float *O = s.buf[rO];
     if (tl.y < tl.w)
-        fluid_body<DIR, STAGE, TZ>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
+        fluid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
                                    cprev, tl.z, tl.y);
     else
-        solid_body<DIR, STAGE, TZ>(sh, s.g, s.c, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, ct, cprev, tl.z,
+        solid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, ct, cprev, tl.z,
                                    tl.y - tl.w);
 }
 

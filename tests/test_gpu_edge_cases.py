@@ -1,0 +1,80 @@
+"""GPU parity on the edge cases of the reference's source and grid handling.
+
+Source positions (add_gauss_source, include/tools_func.cuh:21-37, skips
+ix <= 2, ix >= nx-3, iz <= 2 and iz >= nz_f-3 of its 5x5 stencil and runs
+after the surface row's p_k = 0):
+* one row below the pressure-release surface (row-3 p_k becomes non-zero),
+* on the surface row itself,
+* one row above the interface (the centre row and the rows below are lost),
+* on the interface row (every stencil row is skipped: the field stays zero),
+* on the first / last updated column and inside the left PML strip.
+Each shot runs 300 steps; records within 1e-4 relative L2 of the FP64 oracle
+(exactly zero where the oracle's are zero).
+
+Size: the 4001 x 1501 two-scale seafloor grid (the largest workload) for 40
+steps against the oracle, same tolerance.
+"""
+import numpy as np
+import pytest
+
+import msfdm
+from msfdm import modelgen
+import pyoracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _edge_model():
+    m = modelgen.build_model(120, 90, 5.0, 150.0, npd=10, fpeak=25.0, dt=4e-4, nt=300, src_x=[60],
+                             seafloor=[(12.0, 200.0), (4.0, 70.0)], random_std=0.05, corr_len=25.0, seed=7)
+    nzf, nx = m.nz_fluid, m.nx
+    shots = [(60, 4), (60, 3), (60, nzf - 2), (60, nzf - 1), (3, 15), (nx - 4, 15), (10, 15)]
+    m.src_x = [x for x, _ in shots]
+    m.src_z = [z for _, z in shots]
+    return m, shots
+
+
+@pytest.mark.parametrize("shot", range(7))
+def test_source_positions_match_oracle(shot):
+    m, shots = _edge_model()
+    print(f"[edge] shot {shot} at (ix, iz) = {shots[shot]}, nz_f = {m.nz_fluid}", flush=True)
+    o = pyoracle.Oracle(m)
+    s = msfdm.Solver(m)
+    o.reset(shot); o.run(100)
+    s.reset(shot); s.run(100)
+    if shots[shot][1] == 4:
+        # the source reaches row 3 after its p_k = 0: the surface row carries pressure.
+        # Compared near the wavelet's peak (t = 40 ms): later it holds the time integral
+        # of the wavelet, which cancels towards zero.
+        po, pg = o.get("FP")[3], s.get("FP")[3]
+        assert np.abs(po).max() > 0
+        e = rel_l2(pg, po)
+        print(f"  surface-row p at step 100: rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-4, e
+    o.run(m.nt - 100)
+    s.run(m.nt - 100)
+    ro, rg = o.records(), s.records()
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), rg, ro):
+        if not np.any(b):
+            assert not np.any(a), name
+            continue
+        e = rel_l2(a, b)
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-4, (name, e)
+    if shots[shot][1] == m.nz_fluid - 1:
+        assert not np.any(ro[2]) and not np.any(rg[2])   # the whole stencil is skipped
+
+
+def test_largest_grid_matches_oracle():
+    m = modelgen.preset("twoscale4001", nt=40)
+    print(f"[edge] twoscale4001 {m.nx}x{m.nz} 40 steps", flush=True)
+    o = pyoracle.Oracle(m)
+    s = msfdm.Solver(m)
+    o.reset(0); o.run(m.nt)
+    s.reset(0); s.run(m.nt)
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), s.records(), o.records()):
+        assert np.any(b), name
+        e = rel_l2(a, b)
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-4, (name, e)
