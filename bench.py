@@ -225,6 +225,64 @@ def bench_shots(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_solid(args):
+    """Solid-only stepper with a traction-free top (grid.calc_type = 0): the
+    2001 x 1001 body-fitted grid under a 20 m / 600 m sinusoidal topography, 5 %
+    random solid (vp/vs/rho 3200/1800/2300), 20 Hz, dt = 0.25 ms, 30-cell CFS-PML,
+    6000 steps; free-surface vz recorded every 4 steps.  Same JSON fields as the
+    default solver line, with the FP64 oracle's solid-only stepper as the CPU
+    baseline."""
+    import msfdm
+    from msfdm import modelgen
+    model = modelgen.preset(args.solid_workload)
+    s = msfdm.Solver(model)
+    if args.tile_rows:
+        s.tile_rows = args.tile_rows
+    else:
+        s.autotune(64)
+    s.reset(0)
+    s.run(args.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    cells = model.nx * model.nz
+    bytes_step = s.bytes_per_step()
+    reps = args.kernel_reps
+    t_stage = s.time_kernel(0, reps)
+    t_stage4 = s.time_kernel(1, reps)
+    b_stage = s.kernel_bytes(0)
+    achieved = b_stage / (t_stage * 1e-3) / 1e9
+    kname, traffic = measured_traffic(args.solid_workload,
+                                      [f"free_stage_kernel<0, 2, {s.tile_rows}, {m}>" for m in (1, 0)])
+    cpu = cpu_baseline(model, args.cpu_steps) if args.cpu_steps > 0 else None
+    out = {
+        "metric": "Gcell-updates/s (plain FP32 solid-only solver, traction-free surface, 1 GPU)",
+        "value": round(cells * args.steps / wall / 1e9, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (generated body-fitted model, seeded random solid)",
+        "config": {
+            "workload": f"{args.solid_workload}: {model.nx}x{model.nz} solid grid, free surface on row 3, "
+                        f"npd={model.npd}, dt={model.dt}, fpeak={model.fpeak}",
+            "nx": model.nx, "nz": model.nz, "time_steps": args.steps, "tile_rows": s.tile_rows,
+            "hbm_GBps_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9, 1),
+            "hbm_frac_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
+            "bytes_per_step_alg": bytes_step,
+            "kernel_ms": {"free_stage_kernel_stage2": round(t_stage, 5),
+                          "free_stage_kernel_stage4": round(t_stage4, 5)},
+            "stage4_GBps_alg": round(s.kernel_bytes(1) / (t_stage4 * 1e-3) / 1e9, 1),
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": f"{kname or 'free_stage_kernel'} (RK stage 2, F direction)",
+                     "alg_bytes_per_launch": b_stage},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,7 +293,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=100)
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16],
                     help="tile height of the stage kernels (0 = the library's autotuner)")
-    ap.add_argument("--mode", default="solver", choices=["solver", "batch", "gradient", "shots"])
+    ap.add_argument("--mode", default="solver", choices=["solver", "batch", "gradient", "shots", "solid"])
+    ap.add_argument("--solid-workload", default="solid_topo2001")
     ap.add_argument("--shots", type=int, default=32)
     ap.add_argument("--shots-workload", default="flat401")
     ap.add_argument("--shots-steps", type=int, default=0, help="steps per shot timed (0 = the full record)")
@@ -269,6 +328,9 @@ def main():
         return
     if args.mode == "shots":
         bench_shots(args)
+        return
+    if args.mode == "solid":
+        bench_solid(args)
         return
 
     import msfdm

@@ -50,8 +50,12 @@ enum {
 };
 
 /* Scalar part of the model: the JSON keys read by MemManager::MemManager
- * (src/elements.cu:72-111).  Only calc_type = 1 (FANDS) with
- * boundary_type = 2 (PANDF) is accepted. */
+ * (src/elements.cu:72-111).  boundary_type = 2 (PANDF: CFS-PML sides and
+ * bottom, free top) with calc_type = 1 (FANDS, fluid over solid;
+ * PmlFs::FluidSolid) or calc_type = 0 (SOLID, a solid grid with a
+ * traction-free top row 3; PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307)
+ * is accepted.  A solid-only model has nz_fluid = 0, no fluid arrays (K and
+ * rho_f may be NULL) and metrics of nz_solid rows. */
 typedef struct msfdm_desc {
     int nx, nz_fluid, nz_solid;
     int calc_type, boundary_type;
@@ -62,8 +66,9 @@ typedef struct msfdm_desc {
     int npow;
     float vp_max, rc, beta0;
     int nsrc;
-    const int *src_x; /* fluid-array column of each shot (grid.src_pos.posx) */
-    const int *src_z; /* fluid-array row of each shot (grid.src_pos.posz) */
+    const int *src_x; /* column of each shot (grid.src_pos.posx) */
+    const int *src_z; /* row of each shot (grid.src_pos.posz): fluid-array row
+                         (calc_type 1) or solid-array row (calc_type 0) */
 } msfdm_desc;
 
 typedef struct msfdm_model msfdm_model;
@@ -107,8 +112,9 @@ void msfdm_solver_destroy(msfdm_solver *s);
 int msfdm_reset_shot(msfdm_solver *s, int isrc);
 
 /* Advance nsteps RK4 steps (each = PmlFs::FluidSolid::update_RK4 with
- * rk4dire = it & 1, src/main.cu:136-176, plus the interface filter), sampling
- * the seafloor receiver rows on the device after steps with it % gap == 0
+ * rk4dire = it & 1, src/main.cu:136-176, plus the interface filter; on a
+ * solid-only grid PmlFs::SolidOnly::update_RK4, src/pml_fs_kernels.cu:105-381),
+ * sampling the receiver rows on the device after steps with it % gap == 0
  * (gap = (int)(1e-3/dt + 0.5), src/main.cu:92).  Asynchronous. */
 int msfdm_run(msfdm_solver *s, int nsteps);
 /* Wait for all queued work. */
@@ -120,7 +126,8 @@ int msfdm_step_index(const msfdm_solver *s, int *it);
 int msfdm_num_records(const msfdm_solver *s, int *n);
 /* Copy records [first, first+count) of solid-row-0 vx, vz and fluid interface
  * row p (each count x nx floats, time-major): output_record, src/main.cu:245-295.
- * Any output pointer may be NULL. */
+ * A solid-only grid records vz of the free-surface row 3 only (vz; vx and p
+ * read as zeros), src/main.cu:297-322.  Any output pointer may be NULL. */
 int msfdm_get_records(msfdm_solver *s, int first, int count, float *vx, float *vz, float *p);
 
 /* Full-array field access in the reference layout (ADE fields expand to the
@@ -182,6 +189,9 @@ int msfdm_grid_metrics(const double *x, const double *z, int nz, int nx, int dev
                        double *xi_z, double *ze_x, double *ze_z);
 
 /* --- Discrete adjoint (no reference counterpart) ------------------------- */
+/* The adjoint, multi-shot and batched entry points, and msfdm_update_media,
+ * serve fluid-solid models (calc_type 1); with a solid-only model they return
+ * MSFDM_EINVAL. */
 /* Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
  * arrays for solid vx, solid vz, fluid p as returned by msfdm_get_records) with
  * respect to the model arrays K, rho_f ([nz_f][nx]) and C11, C13, C33, C55,

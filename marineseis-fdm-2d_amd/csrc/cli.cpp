@@ -5,6 +5,9 @@
 //   <shot>/snapshot/fluid/{vz,vx}/{vz,vx}_%06d.bin   full [nz_f][nx] arrays
 //   <shot>/record/solid_vx.bin, solid_vz.bin          solid row 0, [nrec][nx]
 //   <shot>/record/fluid_p.bin                         fluid row nz_f-1, [nrec][nx]
+// A solid-only model (grid.calc_type = 0) writes the solid snapshots and
+// <shot>/record/solid_vz.bin only, vz of the free-surface row 3
+// (src/main.cu:120-128, 297-322, 327-339).
 // Snapshots are taken after steps with it % snapshot == 0, records after steps
 // with it % gap == 0 (src/main.cu:194-211).  Each shot starts from a zero
 // wavefield (the reference carries the previous shot's field over).
@@ -84,16 +87,20 @@ extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
     std::printf("[output] cleared %s\n", out_root_s.c_str());
 
     const int nx = d.nx;
-    const size_t nf = (size_t)d.nz_fluid * nx, ns = (size_t)d.nz_solid * nx;
+    const bool fluid = d.calc_type == 1;
+    const size_t nf = fluid ? (size_t)d.nz_fluid * nx : 0, ns = (size_t)d.nz_solid * nx;
     std::vector<float> svz(ns), svx(ns), fvz(nf), fvx(nf);
     for (int isrc = 0; isrc < d.nsrc; ++isrc) {
         const fs::path shot = out_root / std::to_string(isrc);
         if (!make_dirs(shot / "snapshot", err) || !make_dirs(shot / "record", err)) return fail(err);
-        std::ofstream rvx(shot / "record" / "solid_vx.bin", std::ios::binary | std::ios::trunc);
+        std::ofstream rvx, rp;
         std::ofstream rvz(shot / "record" / "solid_vz.bin", std::ios::binary | std::ios::trunc);
-        std::ofstream rp(shot / "record" / "fluid_p.bin", std::ios::binary | std::ios::trunc);
-        if (!rp) return fail("cannot open fluid record files");
-        if (!rvx || !rvz) return fail("cannot open solid record files");
+        if (fluid) {
+            rvx.open(shot / "record" / "solid_vx.bin", std::ios::binary | std::ios::trunc);
+            rp.open(shot / "record" / "fluid_p.bin", std::ios::binary | std::ios::trunc);
+            if (!rp) return fail("cannot open fluid record files");
+        }
+        if ((fluid && !rvx) || !rvz) return fail("cannot open solid record files");
         if (msfdm_reset_shot(solver.get(), isrc) != MSFDM_OK) return 1;
 
         int it = 0;
@@ -106,12 +113,16 @@ extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
             if (snap < d.nt) {
                 std::printf("finished %0.2f%%\r", 100.0 * snap / d.nt);
                 std::fflush(stdout);
-                if (msfdm_get_field(solver.get(), MSFDM_SVZ, svz.data()) || msfdm_get_field(solver.get(), MSFDM_SVX, svx.data()) ||
-                    msfdm_get_field(solver.get(), MSFDM_FVZ, fvz.data()) || msfdm_get_field(solver.get(), MSFDM_FVX, fvx.data()))
+                if (msfdm_get_field(solver.get(), MSFDM_SVZ, svz.data()) || msfdm_get_field(solver.get(), MSFDM_SVX, svx.data()))
+                    return 1;
+                if (fluid && (msfdm_get_field(solver.get(), MSFDM_FVZ, fvz.data()) ||
+                              msfdm_get_field(solver.get(), MSFDM_FVX, fvx.data())))
                     return 1;
                 const fs::path sd = shot / "snapshot";
-                if (!write_field(sd / "solid" / "vz", "vz", snap, svz, err) || !write_field(sd / "solid" / "vx", "vx", snap, svx, err) ||
-                    !write_field(sd / "fluid" / "vz", "vz", snap, fvz, err) || !write_field(sd / "fluid" / "vx", "vx", snap, fvx, err))
+                if (!write_field(sd / "solid" / "vz", "vz", snap, svz, err) || !write_field(sd / "solid" / "vx", "vx", snap, svx, err))
+                    return fail(err);
+                if (fluid && (!write_field(sd / "fluid" / "vz", "vz", snap, fvz, err) ||
+                              !write_field(sd / "fluid" / "vx", "vx", snap, fvx, err)))
                     return fail(err);
             }
         }
@@ -123,13 +134,17 @@ extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
         std::vector<float> bx((size_t)chunk * nx), bz((size_t)chunk * nx), bp((size_t)chunk * nx);
         for (int r0 = 0; r0 < nrec; r0 += chunk) {
             const int n = std::min(chunk, nrec - r0);
-            if (msfdm_get_records(solver.get(), r0, n, bx.data(), bz.data(), bp.data()) != MSFDM_OK) return 1;
+            if (msfdm_get_records(solver.get(), r0, n, fluid ? bx.data() : nullptr, bz.data(), fluid ? bp.data() : nullptr) !=
+                MSFDM_OK)
+                return 1;
             const std::streamsize bytes = (std::streamsize)((size_t)n * nx * sizeof(float));
-            rvx.write(reinterpret_cast<const char *>(bx.data()), bytes);
             rvz.write(reinterpret_cast<const char *>(bz.data()), bytes);
-            rp.write(reinterpret_cast<const char *>(bp.data()), bytes);
-            if (!rp) return fail("failed to write fluid record data");
-            if (!rvx || !rvz) return fail("failed to write solid record data");
+            if (fluid) {
+                rvx.write(reinterpret_cast<const char *>(bx.data()), bytes);
+                rp.write(reinterpret_cast<const char *>(bp.data()), bytes);
+                if (!rp) return fail("failed to write fluid record data");
+            }
+            if ((fluid && !rvx) || !rvz) return fail("failed to write solid record data");
         }
     }
     std::printf("\n");

@@ -20,13 +20,16 @@ int set_error(int code, const std::string &msg) {
 void clear_error() { g_error.clear(); }
 
 int array_rows(const msfdm_desc &d, int which) {
+    const bool solid_only = d.calc_type == 0;   // src/elements.cu:87-99: nz = nz_solid
     switch (which) {
-        case MSFDM_A_K: case MSFDM_A_RHO_F: return d.nz_fluid;
+        case MSFDM_A_K: case MSFDM_A_RHO_F: return solid_only ? 0 : d.nz_fluid;
         case MSFDM_A_C11: case MSFDM_A_C13: case MSFDM_A_C33: case MSFDM_A_C55: case MSFDM_A_RHO_S:
             return d.nz_solid;
-        default: return d.nz_fluid + d.nz_solid - 1;
+        default: return solid_only ? d.nz_solid : d.nz_fluid + d.nz_solid - 1;
     }
 }
+
+int source_rows(const msfdm_desc &d) { return d.calc_type == 0 ? d.nz_solid : d.nz_fluid; }
 
 int record_gap(float dt) {
     const int g = (int)(1e-3 / (double)dt + 0.5);
@@ -34,16 +37,23 @@ int record_gap(float dt) {
 }
 
 int validate_desc(const msfdm_desc &d) {
-    if (d.calc_type != 1 || d.boundary_type != 2)
-        return set_error(MSFDM_EINVAL, "only grid.calc_type = 1 (fluid over solid) with grid.boundary_type = 2 "
-                                       "(PML sides/bottom + free surface) is supported, got calc_type=" +
+    if ((d.calc_type != 0 && d.calc_type != 1) || d.boundary_type != 2)
+        return set_error(MSFDM_EINVAL, "only grid.boundary_type = 2 (PML sides/bottom + free surface) with "
+                                       "grid.calc_type = 1 (fluid over solid) or 0 (solid only) is supported, got calc_type=" +
                                            std::to_string(d.calc_type) + " boundary_type=" + std::to_string(d.boundary_type));
     if (d.npd < 1) return set_error(MSFDM_EINVAL, "PML.npd must be >= 1");
     if (d.nx < 2 * d.npd + 8)
         return set_error(MSFDM_EINVAL, "grid nx=" + std::to_string(d.nx) + " too small for npd=" + std::to_string(d.npd));
-    if (d.nz_fluid < 8) return set_error(MSFDM_EINVAL, "grid.fluid.nz must be >= 8 (free surface row 3 + interface)");
-    if (d.nz_solid < d.npd + 8)
-        return set_error(MSFDM_EINVAL, "grid.solid.nz=" + std::to_string(d.nz_solid) + " too small for npd=" + std::to_string(d.npd));
+    if (d.calc_type == 0) {
+        // the free surface (row 3) and its three conservative rows stay above the bottom PML
+        if (d.nz_fluid != 0) return set_error(MSFDM_EINVAL, "a solid-only grid (calc_type 0) has nz_fluid = 0");
+        if (d.nz_solid < d.npd + 10)
+            return set_error(MSFDM_EINVAL, "grid.solid.nz=" + std::to_string(d.nz_solid) + " too small for npd=" + std::to_string(d.npd));
+    } else {
+        if (d.nz_fluid < 8) return set_error(MSFDM_EINVAL, "grid.fluid.nz must be >= 8 (free surface row 3 + interface)");
+        if (d.nz_solid < d.npd + 8)
+            return set_error(MSFDM_EINVAL, "grid.solid.nz=" + std::to_string(d.nz_solid) + " too small for npd=" + std::to_string(d.npd));
+    }
     if (!(d.dt > 0.0f)) return set_error(MSFDM_EINVAL, "base.dt must be > 0");
     if (!(d.fpeak > 0.0f)) return set_error(MSFDM_EINVAL, "base.fpeak must be > 0");
     if (d.snapshot <= 0) return set_error(MSFDM_EINVAL, "base.snapshot must be > 0");
@@ -105,8 +115,9 @@ extern "C" int msfdm_model_load_json(const char *json_path, msfdm_model **out) {
         d.nsrc = (int)m->src_x.size();
         if (int rc = validate_desc(d)) return rc;
         for (size_t i = 0; i < m->src_x.size(); ++i) {
-            if (m->src_x[i] < 0 || m->src_x[i] >= d.nx || m->src_z[i] < 0 || m->src_z[i] >= d.nz_fluid)
-                return set_error(MSFDM_EINVAL, "source " + std::to_string(i) + " outside the fluid grid");
+            if (m->src_x[i] < 0 || m->src_x[i] >= d.nx || m->src_z[i] < 0 || m->src_z[i] >= source_rows(d))
+                return set_error(MSFDM_EINVAL, "source " + std::to_string(i) +
+                                                   (d.calc_type == 0 ? " outside the solid grid" : " outside the fluid grid"));
         }
         // src/elements.cu:127-176: 12 arrays, CWD-relative paths.
         static const struct { const char *key; int which; } files[] = {
@@ -115,6 +126,8 @@ extern "C" int msfdm_model_load_json(const char *json_path, msfdm_model **out) {
             {"grid.fluid.rho", MSFDM_A_RHO_F}, {"grid.jacob.J", MSFDM_A_J}, {"grid.jacob.xi_x", MSFDM_A_XI_X},
             {"grid.jacob.xi_z", MSFDM_A_XI_Z}, {"grid.jacob.ze_x", MSFDM_A_ZE_X}, {"grid.jacob.ze_z", MSFDM_A_ZE_Z}};
         for (const auto &f : files) {
+            // a solid-only grid has no fluid arrays (src/elements.cu:153-154)
+            if (d.calc_type == 0 && (f.which == MSFDM_A_K || f.which == MSFDM_A_RHO_F)) continue;
             const std::string path = J.get_string(f.key);
             const size_t count = (size_t)array_rows(d, f.which) * (size_t)d.nx;
             std::vector<float> &v = m->a[f.which];
@@ -143,20 +156,21 @@ extern "C" int msfdm_model_create(const msfdm_desc *desc, const float *K, const 
     if (int rc = validate_desc(*desc)) return rc;
     const float *src[MSFDM_NARRAY] = {K, rho_f, C11, C13, C33, C55, rho_s, J, xi_x, xi_z, ze_x, ze_z};
     for (int i = 0; i < MSFDM_NARRAY; ++i)
-        if (!src[i]) return set_error(MSFDM_EINVAL, "null model array " + std::to_string(i));
+        if (!src[i] && array_rows(*desc, i) > 0) return set_error(MSFDM_EINVAL, "null model array " + std::to_string(i));
     if (desc->nsrc > 0 && (!desc->src_x || !desc->src_z)) return set_error(MSFDM_EINVAL, "null source arrays");
     try {
         std::unique_ptr<msfdm_model> m(new msfdm_model());
         m->d = *desc;
         for (int i = 0; i < desc->nsrc; ++i) {
-            if (desc->src_x[i] < 0 || desc->src_x[i] >= desc->nx || desc->src_z[i] < 0 || desc->src_z[i] >= desc->nz_fluid)
-                return set_error(MSFDM_EINVAL, "source " + std::to_string(i) + " outside the fluid grid");
+            if (desc->src_x[i] < 0 || desc->src_x[i] >= desc->nx || desc->src_z[i] < 0 || desc->src_z[i] >= source_rows(*desc))
+                return set_error(MSFDM_EINVAL, "source " + std::to_string(i) +
+                                                   (desc->calc_type == 0 ? " outside the solid grid" : " outside the fluid grid"));
             m->src_x.push_back(desc->src_x[i]);
             m->src_z.push_back(desc->src_z[i]);
         }
         for (int i = 0; i < MSFDM_NARRAY; ++i) {
             const size_t n = (size_t)array_rows(*desc, i) * (size_t)desc->nx;
-            m->a[i].assign(src[i], src[i] + n);
+            if (n > 0) m->a[i].assign(src[i], src[i] + n);
         }
         return finish_model(m, out);
     } catch (const std::bad_alloc &) {

@@ -21,6 +21,10 @@ void clear_error();
 // Rows of model array `which` (fluid, solid or global grid rows).
 int array_rows(const msfdm_desc &d, int which);
 
+// Rows a source row index src_z refers to: the fluid array of a fluid-solid
+// grid, the solid array of a solid-only grid.
+int source_rows(const msfdm_desc &d);
+
 // Validate the scalar description (grid sizes, PML fits, supported modes).
 // Returns MSFDM_OK or sets the error.
 int validate_desc(const msfdm_desc &d);

@@ -73,6 +73,7 @@ struct msfdm_solver {
     std::vector<float *> ckpt;       // checkpoints of the full state every adj_every steps
     std::vector<float *> seg;        // per-step U^n, T2, T3, T4 of one segment
     int adj_every = 0, adj_nsteps = 0, adj_isrc = 0;
+    bool solid_only = false;         // grid.calc_type = 0: solid grid with a traction-free top
 };
 
 namespace {
@@ -171,6 +172,18 @@ template <int DIR, int STAGE, int TZ>
 void launch_media(msfdm_solver *s, const float *S, const float *U, const float *T2, const float *T3, float *O, int j,
                   float ct, float amp, float cprev) {
     const int nbx = (s->d.nx - 4) / D::TX + 1;
+    if (s->solid_only) {   // rows 3..nzs-4 of the solid-only grid
+        const int nb = (s->d.nz_solid - 6 + TZ - 1) / TZ;
+        const int ntiles = nbx * nb;
+        const int grid = 8 * ((ntiles + 7) / 8);
+        if (s->coef.vert)
+            D::free_stage_kernel<DIR, STAGE, TZ, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3,
+                                                                                  O, j, ct, amp, nbx, ntiles);
+        else
+            D::free_stage_kernel<DIR, STAGE, TZ, 0><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3,
+                                                                                  O, j, ct, amp, nbx, ntiles);
+        return;
+    }
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
     const int grid = 8 * ((ntiles + 7) / 8);
@@ -189,6 +202,7 @@ void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, fl
     const float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3];
     if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, T2, T3, O, j, ct, amp, cprev);
     else launch_media<DIR, STAGE, 16>(s, S, U, T2, T3, O, j, ct, amp, cprev);
+    if (s->solid_only) return;   // no interface: the stage kernel writes every row and the record
     if constexpr (STAGE == 4) {
         const float dt = s->d.dt;
         D::iface_final<DIR><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, O, s->kif,
@@ -301,6 +315,13 @@ void free_all(msfdm_solver *s) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
+// The batched, adjoint and multi-shot paths serve the fluid-solid grid only.
+int require_fluid_solid(const msfdm_solver *s) {
+    if (s->solid_only)
+        return set_error(MSFDM_EINVAL, "solid-only grids (grid.calc_type = 0) support the forward stepper only");
+    return MSFDM_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -324,12 +345,14 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     s->d.src_x = s->srcx.data();
     s->d.src_z = s->srcz.data();
     s->gap = record_gap(s->d.dt);
+    s->solid_only = s->d.calc_type == 0;
+    if (s->solid_only) s->d.nz_fluid = 0;
     auto fail = [&](int rc) { free_all(s); delete s; return rc; };
 #define CK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return fail(set_error(MSFDM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); } while (0)
     CK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
 
     const int nx = s->d.nx, nzf = s->d.nz_fluid, nzs = s->d.nz_solid, npd = s->d.npd;
-    const int nz = nzf + nzs - 1;
+    const int nz = s->solid_only ? nzs : nzf + nzs - 1;   // src/elements.cu:87-99
     const long long P = align64(nx);
     D::Geo &g = s->geo;
     g.nx = nx; g.nzf = nzf; g.nzs = nzs; g.P = (int)P; g.npd = npd; g.W2 = 2 * npd;
@@ -357,7 +380,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // Coefficients, padded to pitch P (host prep in double where derived).
     const long long nF = align64(nzf * P), nS = align64(nzs * P), nG = align64(nz * P);
     const long long nX = align64(P), nZ = align64(nzs);
-    const long long total = 2 * nF + 5 * nS + 5 * nG + 3 * nX + 3 * nZ + 2 * nX;
+    const long long total = 2 * nF + 5 * nS + 5 * nG + 3 * nX + 3 * nZ + 2 * nX + 6 * nX;
     CK(hipMalloc(&s->coefmem, total * sizeof(float)));
     CK(hipMemset(s->coefmem, 0, total * sizeof(float)));
     std::vector<float> h;
@@ -365,7 +388,8 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     auto upload = [&](const std::vector<float> &v, long long n) -> const float * {
         float *p = cur;
         cur += n;
-        if (hipMemcpy(p, v.data(), std::min<long long>((long long)v.size(), n) * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        const long long cnt = std::min<long long>((long long)v.size(), n);
+        if (cnt > 0 && hipMemcpy(p, v.data(), cnt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
             return nullptr;
         return p;
     };
@@ -405,9 +429,9 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     ptrs[np++] = c.pza = upload(pa, nZ);
     ptrs[np++] = c.pzb = upload(pb, nZ);
     ptrs[np++] = c.pzc = upload(pc, nZ);
-    // Interface normal (src/elements.cu:892-905) in double.
-    std::vector<float> nif(2 * (size_t)nx);
-    for (int ix = 0; ix < nx; ++ix) {
+    // Interface normal (src/elements.cu:892-905) in double (zero on a solid-only grid).
+    std::vector<float> nif(2 * (size_t)nx, 0.0f);
+    for (int ix = 0; ix < nx && !s->solid_only; ++ix) {
         const size_t gi = (size_t)(nzf - 1) * nx + ix;
         const double zx = m->a[MSFDM_A_ZE_X][gi], zz = m->a[MSFDM_A_ZE_Z][gi];
         const double n = std::sqrt(zx * zx + zz * zz);
@@ -425,6 +449,33 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     const float *nifp = upload(nif, 2 * nX);
     c.nif = reinterpret_cast<const float2 *>(nifp);
     ptrs[np++] = nifp;
+    // Free-surface ALM coefficients of a solid-only grid (src/elements.cu:907-977):
+    // double arithmetic on the float model values of row 3, rounded to float.
+    std::vector<float> alm(6 * (size_t)P, 0.0f);
+    for (int ix = 0; ix < nx && s->solid_only; ++ix) {
+        const size_t i = (size_t)3 * nx + ix;
+        const double xx = m->a[MSFDM_A_XI_X][i], xz = m->a[MSFDM_A_XI_Z][i];
+        const double zx = m->a[MSFDM_A_ZE_X][i], zz = m->a[MSFDM_A_ZE_Z][i];
+        const double c11 = m->a[MSFDM_A_C11][i], c13 = m->a[MSFDM_A_C13][i];
+        const double c33 = m->a[MSFDM_A_C33][i], c55 = m->a[MSFDM_A_C55][i];
+        const double dz11 = c11 * zx * zx + c55 * zz * zz, dz12 = (c13 + c55) * zx * zz;
+        const double dz22 = c55 * zx * zx + c33 * zz * zz;
+        const double dx11 = c11 * zx * xx + c55 * zz * xz, dx12 = c13 * zx * xz + c55 * zz * xx;
+        const double dx21 = c13 * zz * xx + c55 * zx * xz, dx22 = c55 * zx * xx + c33 * zz * xz;
+        const double det = dz11 * dz22 - dz12 * dz12;
+        double m11 = 0, m12 = 0, m21 = 0, m22 = 0;
+        if (std::abs(det) >= 1e-14) {
+            const double i11 = dz22 / det, i12 = -dz12 / det, i22 = dz11 / det;
+            m11 = i11 * dx11 + i12 * dx21;
+            m12 = i11 * dx12 + i12 * dx22;
+            m21 = i12 * dx11 + i22 * dx21;
+            m22 = i12 * dx12 + i22 * dx22;
+        }
+        const double al[6] = {xx - zx * m11, -zz * m21, xz - zz * m11 - zx * m21,
+                              -zx * m12, xz - zz * m22, xx - zz * m12 - zx * m22};
+        for (int k = 0; k < 6; ++k) alm[(size_t)k * P + ix] = (float)al[k];
+    }
+    ptrs[np++] = c.alm = upload(alm, 6 * nX);
     for (int i = 0; i < np; ++i)
         if (!ptrs[i]) return fail(set_error(MSFDM_EDEVICE, "coefficient upload failed"));
 
@@ -443,8 +494,11 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // T4, U, T2, T3, write U) + 4 coefficient floats (fluid 3 fields /
     // 6 coefficients, solid 5 / 9) per updated cell, plus 13 floats x 4
     // memory fields per active PML direction.
-    const double cf = (double)(nzf - 3) * (nx - 6), cs = (double)(nzs - 3) * (nx - 6);
-    const double pmlf = (double)(nzf - 3) * 2 * npd, pmls = (double)(nzs - 3) * 2 * npd + (double)npd * (nx - 6);
+    // Updated rows: fluid 3..nzf-1, solid 0..nzs-4 (solid only: 3..nzs-4).
+    const int srows = s->solid_only ? nzs - 6 : nzs - 3;
+    const double cf = s->solid_only ? 0.0 : (double)(nzf - 3) * (nx - 6), cs = (double)srows * (nx - 6);
+    const double pmlf = s->solid_only ? 0.0 : (double)(nzf - 3) * 2 * npd;
+    const double pmls = (double)srows * 2 * npd + (double)npd * (nx - 6);
     // Vertical-column grids read neither xi_x nor xi_z: 2 coefficients fewer.
     const double ncf = c.vert ? 4 : 6, ncs = c.vert ? 7 : 9;
     s->bytes_step = 4.0 * (cf * (13 * 3 + 4 * ncf) + cs * (13 * 5 + 4 * ncs) + (pmlf + pmls) * 13 * 4);
@@ -453,7 +507,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // stage-4 launch: 5 state floats per field (read T4, U^n, T2, T3, write U) + coefficients
     s->bytes_kernel[1] = 4.0 * (cf * (5 * 3 + ncf) + cs * (5 * 5 + ncs) + (pmlf + pmls) * 5 * 4);
     // interface / filter / record launch: 4 k sets + U^n, normal, output rows
-    s->bytes_kernel[2] = 4.0 * (double)(nx - 6) * (8 * 5 + 2 + 8);
+    s->bytes_kernel[2] = s->solid_only ? 0.0 : 4.0 * (double)(nx - 6) * (8 * 5 + 2 + 8);
     CK(hipStreamSynchronize(s->stream));
 #undef CK
     *out = s;
@@ -628,6 +682,14 @@ extern "C" int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms
         HIPCHK(hipMemset(s->kbuf, 0, s->buf_floats * sizeof(float)));
     }
     float *X = s->kbuf;
+    if (s->solid_only && which == 2) return set_error(MSFDM_EINVAL, "a solid-only grid has no interface kernel");
+    // no record slot is written while timing (the solid-only stage-4 kernel records row 3)
+    struct CtlRestore {
+        msfdm_solver *s;
+        D::Ctl c;
+        ~CtlRestore() { s->ctl = c; }
+    } restore{s, s->ctl};
+    s->ctl.gap = 0x7fffffff;
     auto one = [&]() {
         switch (which) {
             case 0:   // RK stage 2: reads T2 (+ U^n), writes the next stage state
@@ -816,6 +878,9 @@ extern "C" int msfdm_batch_create(const msfdm_model *const *models, const int *s
     clear_error();
     if (!models || !out || nshots <= 0) return set_error(MSFDM_EINVAL, "bad argument");
     *out = nullptr;
+    for (int i = 0; i < nshots; ++i)
+        if (models[i] && models[i]->d.calc_type == 0)
+            return set_error(MSFDM_EINVAL, "solid-only grids (grid.calc_type = 0) support the forward stepper only");
     msfdm_batch *b = new msfdm_batch();
     b->device = device;
     auto fail = [&](int rc) { std::string msg = msfdm_last_error(); batch_free(b); delete b; return set_error(rc, msg); };
@@ -1040,6 +1105,7 @@ namespace {
 enum AdjRole { A_LAM = 0, A_LP = 1, A_ACC = 2, A_TBA = 3, A_TBB = 4 };
 
 int ensure_adjoint(msfdm_solver *s) {
+    if (int rc = require_fluid_solid(s)) return rc;
     const size_t bytes = s->buf_floats * sizeof(float);
     for (auto &p : s->adj)
         if (!p) { HIPCHK(hipMalloc(&p, bytes)); HIPCHK(hipMemset(p, 0, bytes)); }
@@ -1273,6 +1339,7 @@ extern "C" int msfdm_update_media(msfdm_solver *s, const float *K, const float *
                                   const float *C13, const float *C33, const float *C55, const float *rho_s) {
     clear_error();
     if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (int rc = require_fluid_solid(s)) return rc;
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipStreamSynchronize(s->stream));
     const int nx = s->d.nx, P = s->geo.P, nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
@@ -1540,6 +1607,7 @@ extern "C" int msfdm_gradient_shots(msfdm_solver *s, int nshot, const int *isrc,
                                     double *gC13, double *gC33, double *gC55, double *grho_s, int *wave_shots) {
     clear_error();
     if (!s || !isrc || !res || nshot <= 0 || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    if (int rc = require_fluid_solid(s)) return rc;
     const int nsrc = s->srcx.empty() ? 1 : (int)s->srcx.size();
     for (int i = 0; i < nshot; ++i)
         if (isrc[i] < 0 || isrc[i] >= nsrc) return set_error(MSFDM_EINVAL, "shot index out of range");
@@ -1586,6 +1654,7 @@ extern "C" int msfdm_gradient_shots_plan(const msfdm_solver *s, int nsteps, int 
                                          int *every_used) {
     clear_error();
     if (!s || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
+    if (int rc = require_fluid_solid(s)) return rc;
     const ShotsPlan p = plan_shots(s, nsteps, every);
     if (bytes_per_shot) *bytes_per_shot = (double)p.per_shot;
     if (every_used) *every_used = p.every;

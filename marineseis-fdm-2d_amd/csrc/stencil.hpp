@@ -15,6 +15,12 @@
 //   iface_final<DIR>           stage 4: the interface rows of U' (fl_so_boundary of
 //                              all four stages), fl_so_filter (src/pml_fs_kernels.cu:428-525,
 //                              out of place) and the receiver-row record.
+//   free_stage_kernel<DIR,STAGE,TZ,MET>
+//                              the solid-only grid with a traction-free top
+//                              (PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307):
+//                              solid_body<..., FREE = true>, ALM surface row, odd-image
+//                              ghost fluxes, source on both normal stresses, and the
+//                              row-3 vz record in stage 4 (one launch per stage).
 //
 // The stage kernels read the stage state once through an LDS tile with a 3-cell
 // halo on every side, build the interface / free-surface ghost fluxes
@@ -64,6 +70,7 @@ struct Coef {
     const float *pxa, *pxb, *pxc;                        // xi PML: al+d/be, d/be, 1/be  [nx]
     const float *pza, *pzb, *pzc;                        // solid zeta PML               [nzs]
     const float2 *nif;                                   // interface normal            [nx]
+    const float *alm;                                    // free-surface ALM, solid-only grid [6][P]
     // Vertical grid columns (x = h xi everywhere): xi_x is the constant xix0 and
     // xi_z = 0, so neither array is read.
     int vert;
@@ -167,19 +174,24 @@ __device__ __forceinline__ void rk(const float *__restrict__ U, const float *__r
 template <int STAGE>
 __device__ __forceinline__ constexpr bool prefetch_u() { return STAGE == 2 || STAGE == 3 || (STAGE == 4 && MSFDM_PREFETCH_U4); }
 template <int STAGE>
-__device__ __forceinline__ void rk_u(const float *__restrict__ U, const float *__restrict__ T2,
-                                     const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
-                                     float k, float ct, float u) {
+__device__ __forceinline__ float rk_u(const float *__restrict__ U, const float *__restrict__ T2,
+                                      const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
+                                      float k, float ct, float u) {
+    float v;
     if constexpr (STAGE == 0) {
-        O[i] = k;
+        v = k;
+        O[i] = v;
+        return v;
     } else if constexpr (STAGE == 1) {
-        store_out(O + i, s + ct * k);
+        v = s + ct * k;
     } else if constexpr (STAGE == 4) {
         if constexpr (!prefetch_u<STAGE>()) u = U[i];
-        store_out(O + i, u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k);
+        v = u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k;
     } else {
-        store_out(O + i, u + ct * k);
+        v = u + ct * k;
     }
+    store_out(O + i, v);
+    return v;
 }
 
 // Pointwise inputs of one row of a thread, loaded one row ahead of their use so
@@ -527,32 +539,42 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 }
 
 // ---------------------------------------------------------------------------
-// Solid stage: rows 0..nzs-4, columns 3..nx-4.  Metrics at global row iz+nzf-1.
+// Solid stage.  FREE = false: the solid below the fluid, rows 0..nzs-4 (row 0
+// the interface), metrics at global row iz+nzf-1 (src/tools_func.cu:22, 57).
+// FREE = true: a solid-only grid (PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307),
+// rows 3..nzs-4 with the traction-free surface on row 3, metrics at row iz, the
+// source on both normal stresses and the receiver row 3 recorded by stage 4.
+// Columns 3..nx-4 in both.
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ, int MET>
+template <int DIR, int STAGE, int TZ, int MET, bool FREE = false>
 __device__ __forceinline__ void
-solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const float *__restrict__ S,
+solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
            const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
-           float *__restrict__ O, float *__restrict__ kif, float ct, float cprev, int bx, int bz) {
+           float *__restrict__ O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
+           int bz) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    const int x0 = bx * TX, z0 = bz * TZ;
+    const int x0 = bx * TX, z0 = (FREE ? 3 : 0) + bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     const int ix = x0 + tx;
     const bool active = ix >= 3 && ix <= nx - 4;
     const int scol = side_col(ix, nx, npd);
-    const long long moff = (long long)(g.nzf - 1) * P;   // solid row -> global metric row
+    const long long moff = FREE ? 0 : (long long)(g.nzf - 1) * P;   // solid row -> global metric row
+    constexpr int ztop = FREE ? 3 : 0;                               // first updated row
     float pxa = 0.f, pxb = 0.f, pxc = 0.f;
+    float amp = 0.0f;
     const float *const cp[5] = {c.C11, c.C13, c.C33, c.C55, c.rhos_inv};
     RowPre<5, 5> nxt;
     if (active) {
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
+        if constexpr (FREE)
+            amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
         const long long a0 = (long long)min(z0 + ty, nzs - 4) * P + ix;
         prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_SVX, a0, moff + a0);
     }
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
-    if constexpr (STAGE >= 2) {
+    if constexpr (STAGE >= 2 && !FREE) {
         if (z0 == 0) {   // the band holding solid row 0 (tile row HALO)
             rebuild_interface_row<DIR, STAGE, 5, TZ>(sh, g, c, S, kif, HALO, 3, x0, cprev, true);
             __syncthreads();
@@ -583,21 +605,36 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
         const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
         const bool side = scol >= 0, bottom = iz >= zb0;
+        const int dist = iz - ztop;       // rows below the interface / free surface
+        const bool surf = FREE && dist == 0;
 
-        // ---- stress (include/pml_fs_kernels.cuh:509-628)
+        // ---- stress (include/pml_fs_kernels.cuh:509-628; solid-only: 10-187)
         const float dvxx = dx_lds<DIR>(svx[r], cc);
-        const float dvxz = mac_dz<DIR>(svx, r, cc, iz, true);
         const float dvzx = dx_lds<DIR>(svz[r], cc);
-        const float dvzz = mac_dz<DIR>(svz, r, cc, iz, true);
+        float dvxz = 0.0f, dvzz = 0.0f;
         const float c11 = cur.cf[0], c13 = cur.cf[1], c33 = cur.cf[2], c55 = cur.cf[3];
-        const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
-        float ksx = c11 * exx + c13 * ezz;
-        float ksz = c13 * exx + c33 * ezz;
-        float ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+        float ksx, ksz, ksxz;
+        float a1xx = 0.f, a1zz = 0.f, a1xz = 0.f, a2xx = 0.f, a2zz = 0.f, a2xz = 0.f;
+        if (surf) {
+            // traction-free row: zeta derivatives eliminated through the ALM
+            // coefficients (src/elements.cu:907-977)
+            a1xx = c.alm[0 * (long long)P + ix]; a1zz = c.alm[1 * (long long)P + ix]; a1xz = c.alm[2 * (long long)P + ix];
+            a2xx = c.alm[3 * (long long)P + ix]; a2zz = c.alm[4 * (long long)P + ix]; a2xz = c.alm[5 * (long long)P + ix];
+            ksx = (c11 * a1xx + c13 * a1zz) * dvxx + (c11 * a2xx + c13 * a2zz) * dvzx;
+            ksz = (c13 * a1xx + c33 * a1zz) * dvxx + (c13 * a2xx + c33 * a2zz) * dvzx;
+            ksxz = c55 * (a1xz * dvxx + a2xz * dvzx);
+        } else {
+            dvxz = mac_dz<DIR>(svx, r, cc, dist, true);
+            dvzz = mac_dz<DIR>(svz, r, cc, dist, true);
+            const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
+            ksx = c11 * exx + c13 * ezz;
+            ksz = c13 * exx + c33 * ezz;
+            ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+        }
 
-        // ---- velocity (include/pml_fs_kernels.cuh:631-752)
+        // ---- velocity (include/pml_fs_kernels.cuh:631-752; solid-only: 189-305)
         float rvx, rvz, Dxx, Dxz, Dzx, Dzz;
-        if (iz < 3) {
+        if (dist < 3) {
             // zeta traction flux at real solid row rr of this column
             auto fze_row = [&](int rr, float &fx, float &fz) {
                 const long long q = moff + (long long)rr * P + ix;
@@ -618,13 +655,23 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
                 ax[q] = Jq * (xxq * ssx[r][cc + m] + xzq * ssxz[r][cc + m]);
                 az[q] = Jq * (xxq * ssxz[r][cc + m] + xzq * ssz[r][cc + m]);
                 const int rr = iz + m;
-                if (rr >= 0) fze_row(rr, bx[q], bz[q]);
-                else {   // Gs ghost n = -rr: 2 F(0) - F(n)
-                    float fx0, fz0, fxn, fzn;
-                    fze_row(0, fx0, fz0);
-                    fze_row(-rr, fxn, fzn);
-                    bx[q] = 2.0f * fx0 - fxn;
-                    bz[q] = 2.0f * fz0 - fzn;
+                if constexpr (FREE) {
+                    if (rr >= 3) fze_row(rr, bx[q], bz[q]);
+                    else {   // Gfs ghost n = 3 - rr: -F(3 + n) (src/pml_fs_kernels.cu:7-19)
+                        float fxn, fzn;
+                        fze_row(6 - rr, fxn, fzn);
+                        bx[q] = -fxn;
+                        bz[q] = -fzn;
+                    }
+                } else {
+                    if (rr >= 0) fze_row(rr, bx[q], bz[q]);
+                    else {   // Gs ghost n = -rr: 2 F(0) - F(n)
+                        float fx0, fz0, fxn, fzn;
+                        fze_row(0, fx0, fz0);
+                        fze_row(-rr, fxn, fzn);
+                        bx[q] = 2.0f * fx0 - fxn;
+                        bz[q] = 2.0f * fz0 - fzn;
+                    }
                 }
             }
             const float sx = drp5<DIR>(ax[0], ax[1], ax[2], ax[3], ax[4]) + drp5<DIR>(bx[0], bx[1], bx[2], bx[3], bx[4]);
@@ -632,10 +679,23 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             const float Jc = c.J[gi];
             rvx = sx / Jc;
             rvz = sz / Jc;
-            const float pxi = (xx * sx + xz * sz) / (gxixi * Jc);
-            const float pze = (zx * sx + zz * sz) / (gzeze * Jc);
-            Dxx = xx * pxi; Dxz = xz * pxi;
-            Dzx = zx * pze; Dzz = zz * pze;
+            if constexpr (FREE) {
+                // xi-memory source from distance-dependent zeta derivatives, no
+                // zeta-memory source (include/pml_fs_kernels.cuh:212-233)
+                const float dxsx = dx_lds<DIR>(ssx[r], cc), dxsz = dx_lds<DIR>(ssz[r], cc);
+                const float dxsxz = dx_lds<DIR>(ssxz[r], cc);
+                const float dzsx = mac_dz<DIR>(ssx, r, cc, dist, true), dzsz = mac_dz<DIR>(ssz, r, cc, dist, true);
+                const float dzsxz = mac_dz<DIR>(ssxz, r, cc, dist, true);
+                Dxx = xx * (dxsx + gxize / gxixi * dzsx) + xz * (dxsxz + gxize / gxixi * dzsxz);
+                Dxz = xx * (dxsxz + gxize / gxixi * dzsxz) + xz * (dxsz + gxize / gxixi * dzsz);
+                Dzx = 0.0f;
+                Dzz = 0.0f;
+            } else {
+                const float pxi = (xx * sx + xz * sz) / (gxixi * Jc);
+                const float pze = (zx * sx + zz * sz) / (gzeze * Jc);
+                Dxx = xx * pxi; Dxz = xz * pxi;
+                Dzx = zx * pze; Dzz = zz * pze;
+            }
         } else {
             const float a = dx_lds<DIR>(ssx[r], cc), b = dz_lds<DIR>(ssx, r, cc);
             const float cz = dx_lds<DIR>(ssz[r], cc), d = dz_lds<DIR>(ssz, r, cc);
@@ -656,14 +716,25 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             const long long ai = (long long)iz * g.W2 + scol;
             const float Vx = S[g.off[MSFDM_SXVX] + ai], Vz = S[g.off[MSFDM_SXVZ] + ai];
             const float Ex = S[g.off[MSFDM_SXEX] + ai], Ez = S[g.off[MSFDM_SXEZ] + ai];
-            const float Dvx = dvxx + (gxize / gxixi) * dvxz;
-            const float Dvz = dvzx + (gxize / gxixi) * dvzz;
-            const float kEx = -pxa * Ex + pxb * Dvx;
-            const float kEz = -pxa * Ez + pxb * Dvz;
             const float q1 = pxc - 1.0f;
-            ksx += c11 * q1 * xx * Dvx + c13 * q1 * xz * Dvz - pxc * c11 * xx * Ex - pxc * c13 * xz * Ez;
-            ksz += c13 * q1 * xx * Dvx + c33 * q1 * xz * Dvz - pxc * c13 * xx * Ex - pxc * c33 * xz * Ez;
-            ksxz += c55 * q1 * (xz * Dvx + xx * Dvz) - pxc * c55 * xz * Ex - pxc * c55 * xx * Ez;
+            float kEx, kEz;
+            if (surf) {   // include/pml_fs_kernels.cuh:50-79: memory of the xi derivatives
+                kEx = -pxa * Ex + pxb * dvxx;
+                kEz = -pxa * Ez + pxb * dvzx;
+                const float cxx_x = c11 * a1xx + c13 * a1zz, cxx_z = c11 * a2xx + c13 * a2zz;
+                const float czz_x = c13 * a1xx + c33 * a1zz, czz_z = c13 * a2xx + c33 * a2zz;
+                ksx += q1 * (cxx_x * dvxx + cxx_z * dvzx) - pxc * (cxx_x * Ex + cxx_z * Ez);
+                ksz += q1 * (czz_x * dvxx + czz_z * dvzx) - pxc * (czz_x * Ex + czz_z * Ez);
+                ksxz += c55 * (q1 * (a1xz * dvxx + a2xz * dvzx) - pxc * (a1xz * Ex + a2xz * Ez));
+            } else {
+                const float Dvx = dvxx + (gxize / gxixi) * dvxz;
+                const float Dvz = dvzx + (gxize / gxixi) * dvzz;
+                kEx = -pxa * Ex + pxb * Dvx;
+                kEz = -pxa * Ez + pxb * Dvz;
+                ksx += c11 * q1 * xx * Dvx + c13 * q1 * xz * Dvz - pxc * c11 * xx * Ex - pxc * c13 * xz * Ez;
+                ksz += c13 * q1 * xx * Dvx + c33 * q1 * xz * Dvz - pxc * c13 * xx * Ex - pxc * c33 * xz * Ez;
+                ksxz += c55 * q1 * (xz * Dvx + xx * Dvz) - pxc * c55 * xz * Ex - pxc * c55 * xx * Ez;
+            }
             const float kVx = -pxa * Vx + pxb * Dxx;
             const float kVz = -pxa * Vz + pxb * Dxz;
             kvx += (q1 * Dxx - pxc * Vx) * rinv;
@@ -696,7 +767,17 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZEX] + ai, Ex, kEx, ct);
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SZEZ] + ai, Ez, kEz, ct);
         }
-        if (iz == 0) {
+        // ---- source: 5x5 Gaussian-weighted Ricker on both normal stresses
+        // (solid only, src/pml_fs_kernels.cu:22-27, include/tools_func.cuh:21-37)
+        if constexpr (FREE) {
+            const int sxo = ix - ctl.srcx, szo = iz - ctl.srcz;
+            if (sxo >= -2 && sxo <= 2 && szo >= -2 && szo <= 2) {
+                const float w = amp * SRC_W[szo + 2][sxo + 2];
+                ksx += w;
+                ksz += w;
+            }
+        }
+        if (!FREE && iz == 0) {
             float *kk = kif + (long long)kif_set(STAGE) * P;
             kk[3 * (long long)P + ix] = kvx;
             kk[4 * (long long)P + ix] = kvz;
@@ -713,7 +794,15 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const f
             if constexpr (STAGE != 0) continue;
         }
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cur.u[0]);
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cur.u[1]);
+        const float vz_new = rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cur.u[1]);
+        if constexpr (FREE && STAGE == 4) {
+            // receiver row 3, vz after the step (src/main.cu:297-322), plane 1 of the record buffer
+            const int it = ctl.base[0] + jstep;
+            if (surf && it % ctl.gap == 0) {
+                const int slot = it / ctl.gap;
+                if (slot < ctl.rec_cap) ctl.rec[(long long)ctl.rec_cap * nx + (long long)slot * nx + ix] = vz_new;
+            }
+        }
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct, cur.u[2]);
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cur.u[3]);
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cur.u[4]);
@@ -742,7 +831,25 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
     if (band < nbz_f)
         fluid_body<DIR, STAGE, TZ, MET>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx, band);
     else
-        solid_body<DIR, STAGE, TZ, MET>(sh, g, c, S, U, T2, T3, O, kif, ct, cprev, bx, band - nbz_f);
+        solid_body<DIR, STAGE, TZ, MET>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx,
+                                        band - nbz_f);
+}
+
+// One RK stage of a solid-only grid with a traction-free top (PmlFs::SolidOnly,
+// src/pml_fs_kernels.cu:29-381): 64 x TZ tiles of rows 3..nzs-4, the same
+// XCD-aware block order.  Stage 4 also records vz of row 3.
+template <int DIR, int STAGE, int TZ, int MET>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
+free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
+                  const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O, int jstep,
+                  float ct, float amp_direct, int nbx, int ntiles) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    const int per_xcd = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int band = tile / nbx, bx = tile - band * nbx;
+    solid_body<DIR, STAGE, TZ, MET, true>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f, bx,
+                                          band);
 }
 
 // Raw-k mode (STAGE 0, msfdm_eval_rhs): the interface condition applied to the
@@ -895,8 +1002,8 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
         fluid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
                                    cprev, tl.z, tl.y);
     else
-        solid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, ct, cprev, tl.z,
-                                   tl.y - tl.w);
+        solid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
+                                      cprev, tl.z, tl.y - tl.w);
 }
 
 // grid (column blocks of the widest shot, shots)

@@ -197,7 +197,9 @@ class HostModel:
     def array(self, name: str) -> np.ndarray:
         d = self.desc()
         which = ARRAYS.index(name)
-        rows = d.nz_fluid if which < 2 else (d.nz_solid if which < 7 else d.nz_fluid + d.nz_solid - 1)
+        solid_only = d.calc_type == 0      # no fluid arrays; the grid is the solid array
+        nz = d.nz_solid if solid_only else d.nz_fluid + d.nz_solid - 1
+        rows = (0 if solid_only else d.nz_fluid) if which < 2 else (d.nz_solid if which < 7 else nz)
         out = np.zeros((rows, d.nx), dtype=np.float32)
         check(lib().msfdm_model_get_array(self.h, which, _fptr(out)))
         return out
@@ -220,7 +222,8 @@ class Solver:
     def __init__(self, model, device: int = 0, _handle=None, _owner=None):
         self._hm = model if isinstance(model, HostModel) else HostModel.from_model(model)
         d = self._hm.desc()
-        self.nx, self.nzf, self.nzs = d.nx, d.nz_fluid, d.nz_solid
+        self.solid_only = d.calc_type == 0   # solid grid with a traction-free top
+        self.nx, self.nzf, self.nzs = d.nx, (0 if self.solid_only else d.nz_fluid), d.nz_solid
         self.nt, self.dt = d.nt, d.dt
         self._owner = _owner      # a Batch owns the handles of its shots
         if _handle is not None:
@@ -274,6 +277,8 @@ class Solver:
         count = n - first if count is None else count
         out = [np.zeros((count, self.nx), dtype=np.float32) for _ in range(3)]
         check(lib().msfdm_get_records(self.h, int(first), int(count), *[_fptr(o) for o in out]))
+        if self.solid_only:
+            return [out[1]]   # vz of the free-surface row 3
         return out   # solid vx, solid vz, fluid p
 
     def get(self, f) -> np.ndarray:

@@ -38,7 +38,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-FANDS, PANDF = 1, 2
+SOLID, FANDS = 0, 1      # grid.calc_type
+PANDF = 2                # grid.boundary_type: PML sides/bottom, free top
 
 _MASK64 = (1 << 64) - 1
 
@@ -126,6 +127,9 @@ class Model:
 
     @property
     def nz(self) -> int:
+        # src/elements.cu:87-99: the solid-only grid is the solid array
+        if self.calc_type == SOLID:
+            return self.nz_solid
         return self.nz_fluid + self.nz_solid - 1
 
 
@@ -218,6 +222,68 @@ def build_model(nx: int, nz: int, h: float, water_depth: float, *, npd: int, fpe
                  beta0=round(beta0, 4), src_x=src_x, src_z=src_z, arrays=arrays)
 
 
+def build_solid_model(nx: int, nz: int, h: float, *, npd: int, fpeak: float, dt: float, nt: int,
+                      src_x, src_z, topography=(), solid=(3200.0, 1732.0, 2000.0),
+                      random_std: float = 0.0, corr_len: float = 50.0, seed: int = 0,
+                      snapshot: int | None = None, ppw0: float = 8.0, rc: float = 1e-3,
+                      npow: int = 2, vp_max: float | None = None) -> Model:
+    """Build a solid-only model with a traction-free top (calc_type 0, PANDF).
+
+    The grid follows ``tools/run_models/model_SOLID_PANDF_wavy_if.py:87-148``:
+    ``x = h xi`` and ``z = h zeta + sum A sin(2 pi x / lambda) w(zeta)`` with the
+    surface envelope ``w = cos^2(pi t / 2)``, ``t = (zeta - 3) / (b - 3)``, equal
+    to 1 on the free-surface row 3 and 0 from the bottom-PML row ``b = nz - 4 - npd``
+    down; ``topography`` lists the ``(A, lambda)`` terms.  Metrics are the closed
+    form derivatives of that map.  Sources are solid-array ``(src_x, src_z)``.
+    """
+    if nz < npd + 12 or nx < 2 * npd + 12:
+        raise ValueError("grid too small for the PML width")
+    zeta = np.arange(nz, dtype=np.float64)
+    b_pml = float(nz - 4 - npd)
+    w = np.ones(nz)
+    dw = np.zeros(nz)
+    if topography:
+        seg = (zeta >= 3.0) & (zeta <= b_pml)
+        t = (zeta[seg] - 3.0) / (b_pml - 3.0)
+        w[seg] = np.cos(0.5 * np.pi * t) ** 2
+        dw[seg] = -0.5 * np.pi / (b_pml - 3.0) * np.sin(np.pi * t)
+        w[zeta > b_pml] = 0.0
+    x = h * np.arange(nx, dtype=np.float64)
+    d = np.zeros(nx)
+    dd = np.zeros(nx)
+    for amp, lam in topography:
+        d += amp * np.sin(2 * np.pi * x / lam)
+        dd += amp * 2 * np.pi * h / lam * np.cos(2 * np.pi * x / lam)
+    dz_dxi = dd[None, :] * w[:, None]
+    dz_dze = h + d[None, :] * dw[:, None]
+    if np.any(dz_dze <= 0.2 * h):
+        raise ValueError("topography folds the grid")
+    J = h * dz_dze
+    vp_s, vs_s, rho_s = solid
+    delta = gaussian_random_field((nz, nx), h, corr_len, random_std, seed)
+    vp, vs, rho = vp_s * (1 + delta), vs_s * (1 + delta), rho_s * (1 + delta)
+    mu = rho * vs ** 2
+    lam_ = rho * (vp ** 2 - 2 * vs ** 2)
+    arrays = {
+        "C11": lam_ + 2 * mu, "C13": lam_, "C33": lam_ + 2 * mu, "C55": mu, "rho_s": rho,
+        "K": np.zeros((0, nx)), "rho_f": np.zeros((0, nx)),
+        "J": J, "xi_x": np.full((nz, nx), 1.0 / h), "xi_z": np.zeros((nz, nx)),
+        "ze_x": -dz_dxi / J, "ze_z": h / J,
+    }
+    arrays = {k: np.ascontiguousarray(v, dtype="<f4") for k, v in arrays.items()}
+    src_x = [int(s) for s in np.atleast_1d(src_x)]
+    src_z = [int(s) for s in np.atleast_1d(src_z)]
+    if len(src_z) != len(src_x):
+        raise ValueError("src_x and src_z differ in length")
+    beta0 = vs_s / (0.5 * ppw0 * h * fpeak)      # model_SOLID_PANDF_ortho.py:76
+    return Model(nx=nx, nz_fluid=0, nz_solid=nz, h=h, fpeak=fpeak, dt=dt, nt=nt,
+                 snapshot=(nt if snapshot is None else snapshot), npd=npd,
+                 thickness=(npd - 1) * h, npow=npow,
+                 vp_max=float(vp_max if vp_max is not None else vp_s), rc=rc,
+                 beta0=round(beta0, 4), src_x=src_x, src_z=src_z, arrays=arrays,
+                 calc_type=SOLID, boundary_type=PANDF)
+
+
 # Named workloads.  "flat401" and "seafloor2001" are the plain-solver workloads
 # (401x201 flat, 2001x1001 body-fitted seafloor); the others are reduced cases
 # used for fast parity checks.
@@ -238,6 +304,20 @@ def preset(name: str, **over) -> Model:
                   src_x=[60, 40], src_height=15.0, seafloor=[(12.0, 200.0), (4.0, 70.0)],
                   solid=(3000.0, 1700.0, 2200.0), random_std=0.05, corr_len=25.0, seed=7,
                   vp_max=3000.0)
+    elif name == "solid_flat301":
+        return build_solid_model(**{**dict(nx=301, nz=161, h=5.0, npd=20, fpeak=15.0, dt=5e-4, nt=1500,
+                                           src_x=[150], src_z=[6], solid=(3200.0, 1732.0, 2000.0)), **over})
+    elif name == "solid_topo2001":
+        return build_solid_model(**{**dict(nx=2001, nz=1001, h=2.5, npd=30, fpeak=20.0, dt=2.5e-4, nt=6000,
+                                           src_x=[1000], src_z=[8], topography=[(20.0, 600.0)],
+                                           solid=(3200.0, 1800.0, 2300.0), random_std=0.05,
+                                           corr_len=50.0, seed=2002), **over})
+    elif name == "solid_tiny":
+        return build_solid_model(**{**dict(nx=110, nz=80, h=5.0, npd=10, fpeak=25.0, dt=4e-4, nt=300,
+                                           src_x=[55, 40], src_z=[5, 20],
+                                           topography=[(10.0, 200.0), (3.0, 70.0)],
+                                           solid=(3000.0, 1700.0, 2200.0), random_std=0.05,
+                                           corr_len=25.0, seed=11), **over})
     elif name == "twoscale4001":
         kw = dict(nx=4001, nz=1501, h=2.5, water_depth=800.0, npd=30, fpeak=25.0, dt=2e-4,
                   nt=8000, src_x=[2000], src_height=10.0,
@@ -264,17 +344,19 @@ def write_model(m: Model, model_dir: str, case: str, relative_to: str | None = N
     def path(*p):
         full = os.path.join(model_dir, *p)
         return os.path.relpath(full, relative_to) if relative_to else os.path.abspath(full)
-    for sub in ("solid", "fluid", "jacob"):
+    fluid = m.calc_type != SOLID   # a solid-only model has no grid.fluid block
+    for sub in ("solid", "fluid", "jacob") if fluid else ("solid", "jacob"):
         os.makedirs(os.path.join(model_dir, sub), exist_ok=True)
     grid = {"calc_type": m.calc_type, "boundary_type": m.boundary_type,
             "src_pos": {"posx": list(m.src_x), "posz": list(m.src_z)},
             "solid": {"nx": m.nx, "nz": m.nz_solid},
-            "jacob": {},
-            "fluid": {"nx": m.nx, "nz": m.nz_fluid}}
+            "jacob": {}}
+    if fluid:
+        grid["fluid"] = {"nx": m.nx, "nz": m.nz_fluid}
     for key, arr in _SOLID_FILES:
         m.arrays[arr].tofile(os.path.join(model_dir, "solid", key + ".bin"))
         grid["solid"][key] = path("solid", key + ".bin")
-    for key, arr in _FLUID_FILES:
+    for key, arr in _FLUID_FILES if fluid else ():
         m.arrays[arr].tofile(os.path.join(model_dir, "fluid", key + ".bin"))
         grid["fluid"][key] = path("fluid", key + ".bin")
     for key in _JACOB_FILES:
@@ -302,13 +384,15 @@ def read_model(json_path: str, cwd: str | None = None) -> Model:
             raise ValueError(f"{p}: expected {rows * nx} floats, found {a.size}")
         return a.reshape(rows, nx)
     g = doc["grid"]
-    nx, nzs, nzf = int(g["solid"]["nx"]), int(g["solid"]["nz"]), int(g["fluid"]["nz"])
-    nz = nzf + nzs - 1
+    calc = int(g["calc_type"])
+    nx, nzs = int(g["solid"]["nx"]), int(g["solid"]["nz"])
+    nzf = int(g["fluid"]["nz"]) if calc != SOLID else 0
+    nz = nzf + nzs - 1 if calc != SOLID else nzs
     arrays = {}
     for key, arr in _SOLID_FILES:
         arrays[arr] = load(g["solid"][key], nzs, nx)
     for key, arr in _FLUID_FILES:
-        arrays[arr] = load(g["fluid"][key], nzf, nx)
+        arrays[arr] = load(g["fluid"][key], nzf, nx) if calc != SOLID else np.zeros((0, nx), dtype="<f4")
     for key in _JACOB_FILES:
         arrays[key] = load(g["jacob"][key], nz, nx)
     b, p = doc["base"], doc["PML"]

@@ -7,7 +7,10 @@
 // it.
 //
 // It restates, in double precision and with race-free (sequential) semantics,
-// the reference's PANDF + FANDS stepper `PmlFs::FluidSolid::update_RK4`:
+// the reference's PANDF steppers: the fluid-solid `PmlFs::FluidSolid::update_RK4`
+// and the solid-only `PmlFs::SolidOnly::update_RK4` with its traction-free top
+// (include/pml_fs_kernels.cuh:8-307, src/pml_fs_kernels.cu:1-381,
+// src/elements.cu:907-977).  Fluid-solid:
 //   include/pml_fs_kernels.cuh:331-827   (RHS kernels, interface BC)
 //   src/pml_fs_kernels.cu:383-868        (ghost fluxes, filter, source, RK4)
 //   include/tools_func.cuh:6-188, src/tools_func.cu:1-66 (operators, fluxes)
@@ -29,7 +32,10 @@
 // extensions; flat-grid conservative == non-conservative forms; PML profile
 // against the Rc reflection integral; Ricker closed-form zeros and peak;
 // interface invariants after each step; normal-incidence reflection
-// coefficients at the seafloor and at the pressure-release surface.
+// coefficients at the seafloor and at the pressure-release surface.  Solid
+// only (tests/test_oracle_solid.py): the flat-grid ALM closed form, zero
+// traction of the free-surface stress rates on curvilinear grids, the odd-image
+// ghost rows, and the Rayleigh-wave speed along the free surface.
 
 #include <algorithm>
 #include <cmath>
@@ -119,10 +125,15 @@ struct Oracle {
     // C13, C33, C55, rho_s)
     std::vector<double> Ain[NFIELD], Aout[NFIELD];
     std::vector<double> G[7];
+    // Solid-only grid (grid.calc_type = 0) with a traction-free top row 3:
+    // per-column ALM coefficients al1_xx, al1_zz, al1_xz, al2_xx, al2_zz, al2_xz.
+    bool solid_only = false;
+    std::vector<double> alm[6];
 
     size_t fsize(int f) const { return (size_t)(is_fluid(f) ? nzf : nzs) * nx; }
     // metric index of solid row iz (src/tools_func.cu:22, 57)
-    size_t gs(int iz, int ix) const { return (size_t)(iz + nzf - 1) * nx + ix; }
+    // (the whole grid is solid when nzf == 0: no offset)
+    size_t gs(int iz, int ix) const { return (size_t)(iz + (nzf > 0 ? nzf - 1 : 0)) * nx + ix; }
     size_t gf(int iz, int ix) const { return (size_t)iz * nx + ix; }
 };
 
@@ -454,6 +465,233 @@ void rhs(Oracle &o, double *const *S, double *const *Kd, int dir, double amp) {
     fl_so_boundary(o, Kd, dir);
 }
 
+// ---------------------------------------------------------------------------
+// Solid-only grid with a traction-free top (grid.calc_type = 0, boundary_type
+// = 2): PmlFs::SolidOnly (include/pml_fs_kernels.cuh:8-307,
+// src/pml_fs_kernels.cu:1-381).  Rows 0-2 frozen, row 3 the free surface,
+// rows 3..nzs-4 and columns 3..nx-4 updated; xi-PML on both sides, zeta-PML
+// at the bottom.
+// ---------------------------------------------------------------------------
+
+// MemManager::build_free_surface, calc_type == SOLID (src/elements.cu:907-977):
+// the traction-free condition at row 3, Dz dv/dzeta + Dx dv/dxi = 0, solved
+// for dv/dzeta = -M dv/dxi (M = Dz^-1 Dx), folded into the strain rates:
+// exx = al1_xx vx_xi + al2_xx vz_xi, ezz = al1_zz vx_xi + al2_zz vz_xi,
+// 2exz = al1_xz vx_xi + al2_xz vz_xi.  Double from the float inputs.
+void build_alm(Oracle &o) {
+    for (auto &a : o.alm) a.assign(o.nx, 0.0);
+    for (int ix = 0; ix < o.nx; ++ix) {
+        const size_t i = (size_t)3 * o.nx + ix, g = o.gs(3, ix);
+        const double xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+        const double c11 = o.C11[i], c13 = o.C13[i], c33 = o.C33[i], c55 = o.C55[i];
+        const double dz11 = c11 * zx * zx + c55 * zz * zz;
+        const double dz12 = (c13 + c55) * zx * zz;
+        const double dz22 = c55 * zx * zx + c33 * zz * zz;
+        const double dx11 = c11 * zx * xx + c55 * zz * xz;
+        const double dx12 = c13 * zx * xz + c55 * zz * xx;
+        const double dx21 = c13 * zz * xx + c55 * zx * xz;
+        const double dx22 = c55 * zx * xx + c33 * zz * xz;
+        const double det = dz11 * dz22 - dz12 * dz12;
+        double m11 = 0, m12 = 0, m21 = 0, m22 = 0;
+        if (std::abs(det) >= 1e-14) {
+            const double i11 = dz22 / det, i12 = -dz12 / det, i22 = dz11 / det;
+            m11 = i11 * dx11 + i12 * dx21;
+            m12 = i11 * dx12 + i12 * dx22;
+            m21 = i12 * dx11 + i22 * dx21;
+            m22 = i12 * dx12 + i22 * dx22;
+        }
+        o.alm[0][ix] = xx - zx * m11;
+        o.alm[1][ix] = -zz * m21;
+        o.alm[2][ix] = xz - zz * m11 - zx * m21;
+        o.alm[3][ix] = -zx * m12;
+        o.alm[4][ix] = xz - zz * m22;
+        o.alm[5][ix] = xx - zz * m12 - zx * m22;
+    }
+}
+
+// build_ghost_flux (src/pml_fs_kernels.cu:7-19): Gfs ghost n = -F^zeta(3 + n)
+// of the solid, slot n-1, columns 3..nx-4.
+void build_ghosts_free(Oracle &o, const double *const *S) {
+    const int nx = o.nx;
+    for (int ix = 3; ix <= nx - 4; ++ix)
+        for (int n = 1; n <= 3; ++n) {
+            double fx, fz;
+            fze_s_row(o, S, 3 + n, ix, fx, fz);
+            o.Gfsx[(n - 1) * nx + ix] = -fx;
+            o.Gfsz[(n - 1) * nx + ix] = -fz;
+        }
+}
+
+// update_stress<DIR> (include/pml_fs_kernels.cuh:10-187).
+void stress_solid_free(Oracle &o, const double *const *S, double *const *Kd, int dir) {
+    const int nx = o.nx, nzs = o.nzs, npd = o.npd;
+#pragma omp parallel for schedule(static)
+    for (int iz = 3; iz <= nzs - 4; ++iz) {
+        for (int ix = 3; ix <= nx - 4; ++ix) {
+            const size_t idx = (size_t)iz * nx + ix, g = o.gs(iz, ix);
+            const double *vx = S[SVX], *vz = S[SVZ];
+            const int dist = iz - 3;
+            const double dvxx = drp(dir, [&](int m) { return vx[idx + m]; });
+            const double dvzx = drp(dir, [&](int m) { return vz[idx + m]; });
+            const double c11 = o.C11[idx], c13 = o.C13[idx], c33 = o.C33[idx], c55 = o.C55[idx];
+            const bool side = in_side(o, ix), bottom = iz >= nzs - 3 - npd;
+            if (dist == 0) {
+                const double a1xx = o.alm[0][ix], a1zz = o.alm[1][ix], a1xz = o.alm[2][ix];
+                const double a2xx = o.alm[3][ix], a2zz = o.alm[4][ix], a2xz = o.alm[5][ix];
+                const double cxx_x = c11 * a1xx + c13 * a1zz, cxx_z = c11 * a2xx + c13 * a2zz;
+                const double czz_x = c13 * a1xx + c33 * a1zz, czz_z = c13 * a2xx + c33 * a2zz;
+                double ksx = cxx_x * dvxx + cxx_z * dvzx;
+                double ksz = czz_x * dvxx + czz_z * dvzx;
+                double ksxz = c55 * (a1xz * dvxx + a2xz * dvzx);
+                if (side) {
+                    const double be = o.be_xi[ix], dpb = o.dp_xi[ix] / be, apdp = o.al_xi[ix] + dpb, invb = 1.0 / be;
+                    const double Ex = S[SXEX][idx], Ez = S[SXEZ][idx];
+                    Kd[SXEX][idx] = -apdp * Ex + dpb * dvxx;
+                    Kd[SXEZ][idx] = -apdp * Ez + dpb * dvzx;
+                    ksx += (invb - 1.0) * (cxx_x * dvxx + cxx_z * dvzx) - invb * (cxx_x * Ex + cxx_z * Ez);
+                    ksz += (invb - 1.0) * (czz_x * dvxx + czz_z * dvzx) - invb * (czz_x * Ex + czz_z * Ez);
+                    ksxz += c55 * ((invb - 1.0) * (a1xz * dvxx + a2xz * dvzx) - invb * (a1xz * Ex + a2xz * Ez));
+                }
+                Kd[SSX][idx] = ksx;
+                Kd[SSZ][idx] = ksz;
+                Kd[SSXZ][idx] = ksxz;
+                continue;
+            }
+            const double dvxz = mac_dze(dir, dist, true, [&](int m) { return vx[idx + (std::ptrdiff_t)m * nx]; });
+            const double dvzz = mac_dze(dir, dist, true, [&](int m) { return vz[idx + (std::ptrdiff_t)m * nx]; });
+            const double xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+            double ksx = c11 * (xx * dvxx + zx * dvxz) + c13 * (xz * dvzx + zz * dvzz);
+            double ksz = c13 * (xx * dvxx + zx * dvxz) + c33 * (xz * dvzx + zz * dvzz);
+            double ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+            if (side || bottom) {
+                const double gxixi = xx * xx + xz * xz, gxize = xx * zx + xz * zz, gzeze = zx * zx + zz * zz;
+                if (side) {
+                    const double Dvx = dvxx + (gxize / gxixi) * dvxz;
+                    const double Dvz = dvzx + (gxize / gxixi) * dvzz;
+                    const double be = o.be_xi[ix], dpb = o.dp_xi[ix] / be, apdp = o.al_xi[ix] + dpb, invb = 1.0 / be;
+                    const double Ex = S[SXEX][idx], Ez = S[SXEZ][idx];
+                    Kd[SXEX][idx] = -apdp * Ex + dpb * Dvx;
+                    Kd[SXEZ][idx] = -apdp * Ez + dpb * Dvz;
+                    ksx += c11 * (invb - 1.0) * xx * Dvx + c13 * (invb - 1.0) * xz * Dvz - invb * c11 * xx * Ex - invb * c13 * xz * Ez;
+                    ksz += c13 * (invb - 1.0) * xx * Dvx + c33 * (invb - 1.0) * xz * Dvz - invb * c13 * xx * Ex - invb * c33 * xz * Ez;
+                    ksxz += c55 * (invb - 1.0) * (xz * Dvx + xx * Dvz) - invb * c55 * xz * Ex - invb * c55 * xx * Ez;
+                }
+                if (bottom) {
+                    const double Dvx = dvxz + (gxize / gzeze) * dvxx;
+                    const double Dvz = dvzz + (gxize / gzeze) * dvzx;
+                    const double be = o.be_zs[iz], dpb = o.dp_zs[iz] / be, apdp = o.al_zs[iz] + dpb, invb = 1.0 / be;
+                    const double Ex = S[SZEX][idx], Ez = S[SZEZ][idx];
+                    Kd[SZEX][idx] = -apdp * Ex + dpb * Dvx;
+                    Kd[SZEZ][idx] = -apdp * Ez + dpb * Dvz;
+                    ksx += c11 * (invb - 1.0) * zx * Dvx + c13 * (invb - 1.0) * zz * Dvz - invb * c11 * zx * Ex - invb * c13 * zz * Ez;
+                    ksz += c13 * (invb - 1.0) * zx * Dvx + c33 * (invb - 1.0) * zz * Dvz - invb * c13 * zx * Ex - invb * c33 * zz * Ez;
+                    ksxz += c55 * (invb - 1.0) * (zz * Dvx + zx * Dvz) - invb * c55 * zz * Ex - invb * c55 * zx * Ez;
+                }
+            }
+            Kd[SSX][idx] = ksx;
+            Kd[SSZ][idx] = ksz;
+            Kd[SSXZ][idx] = ksxz;
+        }
+    }
+}
+
+// update_velocity<DIR> (include/pml_fs_kernels.cuh:189-305).  Rows 3..5 take
+// the conservative flux form with the odd-image ghosts about row 3 and the
+// xi-memory source from distance-dependent zeta derivatives of the stresses
+// (no zeta-memory source there); deeper rows the non-conservative form.
+void velocity_solid_free(Oracle &o, const double *const *S, double *const *Kd, int dir) {
+    const int nx = o.nx, nzs = o.nzs, npd = o.npd;
+#pragma omp parallel for schedule(static)
+    for (int iz = 3; iz <= nzs - 4; ++iz) {
+        for (int ix = 3; ix <= nx - 4; ++ix) {
+            const size_t idx = (size_t)iz * nx + ix, g = o.gs(iz, ix);
+            const double J = o.J[g], xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
+            const double gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
+            const double *sxx = S[SSX], *szz = S[SSZ], *sxz = S[SSXZ];
+            const int dist = iz - 3;
+            double rvx, rvz, Dxx, Dxz, Dzx, Dzz;
+            if (dist < 3) {
+                auto Fxi = [&](int m, int c) {
+                    double fx, fz; fxi_s(o, S, iz, ix + m, fx, fz); return c ? fz : fx;
+                };
+                auto Fze = [&](int m, int c) {
+                    const int r = iz + m;
+                    double fx, fz;
+                    if (r >= 3) fze_s_row(o, S, r, ix, fx, fz);
+                    else { const int n = 3 - r; fx = o.Gfsx[(n - 1) * nx + ix]; fz = o.Gfsz[(n - 1) * nx + ix]; }
+                    return c ? fz : fx;
+                };
+                const double sx = drp(dir, [&](int m) { return Fxi(m, 0); }) + drp(dir, [&](int m) { return Fze(m, 0); });
+                const double sz = drp(dir, [&](int m) { return Fxi(m, 1); }) + drp(dir, [&](int m) { return Fze(m, 1); });
+                rvx = sx / J;
+                rvz = sz / J;
+                auto dx = [&](const double *f) { return drp(dir, [&](int m) { return f[idx + m]; }); };
+                auto dz = [&](const double *f) {
+                    return mac_dze(dir, dist, true, [&](int m) { return f[idx + (std::ptrdiff_t)m * nx]; });
+                };
+                const double dxsx = dx(sxx), dxsz = dx(szz), dxsxz = dx(sxz);
+                const double dzsx = dz(sxx), dzsz = dz(szz), dzsxz = dz(sxz);
+                Dxx = xx * (dxsx + gxize / gxixi * dzsx) + xz * (dxsxz + gxize / gxixi * dzsxz);
+                Dxz = xx * (dxsxz + gxize / gxixi * dzsxz) + xz * (dxsz + gxize / gxixi * dzsz);
+                Dzx = 0.0;
+                Dzz = 0.0;
+            } else {
+                auto dx = [&](const double *f) { return drp(dir, [&](int m) { return f[idx + m]; }); };
+                auto dz = [&](const double *f) { return drp(dir, [&](int m) { return f[idx + (std::ptrdiff_t)m * nx]; }); };
+                const double a = dx(sxx), b = dz(sxx), c = dx(szz), d = dz(szz), e = dx(sxz), f = dz(sxz);
+                rvx = xx * a + zx * b + xz * e + zz * f;
+                rvz = xx * e + zx * f + xz * c + zz * d;
+                Dxx = xx * (a + gxize / gxixi * b) + xz * (e + gxize / gxixi * f);
+                Dxz = xx * (e + gxize / gxixi * f) + xz * (c + gxize / gxixi * d);
+                Dzx = zx * (b + gxize / gzeze * a) + zz * (gxize / gzeze * e + f);
+                Dzz = zx * (gxize / gzeze * e + f) + zz * (gxize / gzeze * c + d);
+            }
+            const double rho = o.rhos[idx];
+            double kvx = rvx / rho, kvz = rvz / rho;
+            if (in_side(o, ix)) {
+                const double be = o.be_xi[ix], dpb = o.dp_xi[ix] / be, apdp = o.al_xi[ix] + dpb, invb = 1.0 / be;
+                Kd[SXVX][idx] = -apdp * S[SXVX][idx] + dpb * Dxx;
+                Kd[SXVZ][idx] = -apdp * S[SXVZ][idx] + dpb * Dxz;
+                kvx += ((invb - 1.0) * Dxx - invb * S[SXVX][idx]) / rho;
+                kvz += ((invb - 1.0) * Dxz - invb * S[SXVZ][idx]) / rho;
+            }
+            if (iz >= nzs - 3 - npd) {
+                const double be = o.be_zs[iz], dpb = o.dp_zs[iz] / be, apdp = o.al_zs[iz] + dpb, invb = 1.0 / be;
+                Kd[SZVX][idx] = -apdp * S[SZVX][idx] + dpb * Dzx;
+                Kd[SZVZ][idx] = -apdp * S[SZVZ][idx] + dpb * Dzz;
+                kvx += ((invb - 1.0) * Dzx - invb * S[SZVX][idx]) / rho;
+                kvz += ((invb - 1.0) * Dzz - invb * S[SZVZ][idx]) / rho;
+            }
+            Kd[SVX][idx] = kvx;
+            Kd[SVZ][idx] = kvz;
+        }
+    }
+}
+
+// add_source (src/pml_fs_kernels.cu:22-27): the 5x5 Gaussian-weighted Ricker
+// on both normal stresses, solid-array indices, skipping rows <= 2 and >= nzs-3.
+void add_source_solid(Oracle &o, double *const *Kd, double amp) {
+    if (o.srcx.empty()) return;
+    const int px = o.srcx[o.isrc], pz = o.srcz[o.isrc];
+    for (int j = -2; j <= 2; ++j)
+        for (int i = -2; i <= 2; ++i) {
+            const int ix = px + i, iz = pz + j;
+            if (ix <= 2 || ix >= o.nx - 3 || iz <= 2 || iz >= o.nzs - 3) continue;
+            const double w = amp * SRC_W[j + 2][i + 2];
+            Kd[SSX][(size_t)iz * o.nx + ix] += w;
+            Kd[SSZ][(size_t)iz * o.nx + ix] += w;
+        }
+}
+
+// linear_operate (src/pml_fs_kernels.cu:29-103): k = L(S) + src.
+void rhs_free(Oracle &o, double *const *S, double *const *Kd, int dir, double amp) {
+    for (int f = 0; f < NFIELD; ++f) std::fill(Kd[f], Kd[f] + o.fsize(f), 0.0);
+    build_ghosts_free(o, S);
+    velocity_solid_free(o, S, Kd, dir);
+    stress_solid_free(o, S, Kd, dir);
+    add_source_solid(o, Kd, amp);
+}
+
 // fl_so_filter (src/pml_fs_kernels.cu:428-525), out of place.
 void filter(Oracle &o, double *const *Ud) {
     const int nx = o.nx, le = 3, ri = nx - 4;
@@ -538,7 +776,8 @@ void step(Oracle &o) {
     const double ct[3] = {dt / 2, dt / 2, dt};
     const double cb[3] = {dt / 6, dt / 3, dt / 3};
     for (int s = 0; s < 4; ++s) {
-        rhs(o, S, Kd, dirs[s], stage_amp(o, o.it, s));
+        if (o.solid_only) rhs_free(o, S, Kd, dirs[s], stage_amp(o, o.it, s));
+        else rhs(o, S, Kd, dirs[s], stage_amp(o, o.it, s));
         for (int f = 0; f < NFIELD; ++f) {
             if (s < 3) {
                 axpy(o.T[f], o.U[f], ct[s], o.K[f]);
@@ -547,6 +786,16 @@ void step(Oracle &o) {
                 axpy(o.U[f], o.Bacc[f], dt / 6, o.K[f]);
             }
         }
+    }
+    if (o.solid_only) {
+        // SOLID + PANDF record: vz of row 3 (src/main.cu:297-322)
+        if (o.it % o.gap == 0) {
+            const size_t r3 = (size_t)3 * o.nx;
+            o.rec_vz.insert(o.rec_vz.end(), o.U[SVZ].begin() + r3, o.U[SVZ].begin() + r3 + o.nx);
+            o.nrec++;
+        }
+        o.it++;
+        return;
     }
     filter(o, Ud);
     if (o.it % o.gap == 0) {
@@ -1037,6 +1286,7 @@ struct orc_desc {
     int nsrc;
     const int *src_x;
     const int *src_z;
+    int calc_type;   // 1 = fluid over solid (FANDS), 0 = solid only (SOLID)
 };
 
 // Model arrays: fluid [nzf][nx] K, rho; solid [nzs][nx] C11, C13, C33, C55,
@@ -1047,8 +1297,9 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
                  const float *J, const float *xix, const float *xiz, const float *zex,
                  const float *zez) {
     Oracle *o = new Oracle();
-    o->nx = d->nx; o->nzf = d->nz_fluid; o->nzs = d->nz_solid; o->npd = d->npd;
-    o->nz = o->nzf + o->nzs - 1;
+    o->solid_only = d->calc_type == 0;
+    o->nx = d->nx; o->nzf = o->solid_only ? 0 : d->nz_fluid; o->nzs = d->nz_solid; o->npd = d->npd;
+    o->nz = o->solid_only ? o->nzs : o->nzf + o->nzs - 1;   // src/elements.cu:87-99
     o->dt_f = d->dt; o->fpeak_f = d->fpeak; o->dt = (double)d->dt;
     o->gap = (int)(1e-3 / (double)d->dt + 0.5);   // src/main.cu:92
     if (o->gap < 1) o->gap = 1;
@@ -1064,9 +1315,10 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
     o->al_zs.resize(o->nzs); o->be_zs.resize(o->nzs); o->dp_zs.resize(o->nzs);
     pml_profile(o->nx, o->npd, d->npow, true, true, damp0, d->beta0, alpha0, o->al_xi.data(), o->be_xi.data(), o->dp_xi.data());
     pml_profile(o->nzs, o->npd, d->npow, false, true, damp0, d->beta0, alpha0, o->al_zs.data(), o->be_zs.data(), o->dp_zs.data());
-    // Interface normal (src/elements.cu:892-905).
-    o->nifx.resize(o->nx); o->nifz.resize(o->nx);
-    for (int ix = 0; ix < o->nx; ++ix) {
+    // Interface normal (src/elements.cu:892-905), or the free-surface ALM.
+    o->nifx.assign(o->nx, 0.0); o->nifz.assign(o->nx, 0.0);
+    if (o->solid_only) build_alm(*o);
+    for (int ix = 0; ix < o->nx && !o->solid_only; ++ix) {
         const size_t g = (size_t)(o->nzf - 1) * o->nx + ix;
         const double s = std::sqrt(o->zex[g] * o->zex[g] + o->zez[g] * o->zez[g]);
         o->nifx[ix] = o->zex[g] / s;
@@ -1130,7 +1382,8 @@ void orc_rhs(void *h, int dir, double amp) {
     Oracle *o = static_cast<Oracle *>(h);
     double *S[NFIELD], *Kd[NFIELD];
     for (int f = 0; f < NFIELD; ++f) { S[f] = o->U[f].data(); Kd[f] = o->K[f].data(); }
-    rhs(*o, S, Kd, dir, amp);
+    if (o->solid_only) rhs_free(*o, S, Kd, dir, amp);
+    else rhs(*o, S, Kd, dir, amp);
 }
 void orc_get_k(void *h, int f, double *out) {
     Oracle *o = static_cast<Oracle *>(h);
@@ -1165,6 +1418,13 @@ void orc_normals(void *h, double *nxo, double *nzo) {
     std::memcpy(nzo, o->nifz.data(), o->nx * sizeof(double));
 }
 double orc_stage_amp(void *h, int it, int stage) { return stage_amp(*static_cast<Oracle *>(h), it, stage); }
+// Free-surface ALM coefficients of a solid-only grid: out[6][nx] (al1_xx,
+// al1_zz, al1_xz, al2_xx, al2_zz, al2_xz); zeros for a fluid-solid grid.
+void orc_alm(void *h, double *out) {
+    Oracle *o = static_cast<Oracle *>(h);
+    for (int k = 0; k < 6; ++k)
+        for (int ix = 0; ix < o->nx; ++ix) out[(size_t)k * o->nx + ix] = o->solid_only ? o->alm[k][ix] : 0.0;
+}
 
 
 // --- adjoint probes ------------------------------------------------------

@@ -30,7 +30,7 @@ class _Desc(C.Structure):
                 ("npow", C.c_int), ("fpeak", C.c_float), ("dt", C.c_float),
                 ("thickness", C.c_float), ("vp_max", C.c_float), ("rc", C.c_float),
                 ("beta0", C.c_float), ("nsrc", C.c_int), ("src_x", C.POINTER(C.c_int)),
-                ("src_z", C.POINTER(C.c_int))]
+                ("src_z", C.POINTER(C.c_int)), ("calc_type", C.c_int)]
 
 
 def lib():
@@ -55,7 +55,7 @@ def lib():
             ("orc_rhs", [vp, C.c_int, C.c_double], None), ("orc_get_k", [vp, C.c_int, dp], None),
             ("orc_get_ghosts", [vp, C.c_int, dp], None), ("orc_filter", [vp], None),
             ("orc_pml", [vp, C.c_int, dp, dp, dp], None), ("orc_normals", [vp, dp, dp], None),
-            ("orc_stage_amp", [vp, C.c_int, C.c_int], C.c_double),
+            ("orc_stage_amp", [vp, C.c_int, C.c_int], C.c_double), ("orc_alm", [vp, dp], None),
             ("orc_op1d", [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int], C.c_double),
             ("orc_set_adj_in", [vp, C.c_int, dp], None), ("orc_get_adj_out", [vp, C.c_int, dp], None),
             ("orc_get_grad", [vp, C.c_int, dp], None), ("orc_zero_grad", [vp], None),
@@ -90,12 +90,15 @@ class Oracle:
         self._keep = (sx, sz)
         d = _Desc(model.nx, model.nz_fluid, model.nz_solid, model.npd, model.npow, model.fpeak,
                   model.dt, model.thickness, model.vp_max, model.rc, model.beta0, len(sx),
-                  sx.ctypes.data_as(C.POINTER(C.c_int)), sz.ctypes.data_as(C.POINTER(C.c_int)))
+                  sx.ctypes.data_as(C.POINTER(C.c_int)), sz.ctypes.data_as(C.POINTER(C.c_int)),
+                  int(getattr(model, "calc_type", 1)))
         a = model.arrays
         arrs = [np.ascontiguousarray(a[k], dtype=np.float32) for k in
                 ("K", "rho_f", "C11", "C13", "C33", "C55", "rho_s", "J", "xi_x", "xi_z", "ze_x", "ze_z")]
         self.h = L.orc_create(C.byref(d), *[_fptr(x) for x in arrs])
-        self.nx, self.nzf, self.nzs = model.nx, model.nz_fluid, model.nz_solid
+        self.solid_only = int(getattr(model, "calc_type", 1)) == 0
+        self.nx, self.nzs = model.nx, model.nz_solid
+        self.nzf = 0 if self.solid_only else model.nz_fluid
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -127,6 +130,8 @@ class Oracle:
         n = lib().orc_num_records(self.h)
         out = [np.zeros((n, self.nx)) for _ in range(3)]
         lib().orc_get_records(self.h, *[_dptr(o) for o in out])
+        if self.solid_only:
+            return [out[1]]   # vz of the free-surface row 3
         return out   # vx_solid, vz_solid, p_fluid
 
     def get(self, f) -> np.ndarray:
@@ -170,6 +175,12 @@ class Oracle:
 
     def stage_amp(self, it: int, stage: int) -> float:
         return lib().orc_stage_amp(self.h, int(it), int(stage))
+
+    def alm(self) -> np.ndarray:
+        """Free-surface ALM coefficients [6][nx] of a solid-only grid."""
+        out = np.zeros((6, self.nx))
+        lib().orc_alm(self.h, _dptr(out))
+        return out
 
 
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
