@@ -2,6 +2,7 @@
 // (src/elements.cu:64-176), with every failure returned as an error code.
 #include "model.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -37,17 +38,26 @@ int record_gap(float dt) {
 }
 
 int validate_desc(const msfdm_desc &d) {
-    if ((d.calc_type != 0 && d.calc_type != 1) || d.boundary_type != 2)
-        return set_error(MSFDM_EINVAL, "only grid.boundary_type = 2 (PML sides/bottom + free surface) with "
-                                       "grid.calc_type = 1 (fluid over solid) or 0 (solid only) is supported, got calc_type=" +
+    // calc_type 1 (fluid over solid) with boundary_type 2 (PML sides/bottom +
+    // free surface); calc_type 0 (solid only) with boundary_type 2, 1 (PML on
+    // four sides) or 0 (Dirichlet)
+    const bool ok = (d.calc_type == 1 && d.boundary_type == 2) ||
+                    (d.calc_type == 0 && d.boundary_type >= 0 && d.boundary_type <= 2);
+    if (!ok)
+        return set_error(MSFDM_EINVAL, "supported: grid.calc_type = 1 (fluid over solid) with grid.boundary_type = 2 "
+                                       "(PML sides/bottom + free surface), and grid.calc_type = 0 (solid only) with "
+                                       "boundary_type 2, 1 (PML on four sides) or 0 (Dirichlet); got calc_type=" +
                                            std::to_string(d.calc_type) + " boundary_type=" + std::to_string(d.boundary_type));
-    if (d.npd < 1) return set_error(MSFDM_EINVAL, "PML.npd must be >= 1");
-    if (d.nx < 2 * d.npd + 8)
+    const bool diri = d.calc_type == 0 && d.boundary_type == 0;   // no PML: npd is not used
+    if (!diri && d.npd < 1) return set_error(MSFDM_EINVAL, "PML.npd must be >= 1");
+    if (d.nx < (diri ? 10 : 2 * d.npd + 8))
         return set_error(MSFDM_EINVAL, "grid nx=" + std::to_string(d.nx) + " too small for npd=" + std::to_string(d.npd));
     if (d.calc_type == 0) {
-        // the free surface (row 3) and its three conservative rows stay above the bottom PML
         if (d.nz_fluid != 0) return set_error(MSFDM_EINVAL, "a solid-only grid (calc_type 0) has nz_fluid = 0");
-        if (d.nz_solid < d.npd + 10)
+        // free top: row 3 and its three conservative rows stay above the bottom PML;
+        // CPML: the top and bottom strips do not meet and the receiver row 30 is updated
+        const int need = d.boundary_type == 2 ? d.npd + 10 : d.boundary_type == 1 ? std::max(2 * d.npd + 8, 34) : 10;
+        if (d.nz_solid < need)
             return set_error(MSFDM_EINVAL, "grid.solid.nz=" + std::to_string(d.nz_solid) + " too small for npd=" + std::to_string(d.npd));
     } else {
         if (d.nz_fluid < 8) return set_error(MSFDM_EINVAL, "grid.fluid.nz must be >= 8 (free surface row 3 + interface)");

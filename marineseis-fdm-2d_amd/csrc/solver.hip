@@ -73,7 +73,8 @@ struct msfdm_solver {
     std::vector<float *> ckpt;       // checkpoints of the full state every adj_every steps
     std::vector<float *> seg;        // per-step U^n, T2, T3, T4 of one segment
     int adj_every = 0, adj_nsteps = 0, adj_isrc = 0;
-    bool solid_only = false;         // grid.calc_type = 0: solid grid with a traction-free top
+    bool solid_only = false;         // grid.calc_type = 0: a solid grid
+    int top = D::TOP_IFACE;          // top kind of the solid stage: interface, free surface or plain
 };
 
 namespace {
@@ -176,12 +177,17 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
         const int nb = (s->d.nz_solid - 6 + TZ - 1) / TZ;
         const int ntiles = nbx * nb;
         const int grid = 8 * ((ntiles + 7) / 8);
-        if (s->coef.vert)
-            D::free_stage_kernel<DIR, STAGE, TZ, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3,
-                                                                                  O, j, ct, amp, nbx, ntiles);
-        else
-            D::free_stage_kernel<DIR, STAGE, TZ, 0><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3,
-                                                                                  O, j, ct, amp, nbx, ntiles);
+#define MSFDM_FREE_LAUNCH(MET, TOP)                                                                          \
+    D::free_stage_kernel<DIR, STAGE, TZ, MET, TOP><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, \
+                                                                                    T3, O, j, ct, amp, nbx, ntiles)
+        if (s->top == D::TOP_FREE) {
+            if (s->coef.vert) MSFDM_FREE_LAUNCH(1, D::TOP_FREE);
+            else MSFDM_FREE_LAUNCH(0, D::TOP_FREE);
+        } else {
+            if (s->coef.vert) MSFDM_FREE_LAUNCH(1, D::TOP_PLAIN);
+            else MSFDM_FREE_LAUNCH(0, D::TOP_PLAIN);
+        }
+#undef MSFDM_FREE_LAUNCH
         return;
     }
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
@@ -254,7 +260,7 @@ int field_rows(const msfdm_solver *s, int f) {
 
 // Copy between the reference layout [rows][nx] and the device layout of field f.
 int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float *host_in) {
-    const int nx = s->d.nx, P = s->geo.P, npd = s->d.npd, W2 = s->geo.W2;
+    const int nx = s->d.nx, P = s->geo.P, npd = s->geo.npd, W2 = s->geo.W2;
     const int rows = field_rows(s, f);
     float *dp = dbuf + s->geo.off[f];
     const bool side = (f >= MSFDM_FXVX && f <= MSFDM_FXEZ) || (f >= MSFDM_SXVX && f <= MSFDM_SXEZ);
@@ -285,13 +291,21 @@ int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float
         }
         return MSFDM_OK;
     }
-    // bottom strip: solid rows nzs-3-npd .. nzs-4
+    // bottom strip: solid rows nzs-3-npd .. nzs-4 (after the top strip, rows 3 .. npd+2,
+    // on the solid-only CPML grid)
     const int zb0 = s->d.nz_solid - 3 - npd;
+    if (host_out) std::fill(host_out, host_out + (size_t)rows * nx, 0.0f);
+    if (npd == 0) return MSFDM_OK;
+    float *bp = dp;
+    if (s->geo.ztop_pml) {
+        if (host_out) HIPCHK(hipMemcpy2D(host_out + 3 * (size_t)nx, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
+        else HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in + 3 * (size_t)nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
+        bp = dp + (size_t)npd * P;
+    }
     if (host_out) {
-        std::fill(host_out, host_out + (size_t)rows * nx, 0.0f);
-        HIPCHK(hipMemcpy2D(host_out + (size_t)zb0 * nx, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy2D(host_out + (size_t)zb0 * nx, nx * sizeof(float), bp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
     } else {
-        HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in + (size_t)zb0 * nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy2D(bp, P * sizeof(float), host_in + (size_t)zb0 * nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
     }
     return MSFDM_OK;
 }
@@ -347,15 +361,20 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     s->gap = record_gap(s->d.dt);
     s->solid_only = s->d.calc_type == 0;
     if (s->solid_only) s->d.nz_fluid = 0;
+    // solid-only boundary kinds: 2 free top + PML (PANDF), 1 PML on four sides
+    // (CPML), 0 Dirichlet: no PML (src/elements.cu:825-827)
+    s->top = !s->solid_only ? D::TOP_IFACE : (s->d.boundary_type == 2 ? D::TOP_FREE : D::TOP_PLAIN);
+    const bool cpml = s->solid_only && s->d.boundary_type == 1, diri = s->solid_only && s->d.boundary_type == 0;
     auto fail = [&](int rc) { free_all(s); delete s; return rc; };
 #define CK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return fail(set_error(MSFDM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); } while (0)
     CK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
 
-    const int nx = s->d.nx, nzf = s->d.nz_fluid, nzs = s->d.nz_solid, npd = s->d.npd;
+    const int nx = s->d.nx, nzf = s->d.nz_fluid, nzs = s->d.nz_solid, npd = diri ? 0 : s->d.npd;
     const int nz = s->solid_only ? nzs : nzf + nzs - 1;   // src/elements.cu:87-99
     const long long P = align64(nx);
     D::Geo &g = s->geo;
     g.nx = nx; g.nzf = nzf; g.nzs = nzs; g.P = (int)P; g.npd = npd; g.W2 = 2 * npd;
+    g.ztop_pml = cpml ? 1 : 0;
     long long off = 0;
     for (int f = 0; f < MSFDM_NFIELD; ++f) {
         g.off[f] = off;
@@ -364,7 +383,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
         else if (f <= MSFDM_FXEZ) n = (long long)nzf * g.W2;
         else if (f <= MSFDM_SSXZ) n = nzs * P;
         else if (f <= MSFDM_SXEZ) n = (long long)nzs * g.W2;
-        else n = (long long)npd * P;
+        else n = (long long)(cpml ? 2 : 1) * npd * P;   // zeta strips: bottom (CPML: top + bottom)
         off += align64(n);
     }
     s->buf_floats = off;
@@ -425,7 +444,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     ptrs[np++] = c.pxa = upload(pa, nX);
     ptrs[np++] = c.pxb = upload(pb, nX);
     ptrs[np++] = c.pxc = upload(pc, nX);
-    pml_profile(nzs, npd, s->d.npow, false, true, damp0, s->d.beta0, alpha0, pa, pb, pc);
+    pml_profile(nzs, npd, s->d.npow, cpml, true, damp0, s->d.beta0, alpha0, pa, pb, pc);
     ptrs[np++] = c.pza = upload(pa, nZ);
     ptrs[np++] = c.pzb = upload(pb, nZ);
     ptrs[np++] = c.pzc = upload(pc, nZ);
@@ -452,7 +471,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // Free-surface ALM coefficients of a solid-only grid (src/elements.cu:907-977):
     // double arithmetic on the float model values of row 3, rounded to float.
     std::vector<float> alm(6 * (size_t)P, 0.0f);
-    for (int ix = 0; ix < nx && s->solid_only; ++ix) {
+    for (int ix = 0; ix < nx && s->top == D::TOP_FREE; ++ix) {
         const size_t i = (size_t)3 * nx + ix;
         const double xx = m->a[MSFDM_A_XI_X][i], xz = m->a[MSFDM_A_XI_Z][i];
         const double zx = m->a[MSFDM_A_ZE_X][i], zz = m->a[MSFDM_A_ZE_Z][i];
@@ -483,6 +502,8 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     s->ctl.gap = s->gap;
     s->ctl.srcx = s->srcx.empty() ? -100 : s->srcx[0];
     s->ctl.srcz = s->srcz.empty() ? -100 : s->srcz[0];
+    // solid-only receiver row (src/main.cu:303-311): 3 (PANDF), 30 (CPML), none (Dirichlet)
+    s->ctl.rec_row = !s->solid_only ? -1 : (s->d.boundary_type == 2 ? 3 : (s->d.boundary_type == 1 ? 30 : -1));
     if (int rc = grow_tables(s, std::max(s->d.nt, 1))) return fail(rc);
 
     // Tile heights: small grids need more blocks to fill 256 CUs.
@@ -498,7 +519,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     const int srows = s->solid_only ? nzs - 6 : nzs - 3;
     const double cf = s->solid_only ? 0.0 : (double)(nzf - 3) * (nx - 6), cs = (double)srows * (nx - 6);
     const double pmlf = s->solid_only ? 0.0 : (double)(nzf - 3) * 2 * npd;
-    const double pmls = (double)srows * 2 * npd + (double)npd * (nx - 6);
+    const double pmls = (double)srows * 2 * npd + (double)(cpml ? 2 : 1) * npd * (nx - 6);
     // Vertical-column grids read neither xi_x nor xi_z: 2 coefficients fewer.
     const double ncf = c.vert ? 4 : 6, ncs = c.vert ? 7 : 9;
     s->bytes_step = 4.0 * (cf * (13 * 3 + 4 * ncf) + cs * (13 * 5 + 4 * ncs) + (pmlf + pmls) * 13 * 4);
@@ -565,6 +586,7 @@ extern "C" int msfdm_run(msfdm_solver *s, int nsteps) {
         }
     }
     s->nrec_done = s->host_it > 0 ? (s->host_it - 1) / s->gap + 1 : 0;
+    if (s->solid_only && s->ctl.rec_row < 0) s->nrec_done = 0;   // Dirichlet grids record nothing
     return MSFDM_OK;
 }
 

@@ -15,12 +15,15 @@
 //   iface_final<DIR>           stage 4: the interface rows of U' (fl_so_boundary of
 //                              all four stages), fl_so_filter (src/pml_fs_kernels.cu:428-525,
 //                              out of place) and the receiver-row record.
-//   free_stage_kernel<DIR,STAGE,TZ,MET>
-//                              the solid-only grid with a traction-free top
-//                              (PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307):
-//                              solid_body<..., FREE = true>, ALM surface row, odd-image
-//                              ghost fluxes, source on both normal stresses, and the
-//                              row-3 vz record in stage 4 (one launch per stage).
+//   free_stage_kernel<DIR,STAGE,TZ,MET,TOP>
+//                              solid-only grids, one launch per stage: TOP_FREE, the
+//                              traction-free top (PmlFs::SolidOnly,
+//                              include/pml_fs_kernels.cuh:8-307: ALM surface row,
+//                              odd-image ghost fluxes); TOP_PLAIN, the CPML and
+//                              Dirichlet grids (include/cpml_kernels.cuh:8-233,
+//                              include/dirichlet_kernels.cuh:7-107: DRP on every row,
+//                              PML on four sides or none).  The source goes on both
+//                              normal stresses; stage 4 records vz of the receiver row.
 //
 // The stage kernels read the stage state once through an LDS tile with a 3-cell
 // halo on every side, build the interface / free-surface ghost fluxes
@@ -60,6 +63,7 @@ constexpr int NTHR = 256;       // 64 x 4 threads per stage block
 
 struct Geo {
     int nx, nzf, nzs, P, npd, W2;
+    int ztop_pml;                 // 1: the solid zeta memory also has a top strip (solid-only CPML grid)
     long long off[MSFDM_NFIELD];  // float offset of each field inside a state buffer
 };
 
@@ -92,6 +96,7 @@ struct Ctl {
     int rec_cap;
     int gap;
     int srcx, srcz;       // fluid-array source position of the current shot
+    int rec_row;          // solid-only grid: recorded row of vz (-1: no records)
 };
 
 // 5-point one-sided derivative from values at ascending offsets
@@ -539,14 +544,17 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 }
 
 // ---------------------------------------------------------------------------
-// Solid stage.  FREE = false: the solid below the fluid, rows 0..nzs-4 (row 0
-// the interface), metrics at global row iz+nzf-1 (src/tools_func.cu:22, 57).
-// FREE = true: a solid-only grid (PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307),
-// rows 3..nzs-4 with the traction-free surface on row 3, metrics at row iz, the
-// source on both normal stresses and the receiver row 3 recorded by stage 4.
-// Columns 3..nx-4 in both.
+// Solid stage.  TOP_IFACE: the solid below the fluid, rows 0..nzs-4 (row 0 the
+// interface), metrics at global row iz+nzf-1 (src/tools_func.cu:22, 57).  The
+// solid-only grids (rows 3..nzs-4, metrics at row iz, the source on both normal
+// stresses, the receiver row recorded by stage 4): TOP_FREE with the
+// traction-free surface on row 3 (PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307),
+// TOP_PLAIN with DRP on every row and zeta PML at the top and bottom (CPML,
+// include/cpml_kernels.cuh:8-233) or no PML (Dirichlet, npd = 0,
+// include/dirichlet_kernels.cuh:7-107).  Columns 3..nx-4 in all.
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ, int MET, bool FREE = false>
+constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2;
+template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
            const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
@@ -554,27 +562,28 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
            int bz) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    const int x0 = bx * TX, z0 = (FREE ? 3 : 0) + bz * TZ;
+    constexpr bool SOLO = TOP != TOP_IFACE;   // a solid-only grid
+    const int x0 = bx * TX, z0 = (SOLO ? 3 : 0) + bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     const int ix = x0 + tx;
     const bool active = ix >= 3 && ix <= nx - 4;
     const int scol = side_col(ix, nx, npd);
-    const long long moff = FREE ? 0 : (long long)(g.nzf - 1) * P;   // solid row -> global metric row
-    constexpr int ztop = FREE ? 3 : 0;                               // first updated row
+    const long long moff = SOLO ? 0 : (long long)(g.nzf - 1) * P;   // solid row -> global metric row
+    constexpr int ztop = SOLO ? 3 : 0;                               // first updated row
     float pxa = 0.f, pxb = 0.f, pxc = 0.f;
     float amp = 0.0f;
     const float *const cp[5] = {c.C11, c.C13, c.C33, c.C55, c.rhos_inv};
     RowPre<5, 5> nxt;
     if (active) {
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
-        if constexpr (FREE)
+        if constexpr (SOLO)
             amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
         const long long a0 = (long long)min(z0 + ty, nzs - 4) * P + ix;
         prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_SVX, a0, moff + a0);
     }
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
-    if constexpr (STAGE >= 2 && !FREE) {
+    if constexpr (STAGE >= 2 && !SOLO) {
         if (z0 == 0) {   // the band holding solid row 0 (tile row HALO)
             rebuild_interface_row<DIR, STAGE, 5, TZ>(sh, g, c, S, kif, HALO, 3, x0, cprev, true);
             __syncthreads();
@@ -605,8 +614,10 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
         const bool side = scol >= 0, bottom = iz >= zb0;
-        const int dist = iz - ztop;       // rows below the interface / free surface
-        const bool surf = FREE && dist == 0;
+        const bool top = TOP == TOP_PLAIN && g.ztop_pml && iz <= npd + 2;
+        // rows below the interface / free surface (the plain top never closes the stencils)
+        const int dist = TOP == TOP_PLAIN ? 3 : iz - ztop;
+        const bool surf = TOP == TOP_FREE && dist == 0;
 
         // ---- stress (include/pml_fs_kernels.cuh:509-628; solid-only: 10-187)
         const float dvxx = dx_lds<DIR>(svx[r], cc);
@@ -655,7 +666,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 ax[q] = Jq * (xxq * ssx[r][cc + m] + xzq * ssxz[r][cc + m]);
                 az[q] = Jq * (xxq * ssxz[r][cc + m] + xzq * ssz[r][cc + m]);
                 const int rr = iz + m;
-                if constexpr (FREE) {
+                if constexpr (TOP == TOP_FREE) {
                     if (rr >= 3) fze_row(rr, bx[q], bz[q]);
                     else {   // Gfs ghost n = 3 - rr: -F(3 + n) (src/pml_fs_kernels.cu:7-19)
                         float fxn, fzn;
@@ -679,7 +690,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             const float Jc = c.J[gi];
             rvx = sx / Jc;
             rvz = sz / Jc;
-            if constexpr (FREE) {
+            if constexpr (TOP == TOP_FREE) {
                 // xi-memory source from distance-dependent zeta derivatives, no
                 // zeta-memory source (include/pml_fs_kernels.cuh:212-233)
                 const float dxsx = dx_lds<DIR>(ssx[r], cc), dxsz = dx_lds<DIR>(ssz[r], cc);
@@ -744,9 +755,10 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SXEX] + ai, Ex, kEx, ct);
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_SXEZ] + ai, Ez, kEz, ct);
         }
-        // ---- zeta-PML memory (bottom strip)
-        if (bottom) {
-            const long long ai = (long long)(iz - zb0) * P + ix;
+        // ---- zeta-PML memory (bottom strip; CPML solid-only grid: top strip
+        // first, rows 3..npd+2, then the bottom strip)
+        if (bottom || top) {
+            const long long ai = (long long)(top ? iz - 3 : (g.ztop_pml ? npd : 0) + iz - zb0) * P + ix;
             const float pza = c.pza[iz], pzb = c.pzb[iz], pzc = c.pzc[iz];
             const float Vx = S[g.off[MSFDM_SZVX] + ai], Vz = S[g.off[MSFDM_SZVZ] + ai];
             const float Ex = S[g.off[MSFDM_SZEX] + ai], Ez = S[g.off[MSFDM_SZEZ] + ai];
@@ -769,7 +781,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         }
         // ---- source: 5x5 Gaussian-weighted Ricker on both normal stresses
         // (solid only, src/pml_fs_kernels.cu:22-27, include/tools_func.cuh:21-37)
-        if constexpr (FREE) {
+        if constexpr (SOLO) {
             const int sxo = ix - ctl.srcx, szo = iz - ctl.srcz;
             if (sxo >= -2 && sxo <= 2 && szo >= -2 && szo <= 2) {
                 const float w = amp * SRC_W[szo + 2][sxo + 2];
@@ -777,7 +789,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 ksz += w;
             }
         }
-        if (!FREE && iz == 0) {
+        if (!SOLO && iz == 0) {
             float *kk = kif + (long long)kif_set(STAGE) * P;
             kk[3 * (long long)P + ix] = kvx;
             kk[4 * (long long)P + ix] = kvz;
@@ -795,10 +807,11 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         }
         rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cur.u[0]);
         const float vz_new = rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cur.u[1]);
-        if constexpr (FREE && STAGE == 4) {
-            // receiver row 3, vz after the step (src/main.cu:297-322), plane 1 of the record buffer
+        if constexpr (SOLO && STAGE == 4) {
+            // receiver row (3 with the free surface, 30 with CPML), vz after the
+            // step (src/main.cu:297-322), plane 1 of the record buffer
             const int it = ctl.base[0] + jstep;
-            if (surf && it % ctl.gap == 0) {
+            if (iz == ctl.rec_row && it % ctl.gap == 0) {
                 const int slot = it / ctl.gap;
                 if (slot < ctl.rec_cap) ctl.rec[(long long)ctl.rec_cap * nx + (long long)slot * nx + ix] = vz_new;
             }
@@ -835,10 +848,11 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
                                         band - nbz_f);
 }
 
-// One RK stage of a solid-only grid with a traction-free top (PmlFs::SolidOnly,
-// src/pml_fs_kernels.cu:29-381): 64 x TZ tiles of rows 3..nzs-4, the same
-// XCD-aware block order.  Stage 4 also records vz of row 3.
-template <int DIR, int STAGE, int TZ, int MET>
+// One RK stage of a solid-only grid (PmlFs::SolidOnly, src/pml_fs_kernels.cu:29-381;
+// Cpml::SolidOnly; Dirichlet, src/dirichlet_kernels.cu:12-163): 64 x TZ tiles
+// of rows 3..nzs-4, the same XCD-aware block order.  Stage 4 also records vz
+// of the receiver row.
+template <int DIR, int STAGE, int TZ, int MET, int TOP>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
                   const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O, int jstep,
@@ -848,8 +862,8 @@ free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const flo
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
-    solid_body<DIR, STAGE, TZ, MET, true>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f, bx,
-                                          band);
+    solid_body<DIR, STAGE, TZ, MET, TOP>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f, bx,
+                                         band);
 }
 
 // Raw-k mode (STAGE 0, msfdm_eval_rhs): the interface condition applied to the

@@ -39,7 +39,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 SOLID, FANDS = 0, 1      # grid.calc_type
-PANDF = 2                # grid.boundary_type: PML sides/bottom, free top
+DIRI, CPML, PANDF = 0, 1, 2   # grid.boundary_type: Dirichlet, PML on 4 sides, PML sides/bottom + free top
 
 _MASK64 = (1 << 64) - 1
 
@@ -226,8 +226,10 @@ def build_solid_model(nx: int, nz: int, h: float, *, npd: int, fpeak: float, dt:
                       src_x, src_z, topography=(), solid=(3200.0, 1732.0, 2000.0),
                       random_std: float = 0.0, corr_len: float = 50.0, seed: int = 0,
                       snapshot: int | None = None, ppw0: float = 8.0, rc: float = 1e-3,
-                      npow: int = 2, vp_max: float | None = None) -> Model:
-    """Build a solid-only model with a traction-free top (calc_type 0, PANDF).
+                      npow: int = 2, vp_max: float | None = None, boundary_type: int = PANDF) -> Model:
+    """Build a solid-only model (calc_type 0): by default with a traction-free
+    top (PANDF); ``boundary_type`` CPML puts the PML on all four sides and DIRI
+    clamps the grid edges (``model_SOLID_PANDF_ortho.py:29-33``).
 
     The grid follows ``tools/run_models/model_SOLID_PANDF_wavy_if.py:87-148``:
     ``x = h xi`` and ``z = h zeta + sum A sin(2 pi x / lambda) w(zeta)`` with the
@@ -238,6 +240,10 @@ def build_solid_model(nx: int, nz: int, h: float, *, npd: int, fpeak: float, dt:
     """
     if nz < npd + 12 or nx < 2 * npd + 12:
         raise ValueError("grid too small for the PML width")
+    if boundary_type not in (DIRI, CPML, PANDF):
+        raise ValueError(f"boundary_type must be 0, 1 or 2, not {boundary_type}")
+    if topography and boundary_type != PANDF:
+        raise ValueError("surface topography needs the free surface (boundary_type 2)")
     zeta = np.arange(nz, dtype=np.float64)
     b_pml = float(nz - 4 - npd)
     w = np.ones(nz)
@@ -281,7 +287,7 @@ def build_solid_model(nx: int, nz: int, h: float, *, npd: int, fpeak: float, dt:
                  thickness=(npd - 1) * h, npow=npow,
                  vp_max=float(vp_max if vp_max is not None else vp_s), rc=rc,
                  beta0=round(beta0, 4), src_x=src_x, src_z=src_z, arrays=arrays,
-                 calc_type=SOLID, boundary_type=PANDF)
+                 calc_type=SOLID, boundary_type=boundary_type)
 
 
 # Named workloads.  "flat401" and "seafloor2001" are the plain-solver workloads
@@ -318,6 +324,21 @@ def preset(name: str, **over) -> Model:
                                            topography=[(10.0, 200.0), (3.0, 70.0)],
                                            solid=(3000.0, 1700.0, 2200.0), random_std=0.05,
                                            corr_len=25.0, seed=11), **over})
+    elif name == "solid_cpml_tiny":
+        return build_solid_model(**{**dict(nx=96, nz=84, h=5.0, npd=10, fpeak=25.0, dt=4e-4, nt=300,
+                                           src_x=[48, 30], src_z=[40, 20], boundary_type=CPML,
+                                           solid=(3000.0, 1700.0, 2200.0), random_std=0.05,
+                                           corr_len=25.0, seed=12), **over})
+    elif name == "solid_diri_tiny":
+        return build_solid_model(**{**dict(nx=90, nz=70, h=5.0, npd=10, fpeak=25.0, dt=4e-4, nt=300,
+                                           src_x=[45], src_z=[30], boundary_type=DIRI,
+                                           solid=(3000.0, 1700.0, 2200.0), random_std=0.05,
+                                           corr_len=25.0, seed=13), **over})
+    elif name == "solid_cpml501":
+        # tools/run_models/model_SOLID_PANDF_ortho.py:35-55 (homogeneous, CPML, 2.5 m)
+        return build_solid_model(**{**dict(nx=501, nz=501, h=2.5, npd=15, fpeak=30.0, dt=1e-4, nt=25000,
+                                           snapshot=200, src_x=[250], src_z=[250], boundary_type=CPML,
+                                           solid=(3200.0, 1732.0, 2000.0)), **over})
     elif name == "twoscale4001":
         kw = dict(nx=4001, nz=1501, h=2.5, water_depth=800.0, npd=30, fpeak=25.0, dt=2e-4,
                   nt=8000, src_x=[2000], src_height=10.0,

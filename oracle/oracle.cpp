@@ -129,6 +129,14 @@ struct Oracle {
     // per-column ALM coefficients al1_xx, al1_zz, al1_xz, al2_xx, al2_zz, al2_xz.
     bool solid_only = false;
     std::vector<double> alm[6];
+    // grid.boundary_type of a solid-only grid: 2 = PML sides/bottom + free
+    // surface (PANDF), 1 = PML on all four sides (CPML), 0 = Dirichlet (DIRI).
+    int bdy = 2;
+    bool zpml(int iz) const {   // zeta-PML rows (bottom; CPML also top, src/elements.cu:866-869)
+        if (solid_only && bdy == 0) return false;
+        const bool bot = iz >= nzs - 3 - npd;
+        return (solid_only && bdy == 1) ? (bot || iz <= npd + 2) : bot;
+    }
 
     size_t fsize(int f) const { return (size_t)(is_fluid(f) ? nzf : nzs) * nx; }
     // metric index of solid row iz (src/tools_func.cu:22, 57)
@@ -203,6 +211,7 @@ void build_ghosts(Oracle &o, const double *const *S) {
 }
 
 inline bool in_side(const Oracle &o, int ix) {
+    if (o.solid_only && o.bdy == 0) return false;   // Dirichlet: no PML (src/elements.cu:825-827)
     const int distx = std::min(ix - 3, o.nx - 4 - ix);
     return distx >= 0 && distx < o.npd;
 }
@@ -522,19 +531,23 @@ void build_ghosts_free(Oracle &o, const double *const *S) {
         }
 }
 
-// update_stress<DIR> (include/pml_fs_kernels.cuh:10-187).
+// update_stress<DIR> (include/pml_fs_kernels.cuh:10-187).  CPML
+// (include/cpml_kernels.cuh:10-128) and Dirichlet (include/dirichlet_kernels.cuh:10-47)
+// grids take the non-surface branch on every row (DRP in zeta throughout), with
+// PML on all four sides (CPML) or none (Dirichlet).
 void stress_solid_free(Oracle &o, const double *const *S, double *const *Kd, int dir) {
-    const int nx = o.nx, nzs = o.nzs, npd = o.npd;
+    const int nx = o.nx, nzs = o.nzs;
+    const bool free_top = o.bdy == 2;
 #pragma omp parallel for schedule(static)
     for (int iz = 3; iz <= nzs - 4; ++iz) {
         for (int ix = 3; ix <= nx - 4; ++ix) {
             const size_t idx = (size_t)iz * nx + ix, g = o.gs(iz, ix);
             const double *vx = S[SVX], *vz = S[SVZ];
-            const int dist = iz - 3;
+            const int dist = free_top ? iz - 3 : 3;
             const double dvxx = drp(dir, [&](int m) { return vx[idx + m]; });
             const double dvzx = drp(dir, [&](int m) { return vz[idx + m]; });
             const double c11 = o.C11[idx], c13 = o.C13[idx], c33 = o.C33[idx], c55 = o.C55[idx];
-            const bool side = in_side(o, ix), bottom = iz >= nzs - 3 - npd;
+            const bool side = in_side(o, ix), bottom = o.zpml(iz);
             if (dist == 0) {
                 const double a1xx = o.alm[0][ix], a1zz = o.alm[1][ix], a1xz = o.alm[2][ix];
                 const double a2xx = o.alm[3][ix], a2zz = o.alm[4][ix], a2xz = o.alm[5][ix];
@@ -598,9 +611,12 @@ void stress_solid_free(Oracle &o, const double *const *S, double *const *Kd, int
 // update_velocity<DIR> (include/pml_fs_kernels.cuh:189-305).  Rows 3..5 take
 // the conservative flux form with the odd-image ghosts about row 3 and the
 // xi-memory source from distance-dependent zeta derivatives of the stresses
-// (no zeta-memory source there); deeper rows the non-conservative form.
+// (no zeta-memory source there); deeper rows the non-conservative form.  CPML
+// and Dirichlet grids (include/cpml_kernels.cuh:130-217,
+// include/dirichlet_kernels.cuh:49-83) take the non-conservative form on every row.
 void velocity_solid_free(Oracle &o, const double *const *S, double *const *Kd, int dir) {
-    const int nx = o.nx, nzs = o.nzs, npd = o.npd;
+    const int nx = o.nx, nzs = o.nzs;
+    const bool free_top = o.bdy == 2;
 #pragma omp parallel for schedule(static)
     for (int iz = 3; iz <= nzs - 4; ++iz) {
         for (int ix = 3; ix <= nx - 4; ++ix) {
@@ -608,7 +624,7 @@ void velocity_solid_free(Oracle &o, const double *const *S, double *const *Kd, i
             const double J = o.J[g], xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
             const double gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
             const double *sxx = S[SSX], *szz = S[SSZ], *sxz = S[SSXZ];
-            const int dist = iz - 3;
+            const int dist = free_top ? iz - 3 : 3;
             double rvx, rvz, Dxx, Dxz, Dzx, Dzz;
             if (dist < 3) {
                 auto Fxi = [&](int m, int c) {
@@ -655,7 +671,7 @@ void velocity_solid_free(Oracle &o, const double *const *S, double *const *Kd, i
                 kvx += ((invb - 1.0) * Dxx - invb * S[SXVX][idx]) / rho;
                 kvz += ((invb - 1.0) * Dxz - invb * S[SXVZ][idx]) / rho;
             }
-            if (iz >= nzs - 3 - npd) {
+            if (o.zpml(iz)) {
                 const double be = o.be_zs[iz], dpb = o.dp_zs[iz] / be, apdp = o.al_zs[iz] + dpb, invb = 1.0 / be;
                 Kd[SZVX][idx] = -apdp * S[SZVX][idx] + dpb * Dzx;
                 Kd[SZVZ][idx] = -apdp * S[SZVZ][idx] + dpb * Dzz;
@@ -683,10 +699,11 @@ void add_source_solid(Oracle &o, double *const *Kd, double amp) {
         }
 }
 
-// linear_operate (src/pml_fs_kernels.cu:29-103): k = L(S) + src.
+// linear_operate (src/pml_fs_kernels.cu:29-103, src/cpml_kernels.cu,
+// src/dirichlet_kernels.cu:12-56): k = L(S) + src.
 void rhs_free(Oracle &o, double *const *S, double *const *Kd, int dir, double amp) {
     for (int f = 0; f < NFIELD; ++f) std::fill(Kd[f], Kd[f] + o.fsize(f), 0.0);
-    build_ghosts_free(o, S);
+    if (o.bdy == 2) build_ghosts_free(o, S);
     velocity_solid_free(o, S, Kd, dir);
     stress_solid_free(o, S, Kd, dir);
     add_source_solid(o, Kd, amp);
@@ -788,10 +805,11 @@ void step(Oracle &o) {
         }
     }
     if (o.solid_only) {
-        // SOLID + PANDF record: vz of row 3 (src/main.cu:297-322)
-        if (o.it % o.gap == 0) {
-            const size_t r3 = (size_t)3 * o.nx;
-            o.rec_vz.insert(o.rec_vz.end(), o.U[SVZ].begin() + r3, o.U[SVZ].begin() + r3 + o.nx);
+        // SOLID records: vz of row 3 (PANDF) or row 30 (CPML); none for
+        // Dirichlet (src/main.cu:199-209, 297-322)
+        if (o.bdy != 0 && o.it % o.gap == 0) {
+            const size_t r = (size_t)(o.bdy == 1 ? 30 : 3) * o.nx;
+            o.rec_vz.insert(o.rec_vz.end(), o.U[SVZ].begin() + r, o.U[SVZ].begin() + r + o.nx);
             o.nrec++;
         }
         o.it++;
@@ -1286,7 +1304,8 @@ struct orc_desc {
     int nsrc;
     const int *src_x;
     const int *src_z;
-    int calc_type;   // 1 = fluid over solid (FANDS), 0 = solid only (SOLID)
+    int calc_type;       // 1 = fluid over solid (FANDS), 0 = solid only (SOLID)
+    int boundary_type;   // 2 = PANDF; solid only also 1 = CPML, 0 = Dirichlet
 };
 
 // Model arrays: fluid [nzf][nx] K, rho; solid [nzs][nx] C11, C13, C33, C55,
@@ -1298,6 +1317,7 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
                  const float *zez) {
     Oracle *o = new Oracle();
     o->solid_only = d->calc_type == 0;
+    o->bdy = o->solid_only ? d->boundary_type : 2;
     o->nx = d->nx; o->nzf = o->solid_only ? 0 : d->nz_fluid; o->nzs = d->nz_solid; o->npd = d->npd;
     o->nz = o->solid_only ? o->nzs : o->nzf + o->nzs - 1;   // src/elements.cu:87-99
     o->dt_f = d->dt; o->fpeak_f = d->fpeak; o->dt = (double)d->dt;
@@ -1314,10 +1334,12 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
     o->al_xi.resize(o->nx); o->be_xi.resize(o->nx); o->dp_xi.resize(o->nx);
     o->al_zs.resize(o->nzs); o->be_zs.resize(o->nzs); o->dp_zs.resize(o->nzs);
     pml_profile(o->nx, o->npd, d->npow, true, true, damp0, d->beta0, alpha0, o->al_xi.data(), o->be_xi.data(), o->dp_xi.data());
-    pml_profile(o->nzs, o->npd, d->npow, false, true, damp0, d->beta0, alpha0, o->al_zs.data(), o->be_zs.data(), o->dp_zs.data());
+    // zeta: bottom PML; the solid-only CPML grid also has a top PML (src/elements.cu:863-869)
+    pml_profile(o->nzs, o->npd, d->npow, o->solid_only && o->bdy == 1, true, damp0, d->beta0, alpha0, o->al_zs.data(),
+                o->be_zs.data(), o->dp_zs.data());
     // Interface normal (src/elements.cu:892-905), or the free-surface ALM.
     o->nifx.assign(o->nx, 0.0); o->nifz.assign(o->nx, 0.0);
-    if (o->solid_only) build_alm(*o);
+    if (o->solid_only && o->bdy == 2) build_alm(*o);
     for (int ix = 0; ix < o->nx && !o->solid_only; ++ix) {
         const size_t g = (size_t)(o->nzf - 1) * o->nx + ix;
         const double s = std::sqrt(o->zex[g] * o->zex[g] + o->zez[g] * o->zez[g]);
@@ -1423,7 +1445,7 @@ double orc_stage_amp(void *h, int it, int stage) { return stage_amp(*static_cast
 void orc_alm(void *h, double *out) {
     Oracle *o = static_cast<Oracle *>(h);
     for (int k = 0; k < 6; ++k)
-        for (int ix = 0; ix < o->nx; ++ix) out[(size_t)k * o->nx + ix] = o->solid_only ? o->alm[k][ix] : 0.0;
+        for (int ix = 0; ix < o->nx; ++ix) out[(size_t)k * o->nx + ix] = o->alm[k].empty() ? 0.0 : o->alm[k][ix];
 }
 
 

@@ -30,7 +30,7 @@ class _Desc(C.Structure):
                 ("npow", C.c_int), ("fpeak", C.c_float), ("dt", C.c_float),
                 ("thickness", C.c_float), ("vp_max", C.c_float), ("rc", C.c_float),
                 ("beta0", C.c_float), ("nsrc", C.c_int), ("src_x", C.POINTER(C.c_int)),
-                ("src_z", C.POINTER(C.c_int)), ("calc_type", C.c_int)]
+                ("src_z", C.POINTER(C.c_int)), ("calc_type", C.c_int), ("boundary_type", C.c_int)]
 
 
 def lib():
@@ -91,7 +91,7 @@ class Oracle:
         d = _Desc(model.nx, model.nz_fluid, model.nz_solid, model.npd, model.npow, model.fpeak,
                   model.dt, model.thickness, model.vp_max, model.rc, model.beta0, len(sx),
                   sx.ctypes.data_as(C.POINTER(C.c_int)), sz.ctypes.data_as(C.POINTER(C.c_int)),
-                  int(getattr(model, "calc_type", 1)))
+                  int(getattr(model, "calc_type", 1)), int(getattr(model, "boundary_type", 2)))
         a = model.arrays
         arrs = [np.ascontiguousarray(a[k], dtype=np.float32) for k in
                 ("K", "rho_f", "C11", "C13", "C33", "C55", "rho_s", "J", "xi_x", "xi_z", "ze_x", "ze_z")]

@@ -77,7 +77,7 @@ def test_solid_error_paths(solid_dir):
     orig = json.load(open(jpath))
     expect(lambda d: d["grid"]["src_pos"]["posz"].__setitem__(0, m.nz_solid), EINVAL, "outside the solid grid")
     json.dump(orig, open(jpath, "w"))
-    expect(lambda d: d["grid"].__setitem__("boundary_type", 1), EINVAL, "boundary_type")
+    expect(lambda d: d["grid"].__setitem__("boundary_type", 3), EINVAL, "boundary_type")
     json.dump(orig, open(jpath, "w"))
     expect(lambda d: d["PML"].__setitem__("npd", m.nz_solid - 9), EINVAL, "too small")
     json.dump(orig, open(jpath, "w"))
@@ -89,3 +89,31 @@ def test_solid_batch_is_rejected_before_any_device_work():
     with pytest.raises(msfdm.MsfdmError) as e:
         msfdm.Batch([m])
     assert e.value.code == EINVAL and "solid-only" in str(e.value)
+
+
+def test_solid_boundary_kinds_and_their_limits():
+    """Solid-only grids accept boundary_type 2 (free top), 1 (PML on four sides:
+    the strips must not meet and the receiver row 30 must be updated) and 0
+    (Dirichlet: no PML, npd unused); a fluid-solid grid only boundary_type 2."""
+    for bt, nz_ok, nz_bad in ((1, 34, 33), (0, 10, 9), (2, 20, 19)):
+        m = modelgen.build_solid_model(nx=60, nz=60, h=5.0, npd=10, fpeak=20.0, dt=4e-4, nt=10, src_x=[30],
+                                       src_z=[8], boundary_type=bt)
+        for nz, ok in ((nz_ok, True), (nz_bad, False)):
+            import dataclasses
+            a = {k: (v[:nz] if v.shape[0] == 60 else v) for k, v in m.arrays.items()}
+            mm = dataclasses.replace(m, nz_solid=nz, arrays=a)
+            if ok:
+                assert msfdm.HostModel.from_model(mm).desc().boundary_type == bt
+            else:
+                with pytest.raises(msfdm.MsfdmError) as e:
+                    msfdm.HostModel.from_model(mm)
+                assert e.value.code == EINVAL and "too small" in str(e.value)
+    fs = modelgen.preset("tiny_curvi")
+    for bt in (0, 1):
+        import dataclasses
+        with pytest.raises(msfdm.MsfdmError) as e:
+            msfdm.HostModel.from_model(dataclasses.replace(fs, boundary_type=bt))
+        assert e.value.code == EINVAL and "supported" in str(e.value)
+    with pytest.raises(ValueError):
+        modelgen.build_solid_model(nx=60, nz=60, h=5.0, npd=10, fpeak=20.0, dt=4e-4, nt=10, src_x=[30],
+                                   src_z=[8], boundary_type=1, topography=[(5.0, 100.0)])
