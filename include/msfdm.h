@@ -31,14 +31,17 @@ enum {
 };
 
 /* Field identifiers for msfdm_get_field / msfdm_set_field.  U_fluid, U_solid and
- * the ADE memory sets of include/elements.h:6-26 (fluid zeta memory is never
- * driven by the reference and is not stored). */
+ * the ADE memory sets of include/elements.h:6-26.  The fluid zeta memory
+ * (MSFDM_FZ*) is driven on the fluid-solid CPML grid only (its top strip,
+ * include/cpml_kernels.cuh:331-342, 399-413); with the pressure-release
+ * surface the reference never drives it and it is not stored. */
 enum {
     MSFDM_FVX = 0, MSFDM_FVZ, MSFDM_FP,
     MSFDM_FXVX, MSFDM_FXVZ, MSFDM_FXEX, MSFDM_FXEZ,
     MSFDM_SVX, MSFDM_SVZ, MSFDM_SSX, MSFDM_SSZ, MSFDM_SSXZ,
     MSFDM_SXVX, MSFDM_SXVZ, MSFDM_SXEX, MSFDM_SXEZ,
     MSFDM_SZVX, MSFDM_SZVZ, MSFDM_SZEX, MSFDM_SZEZ,
+    MSFDM_FZVX, MSFDM_FZVZ, MSFDM_FZEX, MSFDM_FZEZ,
     MSFDM_NFIELD
 };
 
@@ -50,12 +53,15 @@ enum {
 };
 
 /* Scalar part of the model: the JSON keys read by MemManager::MemManager
- * (src/elements.cu:72-111).  boundary_type = 2 (PANDF: CFS-PML sides and
- * bottom, free top) with calc_type = 1 (FANDS, fluid over solid;
- * PmlFs::FluidSolid) or calc_type = 0 (SOLID, a solid grid with a
- * traction-free top row 3; PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307)
- * is accepted.  A solid-only model has nz_fluid = 0, no fluid arrays (K and
- * rho_f may be NULL) and metrics of nz_solid rows. */
+ * (src/elements.cu:72-111).  Accepted: calc_type = 1 (FANDS, fluid over
+ * solid) with boundary_type = 2 (PANDF: CFS-PML sides and bottom,
+ * pressure-release sea surface; PmlFs::FluidSolid) or 1 (CPML: a top PML in
+ * the water instead; Cpml::FluidSolid, include/cpml_kernels.cuh:237-770), and
+ * calc_type = 0 (SOLID) with boundary_type = 2 (traction-free top row 3;
+ * PmlFs::SolidOnly, include/pml_fs_kernels.cuh:8-307), 1 (PML on four sides;
+ * Cpml::SolidOnly) or 0 (Dirichlet edges; include/dirichlet_kernels.cuh).
+ * A solid-only model has nz_fluid = 0, no fluid arrays (K and rho_f may be
+ * NULL) and metrics of nz_solid rows. */
 typedef struct msfdm_desc {
     int nx, nz_fluid, nz_solid;
     int calc_type, boundary_type;
@@ -190,8 +196,8 @@ int msfdm_grid_metrics(const double *x, const double *z, int nz, int nx, int dev
 
 /* --- Discrete adjoint (no reference counterpart) ------------------------- */
 /* The adjoint, multi-shot and batched entry points, and msfdm_update_media,
- * serve fluid-solid models (calc_type 1); with a solid-only model they return
- * MSFDM_EINVAL. */
+ * serve fluid-solid models with the pressure-release surface (calc_type 1,
+ * boundary_type 2); with any other model they return MSFDM_EINVAL. */
 /* Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
  * arrays for solid vx, solid vz, fluid p as returned by msfdm_get_records) with
  * respect to the model arrays K, rho_f ([nz_f][nx]) and C11, C13, C33, C55,

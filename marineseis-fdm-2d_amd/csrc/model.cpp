@@ -41,12 +41,12 @@ int validate_desc(const msfdm_desc &d) {
     // calc_type 1 (fluid over solid) with boundary_type 2 (PML sides/bottom +
     // free surface); calc_type 0 (solid only) with boundary_type 2, 1 (PML on
     // four sides) or 0 (Dirichlet)
-    const bool ok = (d.calc_type == 1 && d.boundary_type == 2) ||
+    const bool ok = (d.calc_type == 1 && (d.boundary_type == 2 || d.boundary_type == 1)) ||
                     (d.calc_type == 0 && d.boundary_type >= 0 && d.boundary_type <= 2);
     if (!ok)
         return set_error(MSFDM_EINVAL, "supported: grid.calc_type = 1 (fluid over solid) with grid.boundary_type = 2 "
-                                       "(PML sides/bottom + free surface), and grid.calc_type = 0 (solid only) with "
-                                       "boundary_type 2, 1 (PML on four sides) or 0 (Dirichlet); got calc_type=" +
+                                       "(PML sides/bottom + free surface) or 1 (PML on four sides), and "
+                                       "grid.calc_type = 0 (solid only) with boundary_type 2, 1 or 0 (Dirichlet); got calc_type=" +
                                            std::to_string(d.calc_type) + " boundary_type=" + std::to_string(d.boundary_type));
     const bool diri = d.calc_type == 0 && d.boundary_type == 0;   // no PML: npd is not used
     if (!diri && d.npd < 1) return set_error(MSFDM_EINVAL, "PML.npd must be >= 1");
@@ -61,6 +61,9 @@ int validate_desc(const msfdm_desc &d) {
             return set_error(MSFDM_EINVAL, "grid.solid.nz=" + std::to_string(d.nz_solid) + " too small for npd=" + std::to_string(d.npd));
     } else {
         if (d.nz_fluid < 8) return set_error(MSFDM_EINVAL, "grid.fluid.nz must be >= 8 (free surface row 3 + interface)");
+        if (d.boundary_type == 1 && d.nz_fluid < d.npd + 6)   // the top strip stays clear of the interface rows
+            return set_error(MSFDM_EINVAL, "grid.fluid.nz=" + std::to_string(d.nz_fluid) + " too small for the top PML, npd=" +
+                                               std::to_string(d.npd));
         if (d.nz_solid < d.npd + 8)
             return set_error(MSFDM_EINVAL, "grid.solid.nz=" + std::to_string(d.nz_solid) + " too small for npd=" + std::to_string(d.npd));
     }

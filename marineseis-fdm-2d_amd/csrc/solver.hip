@@ -193,7 +193,10 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
     const int grid = 8 * ((ntiles + 7) / 8);
-    if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
+    if (s->geo.fcpml)   // top PML in the water; metric form decided at run time
+        D::stage_kernel<DIR, STAGE, TZ, 2, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
+                                                                            s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+    else if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
         D::stage_kernel<DIR, STAGE, TZ, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
                                                                              s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
     else
@@ -254,7 +257,7 @@ void drop_graph(msfdm_solver *s) {
 }
 
 int field_rows(const msfdm_solver *s, int f) {
-    if (f <= MSFDM_FXEZ) return s->d.nz_fluid;
+    if (f <= MSFDM_FXEZ || f >= MSFDM_FZVX) return s->d.nz_fluid;
     return s->d.nz_solid;
 }
 
@@ -264,6 +267,15 @@ int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float
     const int rows = field_rows(s, f);
     float *dp = dbuf + s->geo.off[f];
     const bool side = (f >= MSFDM_FXVX && f <= MSFDM_FXEZ) || (f >= MSFDM_SXVX && f <= MSFDM_SXEZ);
+    if (f >= MSFDM_FZVX) {
+        // fluid zeta memory: top strip, fluid rows 3 .. npd+2 (stored on the CPML grid only)
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (host_out) std::fill(host_out, host_out + (size_t)rows * nx, 0.0f);
+        if (!s->geo.fcpml || npd == 0) return MSFDM_OK;
+        if (host_out) HIPCHK(hipMemcpy2D(host_out + 3 * (size_t)nx, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
+        else HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in + 3 * (size_t)nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
+        return MSFDM_OK;
+    }
     const bool bottom = f >= MSFDM_SZVX;
     HIPCHK(hipStreamSynchronize(s->stream));
     if (!side && !bottom) {
@@ -329,10 +341,13 @@ void free_all(msfdm_solver *s) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
-// The batched, adjoint and multi-shot paths serve the fluid-solid grid only.
+// The batched, adjoint and multi-shot paths serve the fluid-solid grid with
+// the pressure-release surface only.
 int require_fluid_solid(const msfdm_solver *s) {
     if (s->solid_only)
         return set_error(MSFDM_EINVAL, "solid-only grids (grid.calc_type = 0) support the forward stepper only");
+    if (s->geo.fcpml)
+        return set_error(MSFDM_EINVAL, "the fluid-solid CPML grid (grid.boundary_type = 1) supports the forward stepper only");
     return MSFDM_OK;
 }
 
@@ -364,6 +379,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // solid-only boundary kinds: 2 free top + PML (PANDF), 1 PML on four sides
     // (CPML), 0 Dirichlet: no PML (src/elements.cu:825-827)
     s->top = !s->solid_only ? D::TOP_IFACE : (s->d.boundary_type == 2 ? D::TOP_FREE : D::TOP_PLAIN);
+    const bool fcpml = !s->solid_only && s->d.boundary_type == 1;   // top PML in the water
     const bool cpml = s->solid_only && s->d.boundary_type == 1, diri = s->solid_only && s->d.boundary_type == 0;
     auto fail = [&](int rc) { free_all(s); delete s; return rc; };
 #define CK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return fail(set_error(MSFDM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); } while (0)
@@ -375,6 +391,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     D::Geo &g = s->geo;
     g.nx = nx; g.nzf = nzf; g.nzs = nzs; g.P = (int)P; g.npd = npd; g.W2 = 2 * npd;
     g.ztop_pml = cpml ? 1 : 0;
+    g.fcpml = fcpml ? 1 : 0;
     long long off = 0;
     for (int f = 0; f < MSFDM_NFIELD; ++f) {
         g.off[f] = off;
@@ -383,7 +400,8 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
         else if (f <= MSFDM_FXEZ) n = (long long)nzf * g.W2;
         else if (f <= MSFDM_SSXZ) n = nzs * P;
         else if (f <= MSFDM_SXEZ) n = (long long)nzs * g.W2;
-        else n = (long long)(cpml ? 2 : 1) * npd * P;   // zeta strips: bottom (CPML: top + bottom)
+        else if (f <= MSFDM_SZEZ) n = (long long)(cpml ? 2 : 1) * npd * P;   // solid zeta: bottom (CPML: + top)
+        else n = fcpml ? (long long)npd * P : 0;                               // fluid zeta: top strip
         off += align64(n);
     }
     s->buf_floats = off;
@@ -398,8 +416,8 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
 
     // Coefficients, padded to pitch P (host prep in double where derived).
     const long long nF = align64(nzf * P), nS = align64(nzs * P), nG = align64(nz * P);
-    const long long nX = align64(P), nZ = align64(nzs);
-    const long long total = 2 * nF + 5 * nS + 5 * nG + 3 * nX + 3 * nZ + 2 * nX + 6 * nX;
+    const long long nX = align64(P), nZ = align64(nzs), nZf = align64(nzf);
+    const long long total = 2 * nF + 5 * nS + 5 * nG + 3 * nX + 3 * nZ + 3 * nZf + 2 * nX + 6 * nX;
     CK(hipMalloc(&s->coefmem, total * sizeof(float)));
     CK(hipMemset(s->coefmem, 0, total * sizeof(float)));
     std::vector<float> h;
@@ -448,6 +466,11 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     ptrs[np++] = c.pza = upload(pa, nZ);
     ptrs[np++] = c.pzb = upload(pb, nZ);
     ptrs[np++] = c.pzc = upload(pc, nZ);
+    // fluid zeta PML: the top strip of the CPML grid (src/elements.cu:870-873)
+    pml_profile(nzf, fcpml ? npd : 0, s->d.npow, true, false, damp0, s->d.beta0, alpha0, pa, pb, pc);
+    ptrs[np++] = c.pzaf = upload(pa, nZf);
+    ptrs[np++] = c.pzbf = upload(pb, nZf);
+    ptrs[np++] = c.pzcf = upload(pc, nZf);
     // Interface normal (src/elements.cu:892-905) in double (zero on a solid-only grid).
     std::vector<float> nif(2 * (size_t)nx, 0.0f);
     for (int ix = 0; ix < nx && !s->solid_only; ++ix) {
@@ -518,7 +541,7 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     // Updated rows: fluid 3..nzf-1, solid 0..nzs-4 (solid only: 3..nzs-4).
     const int srows = s->solid_only ? nzs - 6 : nzs - 3;
     const double cf = s->solid_only ? 0.0 : (double)(nzf - 3) * (nx - 6), cs = (double)srows * (nx - 6);
-    const double pmlf = s->solid_only ? 0.0 : (double)(nzf - 3) * 2 * npd;
+    const double pmlf = s->solid_only ? 0.0 : (double)(nzf - 3) * 2 * npd + (fcpml ? (double)npd * (nx - 6) : 0.0);
     const double pmls = (double)srows * 2 * npd + (double)(cpml ? 2 : 1) * npd * (nx - 6);
     // Vertical-column grids read neither xi_x nor xi_z: 2 coefficients fewer.
     const double ncf = c.vert ? 4 : 6, ncs = c.vert ? 7 : 9;
@@ -900,9 +923,12 @@ extern "C" int msfdm_batch_create(const msfdm_model *const *models, const int *s
     clear_error();
     if (!models || !out || nshots <= 0) return set_error(MSFDM_EINVAL, "bad argument");
     *out = nullptr;
-    for (int i = 0; i < nshots; ++i)
+    for (int i = 0; i < nshots; ++i) {
         if (models[i] && models[i]->d.calc_type == 0)
             return set_error(MSFDM_EINVAL, "solid-only grids (grid.calc_type = 0) support the forward stepper only");
+        if (models[i] && models[i]->d.boundary_type != 2)
+            return set_error(MSFDM_EINVAL, "the fluid-solid CPML grid (grid.boundary_type = 1) supports the forward stepper only");
+    }
     msfdm_batch *b = new msfdm_batch();
     b->device = device;
     auto fail = [&](int rc) { std::string msg = msfdm_last_error(); batch_free(b); delete b; return set_error(rc, msg); };

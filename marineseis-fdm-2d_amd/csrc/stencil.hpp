@@ -64,6 +64,7 @@ constexpr int NTHR = 256;       // 64 x 4 threads per stage block
 struct Geo {
     int nx, nzf, nzs, P, npd, W2;
     int ztop_pml;                 // 1: the solid zeta memory also has a top strip (solid-only CPML grid)
+    int fcpml;                    // 1: fluid-solid CPML grid (top PML in the water, fluid zeta memory)
     long long off[MSFDM_NFIELD];  // float offset of each field inside a state buffer
 };
 
@@ -73,6 +74,7 @@ struct Coef {
     const float *J, *xix, *xiz, *zex, *zez;              // [nz][P] global rows
     const float *pxa, *pxb, *pxc;                        // xi PML: al+d/be, d/be, 1/be  [nx]
     const float *pza, *pzb, *pzc;                        // solid zeta PML               [nzs]
+    const float *pzaf, *pzbf, *pzcf;                     // fluid zeta PML (CPML top)    [nzf]
     const float2 *nif;                                   // interface normal            [nx]
     const float *alm;                                    // free-surface ALM, solid-only grid [6][P]
     // Vertical grid columns (x = h xi everywhere): xi_x is the constant xix0 and
@@ -367,9 +369,13 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
 }
 
 // ---------------------------------------------------------------------------
-// Fluid stage: rows 3..nzf-1, columns 3..nx-4.
+// Fluid stage: rows 3..nzf-1, columns 3..nx-4.  FT = 0: pressure-release sea
+// surface on row 3 (PmlFs::FluidSolid); FT = 1: a top PML in the water on rows
+// 3..npd+2 instead (Cpml::FluidSolid, include/cpml_kernels.cuh:239-415): no
+// surface rows, the zeta operator closes only at the interface, and the top
+// strip carries the fluid zeta memory.
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ, int MET>
+template <int DIR, int STAGE, int TZ, int MET, int FT = 0>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
            const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
@@ -421,10 +427,11 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz;
         const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
+        const bool top = FT == 1 && iz <= g.npd + 2;   // fluid zeta-PML strip (CPML grid)
 
         // ---- velocity: rho dv/dt = -grad p (include/pml_fs_kernels.cuh:333-450)
-        float rvx, rvz, Dx, Dz;
-        if (dist_flso < 3 || dist_fs < 3) {
+        float rvx, rvz, Dx, Dz, Dzx = 0.f, Dzz = 0.f;
+        if (dist_flso < 3 || (FT == 0 && dist_fs < 3)) {
             const bool at_if = dist_flso < 3;
             // zeta flux -J p (ze_x, ze_z) at real fluid row rr of this column
             auto fze_row = [&](int rr, float &fx, float &fz) {
@@ -472,6 +479,11 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             const float proj = (xx * sx + xz * sz) / (gxixi * Jc);
             Dx = xx * proj;
             Dz = xz * proj;
+            if (top) {   // include/cpml_kernels.cuh:286-291
+                const float pze = (zx * sx + zz * sz) / ((zx * zx + zz * zz) * Jc);
+                Dzx = zx * pze;
+                Dzz = zz * pze;
+            }
         } else {
             const float dpx = dx_lds<DIR>(shp[r], cc);
             const float dpz = dz_lds<DIR>(shp, r, cc);
@@ -481,6 +493,12 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             const float t = (gxixi * dpx + gxize * dpz) / gxixi;
             Dx = -xx * t;
             Dz = -xz * t;
+            if (top) {   // include/cpml_kernels.cuh:304-305
+                const float gzeze = zx * zx + zz * zz;
+                const float tz = (gxize * dpx + gzeze * dpz) / gzeze;
+                Dzx = -zx * tz;
+                Dzz = -zz * tz;
+            }
         }
         const float rinv = cur.cf[0];
         float kvx = rvx * rinv, kvz = rvz * rinv;
@@ -489,10 +507,10 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         float kp = 0.0f, kEx = 0.0f, kEz = 0.0f;
         const float Km = cur.cf[1];
         float dvxx = 0.f, dvzx = 0.f, dvxz = 0.f, dvzz = 0.f;
-        if (dist_fs != 0) {
+        if (FT == 1 || dist_fs != 0) {
             dvxx = dx_lds<DIR>(shvx[r], cc);
             dvzx = dx_lds<DIR>(shvz[r], cc);
-            const int dzn = min(dist_fs, dist_flso);
+            const int dzn = FT == 1 ? dist_flso : min(dist_fs, dist_flso);
             dvxz = mac_dz<DIR>(shvx, r, cc, dzn, false);
             dvzz = mac_dz<DIR>(shvz, r, cc, dzn, false);
             kp = -Km * (xx * dvxx + zx * dvxz + xz * dvzx + zz * dvzz);
@@ -506,7 +524,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             kvx += ((pxc - 1.0f) * Dx - pxc * Vx) * rinv;
             kvz += ((pxc - 1.0f) * Dz - pxc * Vz) * rinv;
             const float Ex = S[g.off[MSFDM_FXEX] + ai], Ez = S[g.off[MSFDM_FXEZ] + ai];
-            if (dist_fs != 0) {
+            if (FT == 1 || dist_fs != 0) {
                 const float gxize = xx * zx + xz * zz;
                 const float Dvx = dvxx + (gxize / gxixi) * dvxz;
                 const float Dvz = dvzx + (gxize / gxixi) * dvzz;
@@ -518,6 +536,28 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXVZ] + ai, Vz, kVz, ct);
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXEX] + ai, Ex, kEx, ct);
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FXEZ] + ai, Ez, kEz, ct);
+        }
+        // ---- zeta-PML memory, top strip of the CPML grid (include/cpml_kernels.cuh:331-342, 399-413)
+        if (top) {
+            const long long ai = (long long)(iz - 3) * P + ix;
+            const float pza = c.pzaf[iz], pzb = c.pzbf[iz], pzc = c.pzcf[iz];
+            const float Vx = S[g.off[MSFDM_FZVX] + ai], Vz = S[g.off[MSFDM_FZVZ] + ai];
+            const float Ex = S[g.off[MSFDM_FZEX] + ai], Ez = S[g.off[MSFDM_FZEZ] + ai];
+            const float q1 = pzc - 1.0f;
+            const float kVx = -pza * Vx + pzb * Dzx;
+            const float kVz = -pza * Vz + pzb * Dzz;
+            kvx += (q1 * Dzx - pzc * Vx) * rinv;
+            kvz += (q1 * Dzz - pzc * Vz) * rinv;
+            const float gxize = xx * zx + xz * zz, gzeze = zx * zx + zz * zz;
+            const float Dvx = dvxz + (gxize / gzeze) * dvxx;
+            const float Dvz = dvzz + (gxize / gzeze) * dvzx;
+            const float kEx = -pza * Ex + pzb * Dvx;
+            const float kEz = -pza * Ez + pzb * Dvz;
+            kp += Km * (-q1 * (zx * Dvx + zz * Dvz) + pzc * (zx * Ex + zz * Ez));
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FZVX] + ai, Vx, kVx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FZVZ] + ai, Vz, kVz, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FZEX] + ai, Ex, kEx, ct);
+            rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FZEZ] + ai, Ez, kEz, ct);
         }
         // ---- source: 5x5 Gaussian-weighted Ricker on p_k (include/tools_func.cuh:21-37)
         {
@@ -553,7 +593,9 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 // include/cpml_kernels.cuh:8-233) or no PML (Dirichlet, npd = 0,
 // include/dirichlet_kernels.cuh:7-107).  Columns 3..nx-4 in all.
 // ---------------------------------------------------------------------------
-constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2;
+// TOP_IFACE_CPML: TOP_IFACE on the fluid-solid CPML grid, whose sigma_xz rate
+// reads its metrics at the solid-array row (include/cpml_kernels.cuh:447-452).
+constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2, TOP_IFACE_CPML = 3;
 template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
@@ -562,7 +604,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
            int bz) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    constexpr bool SOLO = TOP != TOP_IFACE;   // a solid-only grid
+    constexpr bool SOLO = TOP == TOP_FREE || TOP == TOP_PLAIN;   // a solid-only grid
     const int x0 = bx * TX, z0 = (SOLO ? 3 : 0) + bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     const int ix = x0 + tx;
@@ -640,7 +682,13 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
             ksx = c11 * exx + c13 * ezz;
             ksz = c13 * exx + c33 * ezz;
-            ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+            if constexpr (TOP == TOP_IFACE_CPML) {   // metrics of global row iz (not iz + nzf - 1)
+                float xxq, xzq;
+                xi_metrics<MET>(c, si, xxq, xzq);
+                ksxz = c55 * (xzq * dvxx + c.zez[si] * dvxz + xxq * dvzx + c.zex[si] * dvzz);
+            } else {
+                ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+            }
         }
 
         // ---- velocity (include/pml_fs_kernels.cuh:631-752; solid-only: 189-305)
@@ -830,7 +878,8 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 #ifndef MSFDM_STAGE_WAVES
 #define MSFDM_STAGE_WAVES 0   // minimum waves per SIMD requested for stage_kernel (0 = compiler choice)
 #endif
-template <int DIR, int STAGE, int TZ, int MET>
+// CPML = 1: the fluid-solid CPML grid (top PML in the water, Cpml::FluidSolid).
+template <int DIR, int STAGE, int TZ, int MET, int CPML = 0>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
              const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
@@ -842,10 +891,11 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
     if (band < nbz_f)
-        fluid_body<DIR, STAGE, TZ, MET>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx, band);
+        fluid_body<DIR, STAGE, TZ, MET, CPML>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx,
+                                              band);
     else
-        solid_body<DIR, STAGE, TZ, MET>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx,
-                                        band - nbz_f);
+        solid_body<DIR, STAGE, TZ, MET, CPML ? TOP_IFACE_CPML : TOP_IFACE>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep,
+                                                                          ct, amp_direct, cprev, bx, band - nbz_f);
 }
 
 // One RK stage of a solid-only grid (PmlFs::SolidOnly, src/pml_fs_kernels.cu:29-381;

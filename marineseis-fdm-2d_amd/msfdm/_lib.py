@@ -19,9 +19,16 @@ OK, EINVAL, EIO, EPARSE, EDEVICE, ESTATE = range(6)
 
 FIELDS = ["FVX", "FVZ", "FP", "FXVX", "FXVZ", "FXEX", "FXEZ",
           "SVX", "SVZ", "SSX", "SSZ", "SSXZ", "SXVX", "SXVZ", "SXEX", "SXEZ",
-          "SZVX", "SZVZ", "SZEX", "SZEZ"]
+          "SZVX", "SZVZ", "SZEX", "SZEZ", "FZVX", "FZVZ", "FZEX", "FZEZ"]
 FID = {n: i for i, n in enumerate(FIELDS)}
 NFIELD = len(FIELDS)
+
+
+def is_fluid(f: int) -> bool:
+    """Field f lives on the fluid grid ([nz_fluid][nx])."""
+    return f <= FID["FXEZ"] or f >= FID["FZVX"]
+
+
 ARRAYS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s", "J", "xi_x", "xi_z", "ze_x", "ze_z"]
 
 # Exported symbols (checked by the CPU tests against include/msfdm.h).
@@ -250,7 +257,7 @@ class Solver:
             pass
 
     def shape(self, f: int):
-        return (self.nzf if f <= FID["FXEZ"] else self.nzs, self.nx)
+        return (self.nzf if is_fluid(f) else self.nzs, self.nx)
 
     def reset(self, isrc: int = 0):
         check(lib().msfdm_reset_shot(self.h, int(isrc)))

@@ -154,8 +154,9 @@ def build_model(nx: int, nz: int, h: float, water_depth: float, *, npd: int, fpe
                 random_std: float = 0.0, corr_len: float = 50.0, seed: int = 0,
                 snapshot: int | None = None, ppw0: float = 8.0, rc: float = 1e-3,
                 npow: int = 2, vp_max: float | None = None, metrics: str = "analytic",
-                device: int = 0) -> Model:
-    """Build a fluid-over-solid model.
+                device: int = 0, boundary_type: int = PANDF) -> Model:
+    """Build a fluid-over-solid model: pressure-release sea surface on row 3
+    (``boundary_type`` PANDF) or a top PML in the water (CPML).
 
     ``seafloor`` is a list of ``(amplitude, wavelength)`` terms of the interface
     undulation ``sum A sin(2 pi x / lambda)`` around ``water_depth``.  Sources are
@@ -167,6 +168,10 @@ def build_model(nx: int, nz: int, h: float, water_depth: float, *, npd: int, fpe
     nz_solid = nz - nz_fluid + 1
     if nz_solid < npd + 8 or nz_fluid < 8:
         raise ValueError("grid too small for the water depth / PML width")
+    if boundary_type not in (CPML, PANDF):
+        raise ValueError(f"a fluid-solid model takes boundary_type 1 or 2, not {boundary_type}")
+    if boundary_type == CPML and nz_fluid < npd + 6:
+        raise ValueError("water too shallow for the top PML")
     b_pml = float(nz - 4 - npd)
     w, dw = _blend(nz, zeta0, b_pml)
     x = h * np.arange(nx, dtype=np.float64)
@@ -219,7 +224,8 @@ def build_model(nx: int, nz: int, h: float, water_depth: float, *, npd: int, fpe
                  snapshot=(nt if snapshot is None else snapshot), npd=npd,
                  thickness=(npd - 1) * h, npow=npow,
                  vp_max=float(vp_max if vp_max is not None else max(vp_s, vp_f)), rc=rc,
-                 beta0=round(beta0, 4), src_x=src_x, src_z=src_z, arrays=arrays)
+                 beta0=round(beta0, 4), src_x=src_x, src_z=src_z, arrays=arrays,
+                 boundary_type=boundary_type)
 
 
 def build_solid_model(nx: int, nz: int, h: float, *, npd: int, fpeak: float, dt: float, nt: int,
@@ -310,6 +316,15 @@ def preset(name: str, **over) -> Model:
                   src_x=[60, 40], src_height=15.0, seafloor=[(12.0, 200.0), (4.0, 70.0)],
                   solid=(3000.0, 1700.0, 2200.0), random_std=0.05, corr_len=25.0, seed=7,
                   vp_max=3000.0)
+    elif name == "cpml_fs_tiny":
+        kw = dict(nx=120, nz=96, h=5.0, water_depth=150.0, npd=10, fpeak=25.0, dt=4e-4, nt=300,
+                  src_x=[60, 40], src_height=15.0, seafloor=[(12.0, 200.0), (4.0, 70.0)],
+                  solid=(3000.0, 1700.0, 2200.0), random_std=0.05, corr_len=25.0, seed=8,
+                  vp_max=3000.0, boundary_type=CPML)
+    elif name == "cpml_fs401":
+        kw = dict(nx=401, nz=201, h=5.0, water_depth=200.0, npd=20, fpeak=15.0, dt=5e-4, nt=2000,
+                  src_x=[200], src_height=10.0, solid=(3000.0, 1700.0, 2200.0), vp_max=3000.0,
+                  boundary_type=CPML)
     elif name == "solid_flat301":
         return build_solid_model(**{**dict(nx=301, nz=161, h=5.0, npd=20, fpeak=15.0, dt=5e-4, nt=1500,
                                            src_x=[150], src_z=[6], solid=(3200.0, 1732.0, 2000.0)), **over})

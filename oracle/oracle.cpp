@@ -74,9 +74,10 @@ enum Field {
     SVX, SVZ, SSX, SSZ, SSXZ,          // solid state
     SXVX, SXVZ, SXEX, SXEZ,            // solid xi-PML memory
     SZVX, SZVZ, SZEX, SZEZ,            // solid zeta-PML memory (bottom)
+    FZVX, FZVZ, FZEX, FZEZ,            // fluid zeta-PML memory (top; fluid-solid CPML grid)
     NFIELD
 };
-inline bool is_fluid(int f) { return f <= FXEZ; }
+inline bool is_fluid(int f) { return f <= FXEZ || f >= FZVX; }
 
 // One-sided 5-point operators along a line; f(m) returns the value at offset m
 // (include/tools_func.cuh:51-157).  Unit computational spacing: metrics carry 1/h.
@@ -129,9 +130,11 @@ struct Oracle {
     // per-column ALM coefficients al1_xx, al1_zz, al1_xz, al2_xx, al2_zz, al2_xz.
     bool solid_only = false;
     std::vector<double> alm[6];
-    // grid.boundary_type of a solid-only grid: 2 = PML sides/bottom + free
-    // surface (PANDF), 1 = PML on all four sides (CPML), 0 = Dirichlet (DIRI).
+    // grid.boundary_type: 2 = PML sides/bottom + free surface (PANDF), 1 = PML
+    // on all four sides (CPML), 0 = Dirichlet (DIRI, solid-only grids).
     int bdy = 2;
+    bool fcpml() const { return !solid_only && bdy == 1; }   // fluid-solid CPML grid
+    std::vector<double> al_zf, be_zf, dp_zf;                  // fluid zeta PML (top strip)
     bool zpml(int iz) const {   // zeta-PML rows (bottom; CPML also top, src/elements.cu:866-869)
         if (solid_only && bdy == 0) return false;
         const bool bot = iz >= nzs - 3 - npd;
@@ -216,18 +219,21 @@ inline bool in_side(const Oracle &o, int ix) {
     return distx >= 0 && distx < o.npd;
 }
 
-// update_velocity_fluid<DIR> (include/pml_fs_kernels.cuh:333-450).
+// update_velocity_fluid<DIR> (include/pml_fs_kernels.cuh:333-450).  On the
+// fluid-solid CPML grid (include/cpml_kernels.cuh:239-342) only the interface
+// rows take the conservative form and rows 3..npd+2 carry the top zeta memory.
 void velocity_fluid(Oracle &o, const double *const *S, double *const *Kd, int dir) {
     const int nx = o.nx, nzf = o.nzf;
+    const bool cpml = o.fcpml();
 #pragma omp parallel for schedule(static)
     for (int iz = 3; iz < nzf; ++iz) {
         for (int ix = 3; ix <= nx - 4; ++ix) {
             const size_t idx = (size_t)iz * nx + ix, g = o.gf(iz, ix);
             const double J = o.J[g], xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
             const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
-            const double gxixi = xx * xx + xz * xz;
-            double rvx, rvz, Dx, Dz;
-            if (dist_flso < 3 || dist_fs < 3) {
+            const double gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz;
+            double rvx, rvz, Dx, Dz, Dzx, Dzz;
+            if (dist_flso < 3 || (!cpml && dist_fs < 3)) {
                 const bool at_if = dist_flso < 3;
                 auto Fxi = [&](int m, int c) {
                     double fx, fz; fxi_f(o, S[FP], iz, ix + m, fx, fz); return c ? fz : fx;
@@ -250,6 +256,8 @@ void velocity_fluid(Oracle &o, const double *const *S, double *const *Kd, int di
                 rvz = sz / J;
                 Dx = xx * (xx * sx + xz * sz) / (gxixi * J);
                 Dz = xz * (xx * sx + xz * sz) / (gxixi * J);
+                Dzx = zx * (zx * sx + zz * sz) / (gzeze * J);
+                Dzz = zz * (zx * sx + zz * sz) / (gzeze * J);
             } else {
                 const double *p = S[FP];
                 const double dpx = drp(dir, [&](int m) { return p[idx + m]; });
@@ -259,6 +267,8 @@ void velocity_fluid(Oracle &o, const double *const *S, double *const *Kd, int di
                 const double gxize = xx * zx + xz * zz;
                 Dx = -xx * (gxixi * dpx + gxize * dpz) / gxixi;
                 Dz = -xz * (gxixi * dpx + gxize * dpz) / gxixi;
+                Dzx = -zx * (gxize * dpx + gzeze * dpz) / gzeze;
+                Dzz = -zz * (gxize * dpx + gzeze * dpz) / gzeze;
             }
             const double rho = o.rhof[idx];
             Kd[FVX][idx] = rvx / rho;
@@ -270,22 +280,33 @@ void velocity_fluid(Oracle &o, const double *const *S, double *const *Kd, int di
                 Kd[FVX][idx] += ((invb - 1.0) * Dx - invb * S[FXVX][idx]) / rho;
                 Kd[FVZ][idx] += ((invb - 1.0) * Dz - invb * S[FXVZ][idx]) / rho;
             }
+            if (cpml && iz <= o.npd + 2) {
+                const double be = o.be_zf[iz], dpb = o.dp_zf[iz] / be, apdp = o.al_zf[iz] + dpb, invb = 1.0 / be;
+                Kd[FZVX][idx] = -apdp * S[FZVX][idx] + dpb * Dzx;
+                Kd[FZVZ][idx] = -apdp * S[FZVZ][idx] + dpb * Dzz;
+                Kd[FVX][idx] += ((invb - 1.0) * Dzx - invb * S[FZVX][idx]) / rho;
+                Kd[FVZ][idx] += ((invb - 1.0) * Dzz - invb * S[FZVZ][idx]) / rho;
+            }
         }
     }
 }
 
-// update_stress_fluid<DIR> (include/pml_fs_kernels.cuh:452-507).
+// update_stress_fluid<DIR> (include/pml_fs_kernels.cuh:452-507).  On the
+// fluid-solid CPML grid (include/cpml_kernels.cuh:344-415) there is no
+// pressure-release row: the zeta operator closes only at the interface and
+// rows 3..npd+2 carry the top zeta memory.
 void stress_fluid(Oracle &o, const double *const *S, double *const *Kd, int dir) {
     const int nx = o.nx, nzf = o.nzf;
+    const bool cpml = o.fcpml();
 #pragma omp parallel for schedule(static)
     for (int iz = 3; iz < nzf; ++iz) {
         for (int ix = 3; ix <= nx - 4; ++ix) {
             const size_t idx = (size_t)iz * nx + ix, g = o.gf(iz, ix);
-            if (iz == 3) { Kd[FP][idx] = 0.0; continue; }   // pressure-release row
+            if (!cpml && iz == 3) { Kd[FP][idx] = 0.0; continue; }   // pressure-release row
             const double *vx = S[FVX], *vz = S[FVZ];
             const double dvxx = drp(dir, [&](int m) { return vx[idx + m]; });
             const double dvzx = drp(dir, [&](int m) { return vz[idx + m]; });
-            const int dzn = std::min(iz - 3, nzf - 1 - iz);
+            const int dzn = cpml ? nzf - 1 - iz : std::min(iz - 3, nzf - 1 - iz);
             const double dvxz = mac_dze(dir, dzn, false, [&](int m) { return vx[idx + (std::ptrdiff_t)m * nx]; });
             const double dvzz = mac_dze(dir, dzn, false, [&](int m) { return vz[idx + (std::ptrdiff_t)m * nx]; });
             const double xx = o.xix[g], xz = o.xiz[g], zx = o.zex[g], zz = o.zez[g];
@@ -300,6 +321,15 @@ void stress_fluid(Oracle &o, const double *const *S, double *const *Kd, int dir)
                 Kd[FXEZ][idx] = -apdp * S[FXEZ][idx] + dpb * Dvz;
                 Kd[FP][idx] += Km * (-(invb - 1.0) * xx * Dvx - (invb - 1.0) * xz * Dvz
                                      + invb * xx * S[FXEX][idx] + invb * xz * S[FXEZ][idx]);
+            }
+            if (cpml && iz <= o.npd + 2) {
+                const double gxize = xx * zx + xz * zz, gzeze = zx * zx + zz * zz;
+                const double Dvx = dvxz + (gxize / gzeze) * dvxx;
+                const double Dvz = dvzz + (gxize / gzeze) * dvzx;
+                const double be = o.be_zf[iz], dpb = o.dp_zf[iz] / be, apdp = o.al_zf[iz] + dpb, invb = 1.0 / be;
+                Kd[FZEX][idx] = -apdp * S[FZEX][idx] + dpb * Dvx;
+                Kd[FZEZ][idx] = -apdp * S[FZEZ][idx] + dpb * Dvz;
+                Kd[FP][idx] += Km * (-(invb - 1.0) * (zx * Dvx + zz * Dvz) + invb * (zx * S[FZEX][idx] + zz * S[FZEZ][idx]));
             }
         }
     }
@@ -321,7 +351,15 @@ void stress_solid(Oracle &o, const double *const *S, double *const *Kd, int dir)
             const double c11 = o.C11[idx], c13 = o.C13[idx], c33 = o.C33[idx], c55 = o.C55[idx];
             double ksx = c11 * (xx * dvxx + zx * dvxz) + c13 * (xz * dvzx + zz * dvzz);
             double ksz = c13 * (xx * dvxx + zx * dvxz) + c33 * (xz * dvzx + zz * dvzz);
-            double ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+            double ksxz;
+            if (o.fcpml()) {
+                // include/cpml_kernels.cuh:447-452 reads this term's metrics at the
+                // solid-array index (global row iz, not iz + nzf - 1); kept for parity
+                const size_t q = idx;
+                ksxz = c55 * (o.xiz[q] * dvxx + o.zez[q] * dvxz + o.xix[q] * dvzx + o.zex[q] * dvzz);
+            } else {
+                ksxz = c55 * (xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz);
+            }
             const bool side = in_side(o, ix), bottom = iz >= nzs - 3 - npd;
             if (side || bottom) {
                 const double gxixi = xx * xx + xz * xz, gxize = xx * zx + xz * zz, gzeze = zx * zx + zz * zz;
@@ -1317,7 +1355,7 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
                  const float *zez) {
     Oracle *o = new Oracle();
     o->solid_only = d->calc_type == 0;
-    o->bdy = o->solid_only ? d->boundary_type : 2;
+    o->bdy = d->boundary_type;
     o->nx = d->nx; o->nzf = o->solid_only ? 0 : d->nz_fluid; o->nzs = d->nz_solid; o->npd = d->npd;
     o->nz = o->solid_only ? o->nzs : o->nzf + o->nzs - 1;   // src/elements.cu:87-99
     o->dt_f = d->dt; o->fpeak_f = d->fpeak; o->dt = (double)d->dt;
@@ -1334,6 +1372,11 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
     o->al_xi.resize(o->nx); o->be_xi.resize(o->nx); o->dp_xi.resize(o->nx);
     o->al_zs.resize(o->nzs); o->be_zs.resize(o->nzs); o->dp_zs.resize(o->nzs);
     pml_profile(o->nx, o->npd, d->npow, true, true, damp0, d->beta0, alpha0, o->al_xi.data(), o->be_xi.data(), o->dp_xi.data());
+    // fluid zeta: top PML on the fluid-solid CPML grid (src/elements.cu:870-873)
+    o->al_zf.assign(o->nzf, 0.0); o->be_zf.assign(o->nzf, 1.0); o->dp_zf.assign(o->nzf, 0.0);
+    if (o->fcpml())
+        pml_profile(o->nzf, o->npd, d->npow, true, false, damp0, d->beta0, alpha0, o->al_zf.data(), o->be_zf.data(),
+                    o->dp_zf.data());
     // zeta: bottom PML; the solid-only CPML grid also has a top PML (src/elements.cu:863-869)
     pml_profile(o->nzs, o->npd, d->npow, o->solid_only && o->bdy == 1, true, damp0, d->beta0, alpha0, o->al_zs.data(),
                 o->be_zs.data(), o->dp_zs.data());
@@ -1426,10 +1469,13 @@ void orc_filter(void *h) {
     filter(*o, Ud);
 }
 
-// PML profiles: axis 0 = xi [nx], 1 = solid zeta [nzs]; out = al, be, dp.
+// PML profiles: axis 0 = xi [nx], 1 = solid zeta [nzs], 2 = fluid zeta [nzf];
+// out = al, be, dp.
 void orc_pml(void *h, int axis, double *al, double *be, double *dp) {
     Oracle *o = static_cast<Oracle *>(h);
-    const auto &A = axis ? o->al_zs : o->al_xi, &Bv = axis ? o->be_zs : o->be_xi, &D = axis ? o->dp_zs : o->dp_xi;
+    const auto &A = axis == 0 ? o->al_xi : axis == 1 ? o->al_zs : o->al_zf;
+    const auto &Bv = axis == 0 ? o->be_xi : axis == 1 ? o->be_zs : o->be_zf;
+    const auto &D = axis == 0 ? o->dp_xi : axis == 1 ? o->dp_zs : o->dp_zf;
     std::memcpy(al, A.data(), A.size() * sizeof(double));
     std::memcpy(be, Bv.data(), Bv.size() * sizeof(double));
     std::memcpy(dp, D.data(), D.size() * sizeof(double));

@@ -16,13 +16,13 @@ _LIB = None
 # Field numbering (matches oracle.cpp and include/msfdm.h).
 FIELDS = ["FVX", "FVZ", "FP", "FXVX", "FXVZ", "FXEX", "FXEZ",
           "SVX", "SVZ", "SSX", "SSZ", "SSXZ", "SXVX", "SXVZ", "SXEX", "SXEZ",
-          "SZVX", "SZVZ", "SZEX", "SZEZ"]
+          "SZVX", "SZVZ", "SZEX", "SZEZ", "FZVX", "FZVZ", "FZEX", "FZEZ"]
 FID = {n: i for i, n in enumerate(FIELDS)}
 NFIELD = len(FIELDS)
 
 
 def is_fluid(f: int) -> bool:
-    return f <= FID["FXEZ"]
+    return f <= FID["FXEZ"] or f >= FID["FZVX"]
 
 
 class _Desc(C.Structure):
@@ -163,7 +163,8 @@ class Oracle:
         lib().orc_filter(self.h)
 
     def pml(self, axis: int):
-        n = self.nx if axis == 0 else self.nzs
+        """axis 0 = xi, 1 = solid zeta, 2 = fluid zeta (top strip of the CPML grid)."""
+        n = self.nx if axis == 0 else (self.nzs if axis == 1 else self.nzf)
         out = [np.zeros(n) for _ in range(3)]
         lib().orc_pml(self.h, axis, *[_dptr(o) for o in out])
         return out   # alpha, beta, d

@@ -1,0 +1,133 @@
+"""GPU parity of the fluid-solid CPML stepper (grid.calc_type = 1, boundary_type = 1:
+a top PML in the water instead of the pressure-release surface, Cpml::FluidSolid,
+include/cpml_kernels.cuh:237-770, src/cpml_kernels.cu:363-892) through the C ABI
+against the FP64 oracle.
+
+Tolerances: per-field right-hand sides on random states within 2e-5 of the
+field's max |k|; records (solid row-0 vx, vz, fluid interface-row p) within a
+relative L2 error of 1e-4 over the whole record.
+"""
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import msfdm
+from msfdm import modelgen
+import pyoracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+RECORD_TOL = 1e-4
+
+
+def _general_metrics(m):
+    a = dict(m.arrays)
+    nz, nx = a["xi_x"].shape
+    zz, xx = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    a["xi_x"] = (a["xi_x"] * (1.0 + 0.03 * np.sin(xx / 9.0) * np.cos(zz / 7.0))).astype(np.float32)
+    a["xi_z"] = (0.02 * a["xi_x"] * np.sin(zz / 11.0 + xx / 13.0)).astype(np.float32)
+    return dataclasses.replace(m, arrays=a)
+
+
+def _model(kind):
+    if kind == "general":
+        return _general_metrics(modelgen.preset("cpml_fs_tiny"))
+    return modelgen.preset(kind)
+
+
+@pytest.mark.parametrize("kind", ["cpml_fs_tiny", "general"])
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("tile_rows", [8, 16])
+def test_cpml_rhs_matches_oracle_on_random_state(kind, direction, tile_rows):
+    print(f"[cpml rhs] {kind} dir={direction} tile_rows={tile_rows}", flush=True)
+    m = _model(kind)
+    o = pyoracle.Oracle(m)
+    s = msfdm.Solver(m)
+    s.tile_rows = tile_rows
+    rng = np.random.default_rng(41 + direction)
+    for i, name in enumerate(msfdm.FIELDS):
+        rows = m.nz_fluid if msfdm._lib.is_fluid(i) else m.nz_solid
+        a32 = rng.standard_normal((rows, m.nx)).astype(np.float32)
+        o.set(name, a32.astype(np.float64))
+        s.set(name, a32)
+    amp = 0.731
+    o.rhs(direction, amp)
+    s.eval_rhs(direction, amp)
+    for name in msfdm.FIELDS:
+        ko = o.get_k(name)
+        kg = s.get_rhs(name).astype(np.float64)
+        scale = np.abs(ko).max()
+        assert scale > 0, name
+        err = np.abs(kg - ko).max() / scale
+        assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
+
+
+@pytest.mark.parametrize("kind,nsteps,tile_rows,isrc", [("cpml_fs_tiny", 300, 8, 0), ("cpml_fs_tiny", 300, 16, 1),
+                                                        ("general", 200, 8, 0), ("cpml_fs401", 600, 16, 0)])
+def test_cpml_records_match_oracle(kind, nsteps, tile_rows, isrc):
+    print(f"[cpml records] {kind} nsteps={nsteps} tile_rows={tile_rows} shot={isrc}", flush=True)
+    m = _model(kind)
+    o = pyoracle.Oracle(m)
+    o.reset(isrc)
+    o.run(nsteps)
+    s = msfdm.Solver(m)
+    s.tile_rows = tile_rows
+    s.reset(isrc)
+    s.run(nsteps)
+    s.sync()
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), s.records(), o.records()):
+        assert a.shape == b.shape and np.abs(b).max() > 0
+        err = rel_l2(a, b)
+        print(f"  {name} rel L2 {err:.3e}", flush=True)
+        assert err <= RECORD_TOL, name
+    for name in ("FP", "FVZ", "FZVZ", "FZEX", "SVX", "SSXZ"):
+        assert rel_l2(s.get(name), o.get(name)) <= RECORD_TOL, name
+
+
+def test_cpml_top_strip_and_determinism():
+    """The fluid zeta memory lives on rows 3..npd+2 only; two runs are bitwise
+    equal; graph replay equals direct launches; p on row 3 is not clamped."""
+    m = modelgen.preset("cpml_fs_tiny")
+    runs = []
+    for graph in (True, True, False):
+        s = msfdm.Solver(m)
+        s.set_graph(graph)
+        s.reset(0)
+        s.run(160)
+        s.sync()
+        runs.append({f: s.get(f) for f in ("FP", "FVX", "FZVX", "FZEZ", "SVZ")})
+    for f, a in runs[0].items():
+        assert np.array_equal(a, runs[1][f]) and np.array_equal(a, runs[2][f]), f
+    z = runs[0]["FZVX"]
+    assert not z[:3].any() and not z[3 + m.npd:].any() and np.abs(z[3:3 + m.npd]).max() > 0
+    assert np.abs(runs[0]["FP"][3]).max() > 0
+
+
+def test_cpml_rejects_pandf_only_paths():
+    m = modelgen.preset("cpml_fs_tiny")
+    s = msfdm.Solver(m)
+    with pytest.raises(msfdm.MsfdmError) as e:
+        s.adjoint_forward(0, 20)
+    assert "CPML" in str(e.value)
+    with pytest.raises(msfdm.MsfdmError):
+        msfdm.Batch([m, m])
+
+
+def test_cpml_cli_layout(tmp_path):
+    m = modelgen.preset("cpml_fs_tiny", nt=90, snapshot=40)
+    j = modelgen.write_model(m, str(tmp_path / "model"), "cpml", relative_to=str(tmp_path))
+    r = subprocess.run([msfdm._lib.BIN_PATH, os.path.relpath(j, tmp_path), "out"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = tmp_path / "out" / "0" / "record"
+    assert sorted(os.listdir(rec)) == ["fluid_p.bin", "solid_vx.bin", "solid_vz.bin"]
+    s = msfdm.Solver(m)
+    s.reset(0)
+    s.run(90)
+    for name, a in zip(("solid_vx", "solid_vz", "fluid_p"), s.records()):
+        got = np.fromfile(rec / f"{name}.bin", dtype="<f4").reshape(-1, m.nx)
+        assert np.array_equal(got, a), name

@@ -22,7 +22,7 @@ def _random_state(m, seed):
     rng = np.random.default_rng(seed)
     st = {}
     for i, name in enumerate(msfdm.FIELDS):
-        rows = m.nz_fluid if i <= msfdm.FID["FXEZ"] else m.nz_solid
+        rows = m.nz_fluid if msfdm._lib.is_fluid(i) else m.nz_solid
         st[name] = rng.standard_normal((rows, m.nx))
     return st
 

@@ -82,7 +82,7 @@ def test_json_number_semantics(model_dir):
     (lambda p: _edit_json(p, lambda d: d["grid"]["src_pos"].__setitem__("posz", [])), EPARSE, "out of range"),
     (lambda p: _edit_json(p, lambda d: d["grid"]["fluid"].__setitem__("nx", 7)), EINVAL, "nx_fluid doesn't match nx_solid"),
     (lambda p: _edit_json(p, lambda d: d["grid"].__setitem__("calc_type", 2)), EINVAL, "calc_type"),
-    (lambda p: _edit_json(p, lambda d: d["grid"].__setitem__("boundary_type", 1)), EINVAL, "boundary_type"),
+    (lambda p: _edit_json(p, lambda d: d["grid"].__setitem__("boundary_type", 0)), EINVAL, "boundary_type"),
     (lambda p: _edit_json(p, lambda d: d["grid"]["jacob"].__setitem__("J", "nope.bin")), EIO, "Failed to open file: nope.bin"),
     (lambda p: _edit_json(p, lambda d: d["grid"]["src_pos"].__setitem__("posx", [100000])), EINVAL, "outside"),
 ])

@@ -94,7 +94,7 @@ def test_solid_batch_is_rejected_before_any_device_work():
 def test_solid_boundary_kinds_and_their_limits():
     """Solid-only grids accept boundary_type 2 (free top), 1 (PML on four sides:
     the strips must not meet and the receiver row 30 must be updated) and 0
-    (Dirichlet: no PML, npd unused); a fluid-solid grid only boundary_type 2."""
+    (Dirichlet: no PML, npd unused); a fluid-solid grid boundary_type 2 and 1."""
     for bt, nz_ok, nz_bad in ((1, 34, 33), (0, 10, 9), (2, 20, 19)):
         m = modelgen.build_solid_model(nx=60, nz=60, h=5.0, npd=10, fpeak=20.0, dt=4e-4, nt=10, src_x=[30],
                                        src_z=[8], boundary_type=bt)
@@ -108,12 +108,12 @@ def test_solid_boundary_kinds_and_their_limits():
                 with pytest.raises(msfdm.MsfdmError) as e:
                     msfdm.HostModel.from_model(mm)
                 assert e.value.code == EINVAL and "too small" in str(e.value)
+    import dataclasses
     fs = modelgen.preset("tiny_curvi")
-    for bt in (0, 1):
-        import dataclasses
-        with pytest.raises(msfdm.MsfdmError) as e:
-            msfdm.HostModel.from_model(dataclasses.replace(fs, boundary_type=bt))
-        assert e.value.code == EINVAL and "supported" in str(e.value)
+    with pytest.raises(msfdm.MsfdmError) as e:   # Dirichlet edges serve solid-only grids
+        msfdm.HostModel.from_model(dataclasses.replace(fs, boundary_type=0))
+    assert e.value.code == EINVAL and "supported" in str(e.value)
+    assert msfdm.HostModel.from_model(dataclasses.replace(fs, boundary_type=1)).desc().boundary_type == 1
     with pytest.raises(ValueError):
         modelgen.build_solid_model(nx=60, nz=60, h=5.0, npd=10, fpeak=20.0, dt=4e-4, nt=10, src_x=[30],
                                    src_z=[8], boundary_type=1, topography=[(5.0, 100.0)])

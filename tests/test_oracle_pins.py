@@ -183,12 +183,12 @@ def test_ricker_closed_form():
     assert o.stage_amp(17, 1) == o.stage_amp(17, 2)
 
 
-def _plane_wave_run(water_depth, z_pulse, nsteps, probe_rows, pulse_down=True):
+def _plane_wave_run(water_depth, z_pulse, nsteps, probe_rows, pulse_down=True, boundary_type=2):
     """x-invariant P pulse in the water column of a flat model; records p at probe rows."""
     h = 2.5
     m = modelgen.build_model(nx=240, nz=130, h=h, water_depth=water_depth, npd=10, fpeak=40.0,
                              dt=2e-4, nt=nsteps, src_x=[], src_z=[],
-                             solid=(3000.0, 1700.0, 2200.0), vp_max=3000.0)
+                             solid=(3000.0, 1700.0, 2200.0), vp_max=3000.0, boundary_type=boundary_type)
     o = pyoracle.Oracle(m)
     nzf, nx = m.nz_fluid, m.nx
     rows = np.arange(nzf, dtype=np.float64)
@@ -240,6 +240,38 @@ def test_pressure_release_surface_reflection_is_minus_one():
     a_inc = sig[max(0, i_inc - w):i_inc + w].max()
     a_ref = sig[i_ref - w:i_ref + w].min()
     assert a_ref / a_inc == pytest.approx(-1.0, rel=0.03)
+
+
+def test_cpml_sea_surface_absorbs_the_upgoing_pulse():
+    """With the top PML of the fluid-solid CPML grid (include/cpml_kernels.cuh:239-415)
+    the same upgoing pulse that the pressure-release surface returns with -1
+    comes back with less than 2 % of its amplitude."""
+    m, tr = _plane_wave_run(water_depth=250.0, z_pulse=100.0, nsteps=620, probe_rows=[3 + 24],
+                            pulse_down=False, boundary_type=1)
+    sig = tr[27]
+    dt = m.dt
+    i_inc = int(40 / 1500 / dt); i_ref = int(160 / 1500 / dt)
+    w = int(0.012 / dt)
+    a_inc = sig[max(0, i_inc - w):i_inc + w].max()
+    a_ref = np.abs(sig[i_ref - w:i_ref + w]).max()
+    assert a_inc > 0.5
+    assert a_ref / a_inc < 0.02
+
+
+def test_cpml_fluid_zeta_profile_is_the_top_strip_formula():
+    """The fluid zeta profile of the CPML grid is the lo strip of the profile
+    formula (src/elements.cu:832-854, 870-873): equal to the left xi strip, zero
+    damping below it; the PANDF grid has none."""
+    m = modelgen.preset("cpml_fs_tiny")
+    o = pyoracle.Oracle(m)
+    af, bf, df = o.pml(2)
+    ax, bx, dx = o.pml(0)
+    npd = m.npd
+    for a, b in ((af, ax), (bf, bx), (df, dx)):
+        assert np.array_equal(a[3:3 + npd], b[3:3 + npd])
+    assert np.all(df[3 + npd:] == 0.0) and np.all(df[3:3 + npd] > 0)
+    _, _, d0 = pyoracle.Oracle(modelgen.preset("tiny_curvi")).pml(2)
+    assert not d0.any()
 
 
 def test_oracle_step_invariants():
