@@ -264,6 +264,17 @@ int msfdm_batch_set_graph(msfdm_batch *b, int enable);
 int msfdm_batch_time_run(msfdm_batch *b, int nsteps, float *ms);
 int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows);
 
+/* --- Record analysis ----------------------------------------------------- */
+
+/* Phase-shift dispersion image of a time-major gather [nt][ntr] (float32, as
+ * the record files hold it after the PML columns are cut) with signed offsets
+ * [ntr] (m), sampling rate fs (Hz) and test velocities [nc] (m/s), in FP64 on
+ * HIP device `device`: amp[nf][nc] with nf = nt / 2 + 1, and the frequencies
+ * [nf] in freqs (may be NULL).  Replaces dispersion_imaging of tools/MAS.py:50-98
+ * (tools/test.py:50-99). */
+int msfdm_dispersion_image(const float *gather, int nt, int ntr, const double *offsets, double fs,
+                           const double *velocities, int nc, int device, double *freqs, double *amp);
+
 /* --- Process boundary ---------------------------------------------------- */
 
 /* The whole bin/main_debug run: argument guards, output-dir reset, shot loop,

@@ -11,14 +11,22 @@
 // Snapshots are taken after steps with it % snapshot == 0, records after steps
 // with it % gap == 0 (src/main.cu:194-211).  Each shot starts from a zero
 // wavefield (the reference carries the previous shot's field over).
+//
+// Snapshot files are written by a writer thread while the GPU steps on to the
+// next snapshot (the reference synchronises and writes inline, src/main.cu:196-203);
+// at most two snapshots wait in host memory.
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <system_error>
+#include <thread>
 #include <vector>
 
 #include "model.hpp"
@@ -59,6 +67,66 @@ bool write_field(const fs::path &dir, const char *field, int it, const std::vect
     return true;
 }
 
+// Background writer of snapshot files: jobs are queued by the stepping thread
+// (blocking while two are pending) and written in order; the first error stops
+// the writer and is reported by push() / finish().
+class SnapshotWriter {
+public:
+    struct File {
+        fs::path dir;
+        const char *field;
+        int it;
+        std::vector<float> data;
+    };
+    SnapshotWriter() : worker_([this] { run(); }) {}
+    ~SnapshotWriter() { finish(); }
+    bool push(std::vector<File> job, std::string &err) {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return q_.size() < 2 || !err_.empty(); });
+        if (!err_.empty()) { err = err_; return false; }
+        q_.push_back(std::move(job));
+        cv_.notify_all();
+        return true;
+    }
+    bool finish(std::string *err = nullptr) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            done_ = true;
+        }
+        cv_.notify_all();
+        if (worker_.joinable()) worker_.join();
+        if (err && !err_.empty()) *err = err_;
+        return err_.empty();
+    }
+
+private:
+    void run() {
+        for (;;) {
+            std::vector<File> job;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return !q_.empty() || done_; });
+                if (q_.empty()) return;
+                job = std::move(q_.front());
+            }
+            std::string e;
+            for (auto &f : job)
+                if (!write_field(f.dir, f.field, f.it, f.data, e)) break;
+            std::lock_guard<std::mutex> lk(m_);
+            q_.pop_front();
+            if (!e.empty() && err_.empty()) err_ = e;
+            cv_.notify_all();
+            if (!err_.empty()) return;
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::vector<File>> q_;
+    std::string err_;
+    bool done_ = false;
+    std::thread worker_;
+};
+
 }  // namespace
 
 extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
@@ -89,7 +157,7 @@ extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
     const int nx = d.nx;
     const bool fluid = d.calc_type == 1;
     const size_t nf = fluid ? (size_t)d.nz_fluid * nx : 0, ns = (size_t)d.nz_solid * nx;
-    std::vector<float> svz(ns), svx(ns), fvz(nf), fvx(nf);
+    SnapshotWriter writer;
     for (int isrc = 0; isrc < d.nsrc; ++isrc) {
         const fs::path shot = out_root / std::to_string(isrc);
         if (!make_dirs(shot / "snapshot", err) || !make_dirs(shot / "record", err)) return fail(err);
@@ -113,17 +181,21 @@ extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
             if (snap < d.nt) {
                 std::printf("finished %0.2f%%\r", 100.0 * snap / d.nt);
                 std::fflush(stdout);
-                if (msfdm_get_field(solver.get(), MSFDM_SVZ, svz.data()) || msfdm_get_field(solver.get(), MSFDM_SVX, svx.data()))
-                    return 1;
-                if (fluid && (msfdm_get_field(solver.get(), MSFDM_FVZ, fvz.data()) ||
-                              msfdm_get_field(solver.get(), MSFDM_FVX, fvx.data())))
-                    return 1;
                 const fs::path sd = shot / "snapshot";
-                if (!write_field(sd / "solid" / "vz", "vz", snap, svz, err) || !write_field(sd / "solid" / "vx", "vx", snap, svx, err))
-                    return fail(err);
-                if (fluid && (!write_field(sd / "fluid" / "vz", "vz", snap, fvz, err) ||
-                              !write_field(sd / "fluid" / "vx", "vx", snap, fvx, err)))
-                    return fail(err);
+                std::vector<SnapshotWriter::File> job;
+                job.push_back({sd / "solid" / "vz", "vz", snap, std::vector<float>(ns)});
+                job.push_back({sd / "solid" / "vx", "vx", snap, std::vector<float>(ns)});
+                if (msfdm_get_field(solver.get(), MSFDM_SVZ, job[0].data.data()) ||
+                    msfdm_get_field(solver.get(), MSFDM_SVX, job[1].data.data()))
+                    return 1;
+                if (fluid) {
+                    job.push_back({sd / "fluid" / "vz", "vz", snap, std::vector<float>(nf)});
+                    job.push_back({sd / "fluid" / "vx", "vx", snap, std::vector<float>(nf)});
+                    if (msfdm_get_field(solver.get(), MSFDM_FVZ, job[2].data.data()) ||
+                        msfdm_get_field(solver.get(), MSFDM_FVX, job[3].data.data()))
+                        return 1;
+                }
+                if (!writer.push(std::move(job), err)) return fail(err);
             }
         }
         if (msfdm_synchronize(solver.get()) != MSFDM_OK) return 1;
@@ -147,6 +219,7 @@ extern "C" int msfdm_run_cli(const char *json_path, const char *out_root_c) {
             if ((fluid && !rvx) || !rvz) return fail("failed to write solid record data");
         }
     }
+    if (!writer.finish(&err)) return fail(err);
     std::printf("\n");
     return 0;
 }

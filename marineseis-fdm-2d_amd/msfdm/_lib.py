@@ -47,6 +47,7 @@ EXPORTS = [
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
     "msfdm_update_media", "msfdm_grid_metrics", "msfdm_gradient_shots", "msfdm_gradient_shots_plan",
+    "msfdm_dispersion_image",
 ]
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
 
@@ -129,6 +130,8 @@ def lib():
         "msfdm_grid_metrics": ([C.POINTER(C.c_double)] * 2 + [i, i, i] + [C.POINTER(C.c_double)] * 5, i),
         "msfdm_gradient_shots": ([vp, i, ip, i, i, C.c_double, fp] + [C.POINTER(C.c_double)] * 7 + [ip], i),
         "msfdm_gradient_shots_plan": ([vp, i, i, C.POINTER(C.c_double), ip], i),
+        "msfdm_dispersion_image": ([fp, i, i, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double), i, i,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)], i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
