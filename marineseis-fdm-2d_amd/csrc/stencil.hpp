@@ -490,9 +490,12 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             rvx = -(xx * dpx + zx * dpz);
             rvz = -(xz * dpx + zz * dpz);
             const float gxize = xx * zx + xz * zz;
-            const float t = (gxixi * dpx + gxize * dpz) / gxixi;
-            Dx = -xx * t;
-            Dz = -xz * t;
+            Dx = Dz = 0.0f;
+            if (scol >= 0) {   // xi-PML source: only the side-strip cells pay for the division
+                const float t = (gxixi * dpx + gxize * dpz) / gxixi;
+                Dx = -xx * t;
+                Dz = -xz * t;
+            }
             if (top) {   // include/cpml_kernels.cuh:304-305
                 const float gzeze = zx * zx + zz * zz;
                 const float tz = (gxize * dpx + gzeze * dpz) / gzeze;
@@ -761,11 +764,18 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             const float e = dx_lds<DIR>(ssxz[r], cc), f = dz_lds<DIR>(ssxz, r, cc);
             rvx = xx * a + zx * b + xz * e + zz * f;
             rvz = xx * e + zx * f + xz * cz + zz * d;
-            const float gx = gxize / gxixi, gz = gxize / gzeze;
-            Dxx = xx * (a + gx * b) + xz * (e + gx * f);
-            Dxz = xx * (e + gx * f) + xz * (cz + gx * d);
-            Dzx = zx * (b + gz * a) + zz * (gz * e + f);
-            Dzz = zx * (gz * e + f) + zz * (gz * cz + d);
+            // PML sources: only the strip cells pay for the metric divisions
+            Dxx = Dxz = Dzx = Dzz = 0.0f;
+            if (side) {
+                const float gx = gxize / gxixi;
+                Dxx = xx * (a + gx * b) + xz * (e + gx * f);
+                Dxz = xx * (e + gx * f) + xz * (cz + gx * d);
+            }
+            if (bottom || top) {
+                const float gz = gxize / gzeze;
+                Dzx = zx * (b + gz * a) + zz * (gz * e + f);
+                Dzz = zx * (gz * e + f) + zz * (gz * cz + d);
+            }
         }
         const float rinv = cur.cf[4];
         float kvx = rvx * rinv, kvz = rvz * rinv;

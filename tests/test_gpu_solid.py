@@ -194,3 +194,29 @@ def test_solid_cli_layout_and_records(tmp_path):
         (vz,) = s.records()
         got = np.fromfile(rec / "solid_vz.bin", dtype="<f4").reshape(-1, m.nx)
         assert np.array_equal(got, vz)
+
+
+@pytest.mark.parametrize("kind,row", [("solid_cpml_tiny", 30), ("solid_diri_tiny", None)])
+def test_solid_cli_record_rows(kind, row, tmp_path):
+    """CLI records of the other solid-only kinds (src/main.cu:120-128, 199-209, 303-311):
+    vz of row 30 with PML on four sides; an empty solid_vz.bin for Dirichlet edges."""
+    m = modelgen.preset(kind, nt=60, snapshot=1000)
+    j = modelgen.write_model(m, str(tmp_path / "model"), kind, relative_to=str(tmp_path))
+    r = subprocess.run([msfdm._lib.BIN_PATH, os.path.relpath(j, tmp_path), "out"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = tmp_path / "out" / "0" / "record"
+    assert sorted(os.listdir(rec)) == ["solid_vz.bin"]
+    got = np.fromfile(rec / "solid_vz.bin", dtype="<f4")
+    if row is None:
+        assert got.size == 0
+        return
+    s = msfdm.Solver(m)
+    s.reset(0)
+    s.run(60)
+    (vz,) = s.records()
+    assert np.array_equal(got.reshape(-1, m.nx), vz)
+    o = pyoracle.Oracle(m)
+    o.reset(0)
+    o.run(60)
+    assert rel_l2(vz, o.records()[0]) <= RECORD_TOL
