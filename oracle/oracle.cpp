@@ -7,10 +7,13 @@
 // it.
 //
 // It restates, in double precision and with race-free (sequential) semantics,
-// the reference's PANDF steppers: the fluid-solid `PmlFs::FluidSolid::update_RK4`
-// and the solid-only `PmlFs::SolidOnly::update_RK4` with its traction-free top
+// the reference's steppers: the fluid-solid `PmlFs::FluidSolid::update_RK4`; the
+// solid-only `PmlFs::SolidOnly::update_RK4` with its traction-free top
 // (include/pml_fs_kernels.cuh:8-307, src/pml_fs_kernels.cu:1-381,
-// src/elements.cu:907-977).  Fluid-solid:
+// src/elements.cu:907-977); the CPML steppers `Cpml::SolidOnly` and
+// `Cpml::FluidSolid` (include/cpml_kernels.cuh, src/cpml_kernels.cu) and the
+// solid Dirichlet stepper (include/dirichlet_kernels.cuh, src/dirichlet_kernels.cu).
+// Fluid-solid:
 //   include/pml_fs_kernels.cuh:331-827   (RHS kernels, interface BC)
 //   src/pml_fs_kernels.cu:383-868        (ghost fluxes, filter, source, RK4)
 //   include/tools_func.cuh:6-188, src/tools_func.cu:1-66 (operators, fluxes)
@@ -35,7 +38,11 @@
 // coefficients at the seafloor and at the pressure-release surface.  Solid
 // only (tests/test_oracle_solid.py): the flat-grid ALM closed form, zero
 // traction of the free-surface stress rates on curvilinear grids, the odd-image
-// ghost rows, and the Rayleigh-wave speed along the free surface.
+// ghost rows, and the Rayleigh-wave speed along the free surface.  CPML and
+// Dirichlet (tests/test_oracle_solid_bc.py, tests/test_oracle_pins.py): the
+// curvilinear interior operator against analytic plane-wave derivatives, the
+// top/bottom profile mirror, CPML absorption against the Dirichlet box, and the
+// fluid top PML absorbing the upgoing pulse the pressure-release surface returns.
 
 #include <algorithm>
 #include <cmath>
