@@ -192,7 +192,8 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     }
     const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
     const int ntiles = nbx * (nbf + nbs);
-    const int grid = 8 * ((ntiles + 7) / 8);
+    // each XCD: one eighth of the fluid tiles, then one eighth of the solid tiles (D::xcd_tile)
+    const int grid = 8 * ((nbx * nbf + 7) / 8 + (nbx * nbs + 7) / 8);
     if (s->geo.fcpml)   // top PML in the water; metric form decided at run time
         D::stage_kernel<DIR, STAGE, TZ, 2, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
                                                                             s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);

@@ -881,14 +881,28 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 }
 
 // One RK stage over both media in a single launch.  The grid is 1-D with a
-// multiple of 8 blocks; consecutive blocks go round-robin to the 8 XCDs, so
-// block b takes logical tile (b % 8) * (grid / 8) + b / 8: each XCD works on a
-// contiguous run of tiles (row bands of 64-column tiles) and finds the halo
-// rows of its neighbours in its own L2.  Tiles: fluid bands first, then solid.
+// multiple of 8 blocks; consecutive blocks go round-robin to the 8 XCDs.  Each
+// XCD takes one eighth of the fluid tiles and one eighth of the solid tiles,
+// each as a contiguous run (row bands of 64-column tiles), so every XCD finds
+// the halo rows of its neighbours in its own L2 and gets the same mix of the
+// cheaper fluid and the dearer solid tiles: block b is the (b / 8)-th tile of
+// XCD b % 8, fluid run first (xcd_tile).
 #ifndef MSFDM_STAGE_WAVES
 #define MSFDM_STAGE_WAVES 0   // minimum waves per SIMD requested for stage_kernel (0 = compiler choice)
 #endif
 // CPML = 1: the fluid-solid CPML grid (top PML in the water, Cpml::FluidSolid).
+// Tile of block b when each of the 8 XCDs takes a run of cf fluid tiles then a
+// run of cs solid tiles (nf fluid tiles in total, ntiles in all); -1 past the end.
+__device__ __forceinline__ int xcd_tile(int b, int nf, int ntiles, int cf, int cs) {
+    const int x = b & 7, l = b >> 3;
+    if (l < cf) {
+        const int t = x * cf + l;
+        return t < nf ? t : -1;
+    }
+    const int t = nf + x * cs + (l - cf);
+    return t < ntiles ? t : -1;
+}
+
 template <int DIR, int STAGE, int TZ, int MET, int CPML = 0>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
@@ -896,9 +910,10 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
              float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
              int ntiles) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
-    const int per_xcd = gridDim.x >> 3;
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= ntiles) return;
+    const int nf = nbx * nbz_f;
+    const int cf = (nf + 7) >> 3, cs = (ntiles - nf + 7) >> 3;   // per-XCD runs (the grid is 8 * (cf + cs))
+    const int tile = xcd_tile(blockIdx.x, nf, ntiles, cf, cs);
+    if (tile < 0) return;
     const int band = tile / nbx, bx = tile - band * nbx;
     if (band < nbz_f)
         fluid_body<DIR, STAGE, TZ, MET, CPML>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx,
