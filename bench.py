@@ -254,8 +254,9 @@ def bench_solid(args):
     t_stage4 = s.time_kernel(1, reps)
     b_stage = s.kernel_bytes(0)
     achieved = b_stage / (t_stage * 1e-3) / 1e9
+    # template arguments: direction, stage, tile rows, metric form, top kind (1 = free surface)
     kname, traffic = measured_traffic(args.solid_workload,
-                                      [f"free_stage_kernel<0, 2, {s.tile_rows}, {m}>" for m in (1, 0)])
+                                      [f"free_stage_kernel<0, 2, {s.tile_rows}, {m}, 1>" for m in (1, 0)])
     cpu = cpu_baseline(model, args.cpu_steps) if args.cpu_steps > 0 else None
     out = {
         "metric": "Gcell-updates/s (plain FP32 solid-only solver, traction-free surface, 1 GPU)",
@@ -398,8 +399,9 @@ def main():
     b_stage = s.kernel_bytes(0)
     achieved = b_stage / (t_stage * 1e-3) / 1e9
 
-    # the stage kernel is specialised for vertical grid columns (last template argument 1)
-    kname, traffic = measured_traffic(args.workload, [f"stage_kernel<0, 2, {s.tile_rows}, {m}>" for m in (1, 0)])
+    # template arguments: direction, stage, tile rows, metric form (1 = vertical grid
+    # columns), CPML top (0)
+    kname, traffic = measured_traffic(args.workload, [f"stage_kernel<0, 2, {s.tile_rows}, {m}, 0>" for m in (1, 0)])
 
     cpu = None
     if args.cpu_steps > 0:
