@@ -169,7 +169,7 @@ int grow_tables(msfdm_solver *s, int steps_needed) {
 // ---------------------------------------------------------------------------
 // Launches
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ>
+template <int DIR, int STAGE, int TZ, int FIN = 0>
 void launch_media(msfdm_solver *s, const float *S, const float *U, const float *T2, const float *T3, float *O, int j,
                   float ct, float amp, float cprev) {
     const int nbx = (s->d.nx - 4) / D::TX + 1;
@@ -195,25 +195,29 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     // each XCD: one eighth of the fluid tiles, then one eighth of the solid tiles (D::xcd_tile)
     const int grid = 8 * ((nbx * nbf + 7) / 8 + (nbx * nbs + 7) / 8);
     if (s->geo.fcpml)   // top PML in the water; metric form decided at run time
-        D::stage_kernel<DIR, STAGE, TZ, 2, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
-                                                                            s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+        D::stage_kernel<DIR, STAGE, TZ, 2, 1, FIN><<<grid, D::NTHR, 0, s->stream>>>(
+            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
     else if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
-        D::stage_kernel<DIR, STAGE, TZ, 1><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
-                                                                             s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+        D::stage_kernel<DIR, STAGE, TZ, 1, 0, FIN><<<grid, D::NTHR, 0, s->stream>>>(
+            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
     else
-        D::stage_kernel<DIR, STAGE, TZ, 0><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, T3, O,
-                                                                             s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+        D::stage_kernel<DIR, STAGE, TZ, 0, 0, FIN><<<grid, D::NTHR, 0, s->stream>>>(
+            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
 }
 // One RK stage; cprev is the previous stage's weight (its interface rows are
 // rebuilt by this stage's tiles).  Stage 4 adds the interface / filter / record
-// kernel, the raw-k mode (STAGE 0) the interface condition on the raw k.
-template <int DIR, int STAGE>
-void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, float amp, float cprev) {
+// kernel unless the next step's stage 1 does that work (fin_next), the raw-k
+// mode (STAGE 0) the interface condition on the raw k.  FIN: a stage 1 that
+// finishes the previous step (cprev = dt).
+template <int DIR, int STAGE, int FIN = 0>
+void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, float amp, float cprev,
+                  bool fin_next = false) {
     const float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3];
-    if (s->tz == 8) launch_media<DIR, STAGE, 8>(s, S, U, T2, T3, O, j, ct, amp, cprev);
-    else launch_media<DIR, STAGE, 16>(s, S, U, T2, T3, O, j, ct, amp, cprev);
+    if (s->tz == 8) launch_media<DIR, STAGE, 8, FIN>(s, S, U, T2, T3, O, j, ct, amp, cprev);
+    else launch_media<DIR, STAGE, 16, FIN>(s, S, U, T2, T3, O, j, ct, amp, cprev);
     if (s->solid_only) return;   // no interface: the stage kernel writes every row and the record
     if constexpr (STAGE == 4) {
+        if (fin_next) return;
         const float dt = s->d.dt;
         D::iface_final<DIR><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, O, s->kif,
                                                                                     dt / 2, dt, ct, j);
@@ -223,21 +227,26 @@ void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, fl
 }
 
 // One time step (update_RK4 FBFB / BFBF, src/pml_fs_kernels.cu:771-844) at
-// offset j of the current batch; it is the absolute step index.
+// offset j of the current batch; it is the absolute step index.  Within one
+// launch sequence the interface rows of a step are finished by the next step's
+// stage-1 launch (first: no step before it in the sequence; last: none after).
 template <int D1>
-void enqueue_step_dir(msfdm_solver *s, int j) {
+void enqueue_step_dir(msfdm_solver *s, int j, bool first, bool last) {
     constexpr int D2 = 1 - D1;
     const float dt = s->d.dt;
     float *U = s->buf[R_U], *T2 = s->buf[R_T2], *T3 = s->buf[R_T3], *T4 = s->buf[R_T4];
-    launch_stage<D1, 1>(s, U, T2, j, dt / 2, 0.f, 0.f);
+    if (first || s->solid_only) launch_stage<D1, 1>(s, U, T2, j, dt / 2, 0.f, 0.f);
+    else launch_stage<D1, 1, 1>(s, U, T2, j, dt / 2, 0.f, dt);
     launch_stage<D2, 2>(s, T2, T3, j, dt / 2, 0.f, dt / 2);
     launch_stage<D1, 3>(s, T3, T4, j, dt, 0.f, dt / 2);
-    launch_stage<D2, 4>(s, T4, U, j, dt / 6, 0.f, dt);
+    launch_stage<D2, 4>(s, T4, U, j, dt / 6, 0.f, dt, !last);
 }
+// n steps from absolute step it0; U, the records and every interface row are
+// complete when the sequence ends (nothing reads them in between).
 void enqueue_steps(msfdm_solver *s, int it0, int n) {
     for (int j = 0; j < n; ++j) {
-        if ((it0 + j) & 1) enqueue_step_dir<1>(s, j);
-        else enqueue_step_dir<0>(s, j);
+        if ((it0 + j) & 1) enqueue_step_dir<1>(s, j, j == 0, j == n - 1);
+        else enqueue_step_dir<0>(s, j, j == 0, j == n - 1);
     }
     D::advance_base<<<1, 64, 0, s->stream>>>(s->base, n);
 }
@@ -876,35 +885,39 @@ BatchView view_of(const msfdm_batch *b) {
                      (int)b->shots.size(), b->maxnx, b->base, b->dt, b->stream};
 }
 
-template <int DIR, int STAGE, int TZ>
+template <int DIR, int STAGE, int TZ, int FIN>
 void batch_launch_media(const BatchView &b, int rS, int rO, int j, float ct, float cprev) {
     const int k = TZ == 8 ? 0 : 1;
     const int grid = 8 * ((b.ntiles[k] + 7) / 8);
-    D::batch_stage_kernel<DIR, STAGE, TZ><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k], rS, rO, j,
-                                                                           ct, cprev);
+    D::batch_stage_kernel<DIR, STAGE, TZ, FIN><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k], rS, rO,
+                                                                                j, ct, cprev);
 }
-template <int DIR, int STAGE>
-void batch_launch_stage(const BatchView &b, int rS, int rO, int j, float ct, float cprev) {
-    if (b.tz == 8) batch_launch_media<DIR, STAGE, 8>(b, rS, rO, j, ct, cprev);
-    else batch_launch_media<DIR, STAGE, 16>(b, rS, rO, j, ct, cprev);
+// As launch_stage: FIN, a stage 1 that finishes the previous step (cprev = dt);
+// fin_next, stage 4 leaves its interface rows to the next step's stage 1.
+template <int DIR, int STAGE, int FIN = 0>
+void batch_launch_stage(const BatchView &b, int rS, int rO, int j, float ct, float cprev, bool fin_next = false) {
+    if (b.tz == 8) batch_launch_media<DIR, STAGE, 8, FIN>(b, rS, rO, j, ct, cprev);
+    else batch_launch_media<DIR, STAGE, 16, FIN>(b, rS, rO, j, ct, cprev);
     if constexpr (STAGE == 4) {
+        if (fin_next) return;
         D::batch_iface_final<DIR><<<dim3((b.maxnx + D::FO - 1) / D::FO, b.ns), D::FB, 0, b.stream>>>(b.shots, b.dt / 2,
                                                                                                    b.dt, ct, j);
     }
 }
 template <int D1>
-void batch_step_dir(const BatchView &b, int j) {
+void batch_step_dir(const BatchView &b, int j, bool first = true, bool last = true) {
     constexpr int D2 = 1 - D1;
     const float dt = b.dt;
-    batch_launch_stage<D1, 1>(b, R_U, R_T2, j, dt / 2, 0.f);
+    if (first) batch_launch_stage<D1, 1>(b, R_U, R_T2, j, dt / 2, 0.f);
+    else batch_launch_stage<D1, 1, 1>(b, R_U, R_T2, j, dt / 2, dt);
     batch_launch_stage<D2, 2>(b, R_T2, R_T3, j, dt / 2, dt / 2);
     batch_launch_stage<D1, 3>(b, R_T3, R_T4, j, dt, dt / 2);
-    batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6, dt);
+    batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6, dt, !last);
 }
 void batch_enqueue(const BatchView &b, int it0, int n) {
     for (int j = 0; j < n; ++j) {
-        if ((it0 + j) & 1) batch_step_dir<1>(b, j);
-        else batch_step_dir<0>(b, j);
+        if ((it0 + j) & 1) batch_step_dir<1>(b, j, j == 0, j == n - 1);
+        else batch_step_dir<0>(b, j, j == 0, j == n - 1);
     }
     D::advance_base<<<1, 64, 0, b.stream>>>(b.base, n);
 }

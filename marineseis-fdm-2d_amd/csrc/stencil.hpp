@@ -1,6 +1,6 @@
 // Device code of the fluid-solid RK4 stepper for gfx950.
 //
-// One RK stage = one launch (stage 4: two):
+// One RK stage = one launch:
 //   stage_kernel<DIR,STAGE,TZ,MET> runs, per 64 x TZ tile,
 //   fluid_body                 fluid RHS (update_velocity_fluid + update_stress_fluid,
 //                              include/pml_fs_kernels.cuh:333-507) + xi-PML memory
@@ -12,9 +12,13 @@
 //                              + the previous stage's interface rows, rebuilt from
 //                              its raw interface k with fl_so_boundary
 //                              (include/pml_fs_kernels.cuh:754-827)
-//   iface_final<DIR>           stage 4: the interface rows of U' (fl_so_boundary of
-//                              all four stages), fl_so_filter (src/pml_fs_kernels.cu:428-525,
-//                              out of place) and the receiver-row record.
+//   iface_final<DIR>           after the last step of a launch sequence: the interface
+//                              rows of U' (fl_so_boundary of all four stages),
+//                              fl_so_filter (src/pml_fs_kernels.cu:428-525, out of
+//                              place) and the receiver-row record.  Inside a sequence
+//                              the next step's stage-1 launch (FIN) does this work in
+//                              the tiles that hold an interface row, so a step costs
+//                              four launches.
 //   free_stage_kernel<DIR,STAGE,TZ,MET,TOP>
 //                              solid-only grids, one launch per stage: TOP_FREE, the
 //                              traction-free top (PmlFs::SolidOnly,
@@ -298,8 +302,12 @@ __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const
 
 // fl_so_boundary<DIR> on the eight interface k values (include/pml_fs_kernels.cuh:754-827).
 // The sxz reconstruction keeps the reference's expression (no nx*nz*snn term).
+// The interface arithmetic (interface_bc, iface_pre, iface_filter) is compiled
+// without multiply-add contraction: the kernels that inline it then round it
+// alike, so iface_final and a FIN stage-1 tile give the same bits.
 template <int DIR>
 __device__ __forceinline__ void interface_bc(float k[8], float2 n) {
+#pragma clang fp contract(off)
     const float n1 = n.x, n2 = n.y;
     const float vxf = k[0], vzf = k[1];
     float pf = k[2];
@@ -328,13 +336,148 @@ __device__ __forceinline__ long long iface_index(const Geo &g, int f, int ix) {
     return f < 3 ? g.off[MSFDM_FVX + f] + (long long)(g.nzf - 1) * g.P + ix : g.off[MSFDM_SVX + f - 3] + ix;
 }
 
-// Interface side buffer kif ([KIF_ROWS][P]).  Rows 8(s-1) .. 8(s-1)+7 hold the
-// raw interface-row k of RK stage s (the raw-k mode, stage 0, uses rows 0..7);
-// rows KIF_U .. KIF_U+7 hold U^n on the interface rows, saved by stage 1.  Row f
-// of a set: 0..2 = fluid vx, vz, p on fluid row nzf-1; 3..7 = solid vx, vz, sxx,
-// szz, sxz on solid row 0.
-constexpr int KIF_U = 32, KIF_ROWS = 40;
-__device__ __forceinline__ constexpr int kif_set(int stage) { return stage <= 1 ? 0 : 8 * (stage - 1); }
+// Interface side buffer kif ([KIF_ROWS][P]), sets of eight rows.  Row f of a
+// set: 0..2 = fluid vx, vz, p on fluid row nzf-1; 3..7 = solid vx, vz, sxx, szz,
+// sxz on solid row 0.  Sets: the raw interface-row k of RK stages 2, 3, 4 at
+// rows 8, 16, 24; the stage-1 k and U^n (saved by stage 1) twice, by step parity
+// par = the direction of the step's stages 1 and 3: k1 at rows 0 (par 0) / 40
+// (par 1), U^n at KIF_U (par 0) / KIF_U + 16 (par 1).  A stage-1 launch that
+// also finishes the previous step's interface rows reads that step's sets while
+// it writes its own (raw-k mode, stage 0: rows 0..7).
+constexpr int KIF_U = 32, KIF_ROWS = 56;
+__device__ __forceinline__ constexpr int kif_set(int stage, int par) {
+    return stage == 0 ? 0 : stage == 1 ? (par ? 40 : 0) : 8 * (stage - 1);
+}
+__device__ __forceinline__ constexpr int kif_u(int par) { return par ? KIF_U + 16 : KIF_U; }
+// Step parity of RK stage STAGE (1..4) run in direction DIR (stages go d1 d2 d1 d2).
+template <int DIR, int STAGE>
+__device__ __forceinline__ constexpr int step_par() { return (STAGE & 1) ? DIR : 1 - DIR; }
+
+// Stage-4 interface rows + fl_so_filter (out of place) + receiver records.
+// The interface rows of T2, T3, T4 and the stage-4 k are rebuilt from the four
+// raw k sets and U^n (the interface condition of each stage's direction: the
+// stages run d1 d2 d1 d2 with d2 = DIR), with the same expressions the stage
+// kernels use: T2 = U + dt/2 B(k1), T3 = U + dt/2 B(k2), T4 = U + dt B(k3),
+// U' = U + ((T2 - U) + 2 (T3 - U) + (T4 - U)) / 3 + dt/6 B(k4).
+// iface_pre gives U' before the filter at one inner column, iface_filter the
+// filtered, re-projected U' from the 7 columns around it, iface_record the
+// receiver record of step it.  Two callers run them: iface_final (after the last
+// step of a launch sequence) and a stage-1 tile of the next step (FIN), with the
+// same arithmetic, so both give the same bits.
+template <int DIR>
+__device__ __forceinline__ void iface_pre(const Geo &g, const Coef &c, const float *__restrict__ kif, float c12,
+                                          float c3, float cb, int ix, float v[8]) {
+#pragma clang fp contract(off)
+    constexpr int par = 1 - DIR;   // the step's stage 1 ran in direction 1 - DIR
+    const float2 n = c.nif[ix];
+    const long long P = g.P;
+    float k1[8], k2[8], k3[8], k4[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+        k1[f] = kif[(long long)(kif_set(1, par) + f) * P + ix];
+        k2[f] = kif[(long long)(kif_set(2, par) + f) * P + ix];
+        k3[f] = kif[(long long)(kif_set(3, par) + f) * P + ix];
+        k4[f] = kif[(long long)(kif_set(4, par) + f) * P + ix];
+    }
+    interface_bc<1 - DIR>(k1, n);
+    interface_bc<DIR>(k2, n);
+    interface_bc<1 - DIR>(k3, n);
+    interface_bc<DIR>(k4, n);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+        const float u = kif[(long long)(kif_u(par) + f) * P + ix];   // U^n saved in stage 1
+        const float t2 = u + c12 * k1[f], t3 = u + c12 * k2[f], t4 = u + c3 * k3[f];
+        v[f] = u + ((t2 - u) + 2.0f * (t3 - u) + (t4 - u)) * (1.0f / 3.0f) + cb * k4[f];
+    }
+}
+
+// pre[f * pitch + t + kk], kk = -3..3: U' before the filter around inner column
+// ix (0 at columns that are not inner).
+__device__ __forceinline__ void iface_filter(const float *pre, int pitch, int t, int ix, int nx, float2 n,
+                                             float o[8]) {
+#pragma clang fp contract(off)
+    const int klo = max(-3, 3 - ix), khi = min(3, nx - 4 - ix);
+    float wsum = 0.0f;
+    for (int kk = klo; kk <= khi; ++kk) wsum += FILT_W[kk + 3];
+    float v[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+        float s = 0.0f;
+        for (int kk = klo; kk <= khi; ++kk) s += FILT_W[kk + 3] * pre[f * pitch + t + kk];
+        v[f] = s / wsum;
+    }
+    const float n1 = n.x, n2 = n.y;
+    const float vnf = n1 * v[0] + n2 * v[1], vtf = n2 * v[0] - n1 * v[1];
+    const float vns = n1 * v[3] + n2 * v[4], vts = n2 * v[3] - n1 * v[4];
+    const float snn = n1 * n1 * v[5] + 2.0f * n1 * n2 * v[7] + n2 * n2 * v[6];
+    const float stt = n2 * n2 * v[5] - 2.0f * n1 * n2 * v[7] + n1 * n1 * v[6];
+    const float vn = 0.5f * (vnf + vns), sn = 0.5f * (snn - v[2]);
+    o[0] = n1 * vn + n2 * vtf;
+    o[1] = n2 * vn - n1 * vtf;
+    o[2] = -sn;
+    o[3] = n1 * vn + n2 * vts;
+    o[4] = n2 * vn - n1 * vts;
+    o[5] = n1 * n1 * sn + n2 * n2 * stt;
+    o[6] = n2 * n2 * sn + n1 * n1 * stt;
+    o[7] = n1 * n2 * (sn - stt);
+}
+
+// Receiver record of step it at column ix: solid vx, vz and fluid p of the
+// interface rows after the step (src/main.cu:297-322).
+__device__ __forceinline__ void iface_record(const Ctl &ctl, int it, int nx, int ix, const float o[8]) {
+    if (it % ctl.gap != 0) return;
+    const int slot = it / ctl.gap;
+    if (slot >= ctl.rec_cap) return;
+    const long long plane = (long long)ctl.rec_cap * nx;
+    float *rr = ctl.rec + (long long)slot * nx + ix;
+    rr[0] = o[3];
+    rr[plane] = o[4];
+    rr[2 * plane] = o[2];
+}
+
+// Stage 1 of a step that follows another step of the same launch sequence
+// (FIN): no iface_final ran after the previous step, so the interface rows of
+// U^n still lack its interface condition and filter.  Every tile that holds an
+// interface row computes them in LDS for its 70 loaded columns from the previous
+// step's side-buffer sets (iface_pre on the 76 columns x0-6 .. x0+69, then
+// iface_filter); the tile that owns the row (wb) stores its 64 columns of fields
+// f0 .. f0+NF-1 to U^n, and the solid one writes the previous step's record.
+constexpr int FW = TX + 4 * HALO;   // pre-filter columns of a tile
+template <int DIR, int NF, int TZ>
+__device__ __forceinline__ void finish_interface_row(float (*sh)[TZ + 2 * HALO][LXP], float (*pre)[FW], const Geo &g,
+                                                     const Coef &c, const Ctl &ctl, const float *__restrict__ S,
+                                                     const float *__restrict__ kif, int lrow, int f0, int x0, bool wb,
+                                                     float dt, int jstep) {
+    const int nx = g.nx;
+    const float c12 = dt * 0.5f, c3 = dt, cb = __fdiv_rn(dt, 6.0f);   // the host's dt / 2, dt, dt / 6
+    for (int t = threadIdx.x; t < FW; t += NTHR) {
+        const int ix = x0 - 2 * HALO + t;
+        float v[8];
+        if (ix >= 3 && ix <= nx - 4) {
+            iface_pre<DIR>(g, c, kif, c12, c3, cb, ix, v);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 8; ++f) v[f] = 0.0f;
+        }
+#pragma unroll
+        for (int f = 0; f < 8; ++f) pre[f][t] = v[f];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < LXW; t += NTHR) {
+        const int dx = t - HALO, ix = x0 + dx;
+        if (ix < 3 || ix > nx - 4) continue;
+        float o[8];
+        iface_filter(&pre[0][0], FW, t + HALO, ix, nx, c.nif[ix], o);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sh[f][lrow][XOFF + dx] = o[f0 + f];
+        if (wb && dx >= 0 && dx < TX) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) const_cast<float *>(S)[iface_index(g, f0 + f, ix)] = o[f0 + f];
+            if (f0 == 3) iface_record(ctl, ctl.base[0] + jstep - 1, nx, ix, o);
+        }
+    }
+}
+
 
 // The interface rows of stage state S (RK stage STAGE >= 2) are not written by
 // the previous stage's launch, whose k there still lacks the interface
@@ -351,7 +494,8 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
                                                       const float *__restrict__ S, const float *__restrict__ kif,
                                                       int lrow, int f0, int x0, float cprev, bool wb) {
     const long long P = g.P;
-    const float *kp = kif + (long long)kif_set(STAGE - 1) * P;
+    constexpr int par = step_par<DIR, STAGE>();
+    const float *kp = kif + (long long)kif_set(STAGE - 1, par) * P;
     for (int t = threadIdx.x; t < LXW; t += NTHR) {
         const int dx = t - HALO, ix = x0 + dx;
         if (ix < 3 || ix > g.nx - 4) continue;
@@ -361,7 +505,7 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
         interface_bc<1 - DIR>(k, c.nif[ix]);
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            const float v = kif[(KIF_U + f0 + f) * P + ix] + cprev * k[f0 + f];
+            const float v = kif[(kif_u(par) + f0 + f) * P + ix] + cprev * k[f0 + f];
             sh[f][lrow][XOFF + dx] = v;
             if (wb && dx >= 0 && dx < TX) const_cast<float *>(S)[iface_index(g, f0 + f, ix)] = v;
         }
@@ -375,12 +519,14 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
 // surface rows, the zeta operator closes only at the interface, and the top
 // strip carries the fluid zeta memory.
 // ---------------------------------------------------------------------------
-template <int DIR, int STAGE, int TZ, int MET, int FT = 0>
+// FIN (stage 1 only): also finish the previous step's interface rows
+// (finish_interface_row, LDS scratch fin, cprev = dt).
+template <int DIR, int STAGE, int TZ, int MET, int FT = 0, int FIN = 0>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
            const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
            const float *__restrict__ T3, float *__restrict__ O, float *__restrict__ kif, int jstep,
-           float ct, float amp_direct, float cprev, int bx, int bz) {
+           float ct, float amp_direct, float cprev, int bx, int bz, float (*fin)[FW] = nullptr) {
     // sh[0..2] = vx, vz, p of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     const int x0 = bx * TX, z0 = 3 + bz * TZ;
@@ -403,6 +549,12 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const int lrow = (nzf - 1) - (z0 - HALO);   // tile row of the interface (the tile starts inside the fluid)
         if (lrow < TZ + 2 * HALO) {
             rebuild_interface_row<DIR, STAGE, 3, TZ>(sh, g, c, S, kif, lrow, 0, x0, cprev, lrow - HALO < TZ);
+            __syncthreads();
+        }
+    } else if constexpr (STAGE == 1 && FIN) {
+        const int lrow = (nzf - 1) - (z0 - HALO);
+        if (lrow < TZ + 2 * HALO) {
+            finish_interface_row<DIR, 3, TZ>(sh, fin, g, c, ctl, S, kif, lrow, 0, x0, lrow - HALO < TZ, cprev, jstep);
             __syncthreads();
         }
     }
@@ -569,14 +721,15 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 kp += amp * SRC_W[szo + 2][sxo + 2];
         }
         if (dist_flso == 0) {
-            float *kk = kif + (long long)kif_set(STAGE) * P;
+            constexpr int par = STAGE == 0 ? 0 : step_par<DIR, STAGE>();
+            float *kk = kif + (long long)kif_set(STAGE, par) * P;
             kk[0 * (long long)P + ix] = kvx;
             kk[1 * (long long)P + ix] = kvz;
             kk[2 * (long long)P + ix] = kp;
             if constexpr (STAGE == 1) {   // S = U^n here
-                kif[(long long)(KIF_U + 0) * P + ix] = shvx[r][cc];
-                kif[(long long)(KIF_U + 1) * P + ix] = shvz[r][cc];
-                kif[(long long)(KIF_U + 2) * P + ix] = shp[r][cc];
+                kif[(long long)(kif_u(par) + 0) * P + ix] = shvx[r][cc];
+                kif[(long long)(kif_u(par) + 1) * P + ix] = shvz[r][cc];
+                kif[(long long)(kif_u(par) + 2) * P + ix] = shp[r][cc];
             }
             if constexpr (STAGE != 0) continue;
         }
@@ -599,12 +752,12 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 // TOP_IFACE_CPML: TOP_IFACE on the fluid-solid CPML grid, whose sigma_xz rate
 // reads its metrics at the solid-array row (include/cpml_kernels.cuh:447-452).
 constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2, TOP_IFACE_CPML = 3;
-template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE>
+template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE, int FIN = 0>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
            const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
            float *__restrict__ O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
-           int bz) {
+           int bz, float (*fin)[FW] = nullptr) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     constexpr bool SOLO = TOP == TOP_FREE || TOP == TOP_PLAIN;   // a solid-only grid
@@ -631,6 +784,11 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     if constexpr (STAGE >= 2 && !SOLO) {
         if (z0 == 0) {   // the band holding solid row 0 (tile row HALO)
             rebuild_interface_row<DIR, STAGE, 5, TZ>(sh, g, c, S, kif, HALO, 3, x0, cprev, true);
+            __syncthreads();
+        }
+    } else if constexpr (STAGE == 1 && FIN && !SOLO) {
+        if (z0 == 0) {
+            finish_interface_row<DIR, 5, TZ>(sh, fin, g, c, ctl, S, kif, HALO, 3, x0, true, cprev, jstep);
             __syncthreads();
         }
     }
@@ -848,18 +1006,19 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             }
         }
         if (!SOLO && iz == 0) {
-            float *kk = kif + (long long)kif_set(STAGE) * P;
+            constexpr int par = STAGE == 0 ? 0 : step_par<DIR, STAGE>();
+            float *kk = kif + (long long)kif_set(STAGE, par) * P;
             kk[3 * (long long)P + ix] = kvx;
             kk[4 * (long long)P + ix] = kvz;
             kk[5 * (long long)P + ix] = ksx;
             kk[6 * (long long)P + ix] = ksz;
             kk[7 * (long long)P + ix] = ksxz;
             if constexpr (STAGE == 1) {   // S = U^n here
-                kif[(long long)(KIF_U + 3) * P + ix] = svx[r][cc];
-                kif[(long long)(KIF_U + 4) * P + ix] = svz[r][cc];
-                kif[(long long)(KIF_U + 5) * P + ix] = ssx[r][cc];
-                kif[(long long)(KIF_U + 6) * P + ix] = ssz[r][cc];
-                kif[(long long)(KIF_U + 7) * P + ix] = ssxz[r][cc];
+                kif[(long long)(kif_u(par) + 3) * P + ix] = svx[r][cc];
+                kif[(long long)(kif_u(par) + 4) * P + ix] = svz[r][cc];
+                kif[(long long)(kif_u(par) + 5) * P + ix] = ssx[r][cc];
+                kif[(long long)(kif_u(par) + 6) * P + ix] = ssz[r][cc];
+                kif[(long long)(kif_u(par) + 7) * P + ix] = ssxz[r][cc];
             }
             if constexpr (STAGE != 0) continue;
         }
@@ -903,24 +1062,29 @@ __device__ __forceinline__ int xcd_tile(int b, int nf, int ntiles, int cf, int c
     return t < ntiles ? t : -1;
 }
 
-template <int DIR, int STAGE, int TZ, int MET, int CPML = 0>
+// FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
+// and record (finish_interface_row); cprev is then the time step.
+template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
              const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
              float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
              int ntiles) {
+    static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    __shared__ float fin[FIN ? 8 : 1][FW];
     const int nf = nbx * nbz_f;
     const int cf = (nf + 7) >> 3, cs = (ntiles - nf + 7) >> 3;   // per-XCD runs (the grid is 8 * (cf + cs))
     const int tile = xcd_tile(blockIdx.x, nf, ntiles, cf, cs);
     if (tile < 0) return;
     const int band = tile / nbx, bx = tile - band * nbx;
     if (band < nbz_f)
-        fluid_body<DIR, STAGE, TZ, MET, CPML>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev, bx,
-                                              band);
+        fluid_body<DIR, STAGE, TZ, MET, CPML, FIN>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev,
+                                                   bx, band, fin);
     else
-        solid_body<DIR, STAGE, TZ, MET, CPML ? TOP_IFACE_CPML : TOP_IFACE>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep,
-                                                                          ct, amp_direct, cprev, bx, band - nbz_f);
+        solid_body<DIR, STAGE, TZ, MET, CPML ? TOP_IFACE_CPML : TOP_IFACE, FIN>(sh, g, c, ctl, S, U, T2, T3, O, kif,
+                                                                               jstep, ct, amp_direct, cprev, bx,
+                                                                               band - nbz_f, fin);
 }
 
 // One RK stage of a solid-only grid (PmlFs::SolidOnly, src/pml_fs_kernels.cu:29-381;
@@ -961,12 +1125,8 @@ iface_rhs(Geo g, Coef c, float *__restrict__ O, const float *__restrict__ kif) {
     iface_rhs_body<DIR>(g, c, O, kif, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Stage-4 interface rows + fl_so_filter (out of place) + receiver records.
-// The interface rows of T2, T3, T4 and the stage-4 k are rebuilt from the four
-// raw k sets and U^n (the interface condition of each stage's direction: the
-// stages run d1 d2 d1 d2 with d2 = DIR), with the same expressions the stage
-// kernels use: T2 = U + dt/2 B(k1), T3 = U + dt/2 B(k2), T4 = U + dt B(k3),
-// U' = U + ((T2 - U) + 2 (T3 - U) + (T4 - U)) / 3 + dt/6 B(k4).
+// iface_final: the interface rows of U' and the record after the last step of a
+// launch sequence (within a sequence the next step's stage-1 tiles do it, FIN).
 // A block of FB threads loads FB consecutive interface columns starting at
 // x0 - 3 (one per thread, all loads issued at once) and filters the FO = FB - 6
 // columns x0 .. x0 + FO - 1 that have their full 7-tap neighbourhood in LDS.
@@ -981,31 +1141,10 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB], const Geo &g,
     const int t = threadIdx.x;
     const int ix = x0 - HALO + t;
     const bool inner = ix >= 3 && ix <= nx - 4;
-    const int it = ctl.base[0] + jstep;
-    float2 n = make_float2(0.f, 0.f);
     {
         float v[8];
         if (inner) {
-            n = c.nif[ix];
-            const long long P = g.P;
-            float k1[8], k2[8], k3[8], k4[8];
-#pragma unroll
-            for (int f = 0; f < 8; ++f) {
-                k1[f] = kif[(long long)(kif_set(1) + f) * P + ix];
-                k2[f] = kif[(long long)(kif_set(2) + f) * P + ix];
-                k3[f] = kif[(long long)(kif_set(3) + f) * P + ix];
-                k4[f] = kif[(long long)(kif_set(4) + f) * P + ix];
-            }
-            interface_bc<1 - DIR>(k1, n);
-            interface_bc<DIR>(k2, n);
-            interface_bc<1 - DIR>(k3, n);
-            interface_bc<DIR>(k4, n);
-#pragma unroll
-            for (int f = 0; f < 8; ++f) {
-                const float u = kif[(long long)(KIF_U + f) * P + ix];   // U^n saved in stage 1
-                const float t2 = u + c12 * k1[f], t3 = u + c12 * k2[f], t4 = u + c3 * k3[f];
-                v[f] = u + ((t2 - u) + 2.0f * (t3 - u) + (t4 - u)) * (1.0f / 3.0f) + cb * k4[f];
-            }
+            iface_pre<DIR>(g, c, kif, c12, c3, cb, ix, v);
         } else {
 #pragma unroll
             for (int f = 0; f < 8; ++f) v[f] = 0.0f;
@@ -1015,43 +1154,11 @@ __device__ __forceinline__ void iface_final_body(float (*pre)[FB], const Geo &g,
     }
     __syncthreads();
     if (!inner || t < HALO || t >= FB - HALO) return;
-    const int klo = max(-3, 3 - ix), khi = min(3, nx - 4 - ix);
-    float wsum = 0.0f;
-    for (int kk = klo; kk <= khi; ++kk) wsum += FILT_W[kk + 3];
-    float v[8];
-#pragma unroll
-    for (int f = 0; f < 8; ++f) {
-        float s = 0.0f;
-        for (int kk = klo; kk <= khi; ++kk) s += FILT_W[kk + 3] * pre[f][t + kk];
-        v[f] = s / wsum;
-    }
-    const float n1 = n.x, n2 = n.y;
-    const float vnf = n1 * v[0] + n2 * v[1], vtf = n2 * v[0] - n1 * v[1];
-    const float vns = n1 * v[3] + n2 * v[4], vts = n2 * v[3] - n1 * v[4];
-    const float snn = n1 * n1 * v[5] + 2.0f * n1 * n2 * v[7] + n2 * n2 * v[6];
-    const float stt = n2 * n2 * v[5] - 2.0f * n1 * n2 * v[7] + n1 * n1 * v[6];
-    const float vn = 0.5f * (vnf + vns), sn = 0.5f * (snn - v[2]);
     float o[8];
-    o[0] = n1 * vn + n2 * vtf;
-    o[1] = n2 * vn - n1 * vtf;
-    o[2] = -sn;
-    o[3] = n1 * vn + n2 * vts;
-    o[4] = n2 * vn - n1 * vts;
-    o[5] = n1 * n1 * sn + n2 * n2 * stt;
-    o[6] = n2 * n2 * sn + n1 * n1 * stt;
-    o[7] = n1 * n2 * (sn - stt);
+    iface_filter(&pre[0][0], FB, t, ix, nx, c.nif[ix], o);
 #pragma unroll
     for (int f = 0; f < 8; ++f) Uo[iface_index(g, f, ix)] = o[f];
-    if (it % ctl.gap == 0) {
-        const int slot = it / ctl.gap;
-        if (slot < ctl.rec_cap) {
-            const long long plane = (long long)ctl.rec_cap * nx;
-            float *rr = ctl.rec + (long long)slot * nx + ix;
-            rr[0] = o[3];
-            rr[plane] = o[4];
-            rr[2 * plane] = o[2];
-        }
-    }
+    iface_record(ctl, ctl.base[0] + jstep, nx, ix, o);
 }
 
 template <int DIR>
@@ -1074,12 +1181,14 @@ struct Shot {
     float *kif;
 };
 
-// tiles[t] = (shot, band, bx, fluid bands of that shot)
-template <int DIR, int STAGE, int TZ>
+// tiles[t] = (shot, band, bx, fluid bands of that shot); FIN as in stage_kernel.
+template <int DIR, int STAGE, int TZ, int FIN = 0>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
                    int jstep, float ct, float cprev) {
+    static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    __shared__ float fin[FIN ? 8 : 1][FW];
     const int per_xcd = gridDim.x >> 3;
     const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (t >= ntiles) return;
@@ -1088,11 +1197,11 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     const float *S = s.buf[rS];
     float *O = s.buf[rO];
     if (tl.y < tl.w)
-        fluid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
-                                   cprev, tl.z, tl.y);
+        fluid_body<DIR, STAGE, TZ, 2, 0, FIN>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep,
+                                              ct, 0.0f, cprev, tl.z, tl.y, fin);
     else
-        solid_body<DIR, STAGE, TZ, 2>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep, ct, 0.0f,
-                                      cprev, tl.z, tl.y - tl.w);
+        solid_body<DIR, STAGE, TZ, 2, TOP_IFACE, FIN>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
+                                                      jstep, ct, 0.0f, cprev, tl.z, tl.y - tl.w, fin);
 }
 
 // grid (column blocks of the widest shot, shots)
