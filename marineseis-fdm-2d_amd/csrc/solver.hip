@@ -190,19 +190,21 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
 #undef MSFDM_FREE_LAUNCH
         return;
     }
-    const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
-    const int ntiles = nbx * (nbf + nbs);
-    // each XCD: one eighth of the fluid tiles, then one eighth of the solid tiles (D::xcd_tile)
-    const int grid = 8 * ((nbx * nbf + 7) / 8 + (nbx * nbs + 7) / 8);
+    const int nzf = s->d.nz_fluid;
+    const int nbf = (nzf - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
+    int nfi = 0;   // fluid bands whose loaded rows (z0 - 3 .. z0 + TZ + 2, z0 = 3 + band TZ) hold row nzf - 1
+    for (int b = 0; b < nbf; ++b) nfi += (nzf - 1) - b * TZ < TZ + 2 * D::HALO;
+    // each XCD: one eighth of the interface-band, the other fluid and the other solid tiles (D::xcd_tile)
+    const int grid = 8 * ((nbx * (nfi + 1) + 7) / 8 + (nbx * (nbf - nfi) + 7) / 8 + (nbx * (nbs - 1) + 7) / 8);
     if (s->geo.fcpml)   // top PML in the water; metric form decided at run time
         D::stage_kernel<DIR, STAGE, TZ, 2, 1, FIN><<<grid, D::NTHR, 0, s->stream>>>(
-            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
     else if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
         D::stage_kernel<DIR, STAGE, TZ, 1, 0, FIN><<<grid, D::NTHR, 0, s->stream>>>(
-            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
     else
         D::stage_kernel<DIR, STAGE, TZ, 0, 0, FIN><<<grid, D::NTHR, 0, s->stream>>>(
-            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, ntiles);
+            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
 }
 // One RK stage; cprev is the previous stage's weight (its interface rows are
 // rebuilt by this stage's tiles).  Stage 4 adds the interface / filter / record

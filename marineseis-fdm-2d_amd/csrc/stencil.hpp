@@ -1041,25 +1041,39 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 
 // One RK stage over both media in a single launch.  The grid is 1-D with a
 // multiple of 8 blocks; consecutive blocks go round-robin to the 8 XCDs.  Each
-// XCD takes one eighth of the fluid tiles and one eighth of the solid tiles,
-// each as a contiguous run (row bands of 64-column tiles), so every XCD finds
-// the halo rows of its neighbours in its own L2 and gets the same mix of the
-// cheaper fluid and the dearer solid tiles: block b is the (b / 8)-th tile of
-// XCD b % 8, fluid run first (xcd_tile).
+// XCD takes, in this order, one eighth of the interface-band tiles (the last
+// nfi fluid bands, whose loaded rows hold the interface row, and solid band 0:
+// they also rebuild or finish interface rows, so they are the slowest tiles),
+// column by column so that an XCD holds the whole interface stack of its
+// columns; then one eighth of the other fluid tiles and one eighth of the other
+// solid tiles, each as a contiguous run of row bands, so every XCD finds the
+// halo rows of its neighbours in its own L2 and gets the same mix of cheap,
+// dear and slow tiles.
 #ifndef MSFDM_STAGE_WAVES
 #define MSFDM_STAGE_WAVES 0   // minimum waves per SIMD requested for stage_kernel (0 = compiler choice)
 #endif
-// CPML = 1: the fluid-solid CPML grid (top PML in the water, Cpml::FluidSolid).
-// Tile of block b when each of the 8 XCDs takes a run of cf fluid tiles then a
-// run of cs solid tiles (nf fluid tiles in total, ntiles in all); -1 past the end.
-__device__ __forceinline__ int xcd_tile(int b, int nf, int ntiles, int cf, int cs) {
+// Tile (bx, band) of block b, band -1 past the end of a run; bands 0..nbf-1
+// are fluid, nbf..nbf+nbs-1 solid.
+__device__ __forceinline__ int2 xcd_tile(int b, int nbx, int nbf, int nbs, int nfi) {
     const int x = b & 7, l = b >> 3;
-    if (l < cf) {
-        const int t = x * cf + l;
-        return t < nf ? t : -1;
+    const int ni = nbx * (nfi + 1), nf = nbx * (nbf - nfi), ns = nbx * (nbs - 1);
+    const int ci = (ni + 7) >> 3, cf = (nf + 7) >> 3, cs = (ns + 7) >> 3;
+    if (l < ci) {
+        const int t = x * ci + l;
+        if (t >= ni) return make_int2(0, -1);
+        const int bx = t / (nfi + 1);
+        return make_int2(bx, nbf - nfi + (t - bx * (nfi + 1)));
     }
-    const int t = nf + x * cs + (l - cf);
-    return t < ntiles ? t : -1;
+    if (l < ci + cf) {
+        const int t = x * cf + (l - ci);
+        if (t >= nf) return make_int2(0, -1);
+        const int band = t / nbx;
+        return make_int2(t - band * nbx, band);
+    }
+    const int t = x * cs + (l - ci - cf);
+    if (t >= ns) return make_int2(0, -1);
+    const int band = t / nbx;
+    return make_int2(t - band * nbx, nbf + 1 + band);
 }
 
 // FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
@@ -1069,15 +1083,13 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
              const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
              float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
-             int ntiles) {
+             int nbz_s, int nfi) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
     __shared__ float fin[FIN ? 8 : 1][FW];
-    const int nf = nbx * nbz_f;
-    const int cf = (nf + 7) >> 3, cs = (ntiles - nf + 7) >> 3;   // per-XCD runs (the grid is 8 * (cf + cs))
-    const int tile = xcd_tile(blockIdx.x, nf, ntiles, cf, cs);
-    if (tile < 0) return;
-    const int band = tile / nbx, bx = tile - band * nbx;
+    const int2 tb = xcd_tile(blockIdx.x, nbx, nbz_f, nbz_s, nfi);
+    if (tb.y < 0) return;
+    const int bx = tb.x, band = tb.y;
     if (band < nbz_f)
         fluid_body<DIR, STAGE, TZ, MET, CPML, FIN>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev,
                                                    bx, band, fin);
