@@ -520,8 +520,10 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
 // strip carries the fluid zeta memory.
 // ---------------------------------------------------------------------------
 // FIN (stage 1 only): also finish the previous step's interface rows
-// (finish_interface_row, LDS scratch fin, cprev = dt).
-template <int DIR, int STAGE, int TZ, int MET, int FT = 0, int FIN = 0>
+// (finish_interface_row, LDS scratch fin, cprev = dt).  INT: an interior tile
+// (fluid_interior: no PML strip, surface, interface or boundary row in it), whose
+// boundary branches are compiled out.
+template <int DIR, int STAGE, int TZ, int MET, int FT = 0, int FIN = 0, int INT = 0>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
            const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
@@ -533,7 +535,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     const int P = g.P, nx = g.nx, nzf = g.nzf;
     const int ix = x0 + tx;
     const bool active = ix >= 3 && ix <= nx - 4;
-    const int scol = side_col(ix, nx, g.npd);
+    const int scol = INT ? -1 : side_col(ix, nx, g.npd);
     float amp = 0.0f, pxa = 0.f, pxb = 0.f, pxc = 0.f;
     const float *const cp[2] = {c.rhof_inv, c.Kf};
     RowPre<3, 2> nxt;
@@ -545,13 +547,13 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     }
     load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
     __syncthreads();
-    if constexpr (STAGE >= 2) {
+    if constexpr (STAGE >= 2 && !INT) {
         const int lrow = (nzf - 1) - (z0 - HALO);   // tile row of the interface (the tile starts inside the fluid)
         if (lrow < TZ + 2 * HALO) {
             rebuild_interface_row<DIR, STAGE, 3, TZ>(sh, g, c, S, kif, lrow, 0, x0, cprev, lrow - HALO < TZ);
             __syncthreads();
         }
-    } else if constexpr (STAGE == 1 && FIN) {
+    } else if constexpr (STAGE == 1 && FIN && !INT) {
         const int lrow = (nzf - 1) - (z0 - HALO);
         if (lrow < TZ + 2 * HALO) {
             finish_interface_row<DIR, 3, TZ>(sh, fin, g, c, ctl, S, kif, lrow, 0, x0, lrow - HALO < TZ, cprev, jstep);
@@ -578,8 +580,8 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const long long gi = (long long)iz * P + ix;
         const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz;
-        const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
-        const bool top = FT == 1 && iz <= g.npd + 2;   // fluid zeta-PML strip (CPML grid)
+        const int dist_flso = INT ? 3 : nzf - 1 - iz, dist_fs = INT ? 3 : iz - 3;
+        const bool top = !INT && FT == 1 && iz <= g.npd + 2;   // fluid zeta-PML strip (CPML grid)
 
         // ---- velocity: rho dv/dt = -grad p (include/pml_fs_kernels.cuh:333-450)
         float rvx, rvz, Dx, Dz, Dzx = 0.f, Dzz = 0.f;
@@ -752,7 +754,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 // TOP_IFACE_CPML: TOP_IFACE on the fluid-solid CPML grid, whose sigma_xz rate
 // reads its metrics at the solid-array row (include/cpml_kernels.cuh:447-452).
 constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2, TOP_IFACE_CPML = 3;
-template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE, int FIN = 0>
+template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE, int FIN = 0, int INT = 0>
 __device__ __forceinline__ void
 solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
            const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
@@ -765,7 +767,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     const int ix = x0 + tx;
     const bool active = ix >= 3 && ix <= nx - 4;
-    const int scol = side_col(ix, nx, npd);
+    const int scol = INT ? -1 : side_col(ix, nx, npd);
     const long long moff = SOLO ? 0 : (long long)(g.nzf - 1) * P;   // solid row -> global metric row
     constexpr int ztop = SOLO ? 3 : 0;                               // first updated row
     float pxa = 0.f, pxb = 0.f, pxc = 0.f;
@@ -781,12 +783,12 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     }
     load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
-    if constexpr (STAGE >= 2 && !SOLO) {
+    if constexpr (STAGE >= 2 && !SOLO && !INT) {
         if (z0 == 0) {   // the band holding solid row 0 (tile row HALO)
             rebuild_interface_row<DIR, STAGE, 5, TZ>(sh, g, c, S, kif, HALO, 3, x0, cprev, true);
             __syncthreads();
         }
-    } else if constexpr (STAGE == 1 && FIN && !SOLO) {
+    } else if constexpr (STAGE == 1 && FIN && !SOLO && !INT) {
         if (z0 == 0) {
             finish_interface_row<DIR, 5, TZ>(sh, fin, g, c, ctl, S, kif, HALO, 3, x0, true, cprev, jstep);
             __syncthreads();
@@ -816,11 +818,11 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const long long gi = moff + si;
         const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
         const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
-        const bool side = scol >= 0, bottom = iz >= zb0;
-        const bool top = TOP == TOP_PLAIN && g.ztop_pml && iz <= npd + 2;
+        const bool side = scol >= 0, bottom = !INT && iz >= zb0;
+        const bool top = !INT && TOP == TOP_PLAIN && g.ztop_pml && iz <= npd + 2;
         // rows below the interface / free surface (the plain top never closes the stencils)
-        const int dist = TOP == TOP_PLAIN ? 3 : iz - ztop;
-        const bool surf = TOP == TOP_FREE && dist == 0;
+        const int dist = (INT || TOP == TOP_PLAIN) ? 3 : iz - ztop;
+        const bool surf = !INT && TOP == TOP_FREE && dist == 0;
 
         // ---- stress (include/pml_fs_kernels.cuh:509-628; solid-only: 10-187)
         const float dvxx = dx_lds<DIR>(svx[r], cc);
@@ -1076,6 +1078,23 @@ __device__ __forceinline__ int2 xcd_tile(int b, int nbx, int nbf, int nbs, int n
     return make_int2(t - band * nbx, nbf + 1 + band);
 }
 
+// Interior tiles: every column outside the side strips, every row at least 3
+// rows from a surface, the interface or a closed top, outside the zeta strips,
+// and (fluid) the loaded rows clear of the interface row, so no rebuild either.
+__device__ __forceinline__ bool cols_interior(const Geo &g, int x0) {
+    return x0 >= 3 + g.npd && x0 + TX - 1 <= g.nx - 4 - g.npd;
+}
+template <int FT>
+__device__ __forceinline__ bool fluid_interior(const Geo &g, int x0, int z0, int TZ) {
+    return cols_interior(g, x0) && (FT == 1 ? z0 > g.npd + 2 : z0 >= 6) && z0 + TZ + 2 < g.nzf - 1;
+}
+template <int TOP>
+__device__ __forceinline__ bool solid_interior(const Geo &g, int x0, int z0, int TZ) {
+    constexpr int ztop = (TOP == TOP_FREE || TOP == TOP_PLAIN) ? 3 : 0;
+    return cols_interior(g, x0) && (TOP == TOP_PLAIN || z0 - ztop >= 3) && !(g.ztop_pml && z0 <= g.npd + 2) &&
+           z0 + TZ - 1 < g.nzs - 3 - g.npd;
+}
+
 // FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
 // and record (finish_interface_row); cprev is then the time step.
 template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
@@ -1090,13 +1109,23 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
     const int2 tb = xcd_tile(blockIdx.x, nbx, nbz_f, nbz_s, nfi);
     if (tb.y < 0) return;
     const int bx = tb.x, band = tb.y;
-    if (band < nbz_f)
-        fluid_body<DIR, STAGE, TZ, MET, CPML, FIN>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct, cprev,
-                                                   bx, band, fin);
-    else
-        solid_body<DIR, STAGE, TZ, MET, CPML ? TOP_IFACE_CPML : TOP_IFACE, FIN>(sh, g, c, ctl, S, U, T2, T3, O, kif,
-                                                                               jstep, ct, amp_direct, cprev, bx,
-                                                                               band - nbz_f, fin);
+    constexpr int TOP = CPML ? TOP_IFACE_CPML : TOP_IFACE;
+    if (band < nbz_f) {
+        if (fluid_interior<CPML>(g, bx * TX, 3 + band * TZ, TZ))
+            fluid_body<DIR, STAGE, TZ, MET, CPML, FIN, 1>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
+                                                          cprev, bx, band, fin);
+        else
+            fluid_body<DIR, STAGE, TZ, MET, CPML, FIN, 0>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
+                                                          cprev, bx, band, fin);
+    } else {
+        const int sb = band - nbz_f;
+        if (solid_interior<TOP>(g, bx * TX, sb * TZ, TZ))
+            solid_body<DIR, STAGE, TZ, MET, TOP, FIN, 1>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
+                                                         cprev, bx, sb, fin);
+        else
+            solid_body<DIR, STAGE, TZ, MET, TOP, FIN, 0>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
+                                                         cprev, bx, sb, fin);
+    }
 }
 
 // One RK stage of a solid-only grid (PmlFs::SolidOnly, src/pml_fs_kernels.cu:29-381;
@@ -1113,8 +1142,12 @@ free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const flo
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
-    solid_body<DIR, STAGE, TZ, MET, TOP>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f, bx,
-                                         band);
+    if (solid_interior<TOP>(g, bx * TX, 3 + band * TZ, TZ))
+        solid_body<DIR, STAGE, TZ, MET, TOP, 0, 1>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f,
+                                                   bx, band);
+    else
+        solid_body<DIR, STAGE, TZ, MET, TOP, 0, 0>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f,
+                                                   bx, band);
 }
 
 // Raw-k mode (STAGE 0, msfdm_eval_rhs): the interface condition applied to the
