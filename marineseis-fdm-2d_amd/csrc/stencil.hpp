@@ -1241,12 +1241,22 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     const Shot &s = shots[tl.x];
     const float *S = s.buf[rS];
     float *O = s.buf[rO];
-    if (tl.y < tl.w)
-        fluid_body<DIR, STAGE, TZ, 2, 0, FIN>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif, jstep,
-                                              ct, 0.0f, cprev, tl.z, tl.y, fin);
-    else
-        solid_body<DIR, STAGE, TZ, 2, TOP_IFACE, FIN>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
-                                                      jstep, ct, 0.0f, cprev, tl.z, tl.y - tl.w, fin);
+    if (tl.y < tl.w) {
+        if (fluid_interior<0>(s.g, tl.z * TX, 3 + tl.y * TZ, TZ))
+            fluid_body<DIR, STAGE, TZ, 2, 0, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
+                                                     jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
+        else
+            fluid_body<DIR, STAGE, TZ, 2, 0, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
+                                                     jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
+    } else {
+        const int sb = tl.y - tl.w;
+        if (solid_interior<TOP_IFACE>(s.g, tl.z * TX, sb * TZ, TZ))
+            solid_body<DIR, STAGE, TZ, 2, TOP_IFACE, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
+                                                             s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
+        else
+            solid_body<DIR, STAGE, TZ, 2, TOP_IFACE, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
+                                                             s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
+    }
 }
 
 // grid (column blocks of the widest shot, shots)
