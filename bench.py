@@ -46,6 +46,17 @@ def cpu_baseline(model, nsteps: int):
                       f"(after 2 untimed steps), {dt:.2f} s wall"}
 
 
+def stage2_kernel_names(solid: bool, tile_rows: int):
+    """rocprofv3 names of the RK stage-2 launch (F direction), vertical-column
+    metrics first.  Template arguments: direction, stage, tile rows, metric form
+    (1 = vertical grid columns), then the top kind (free surface: 1) of the
+    solid-only kernel, or the CPML flag (0) and the FIN flag (0) of the
+    fluid-solid one."""
+    if solid:
+        return [f"free_stage_kernel<0, 2, {tile_rows}, {m}, 1>" for m in (1, 0)]
+    return [f"stage_kernel<0, 2, {tile_rows}, {m}, 0, 0>" for m in (1, 0)]
+
+
 def measured_traffic(workload: str, kernels):
     """(kernel name, HBM bytes per launch) of the first of `kernels` found in the
     committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE passes scaled
@@ -254,9 +265,7 @@ def bench_solid(args):
     t_stage4 = s.time_kernel(1, reps)
     b_stage = s.kernel_bytes(0)
     achieved = b_stage / (t_stage * 1e-3) / 1e9
-    # template arguments: direction, stage, tile rows, metric form, top kind (1 = free surface)
-    kname, traffic = measured_traffic(args.solid_workload,
-                                      [f"free_stage_kernel<0, 2, {s.tile_rows}, {m}, 1>" for m in (1, 0)])
+    kname, traffic = measured_traffic(args.solid_workload, stage2_kernel_names(True, s.tile_rows))
     cpu = cpu_baseline(model, args.cpu_steps) if args.cpu_steps > 0 else None
     out = {
         "metric": "Gcell-updates/s (plain FP32 solid-only solver, traction-free surface, 1 GPU)",
@@ -399,9 +408,7 @@ def main():
     b_stage = s.kernel_bytes(0)
     achieved = b_stage / (t_stage * 1e-3) / 1e9
 
-    # template arguments: direction, stage, tile rows, metric form (1 = vertical grid
-    # columns), CPML top (0)
-    kname, traffic = measured_traffic(args.workload, [f"stage_kernel<0, 2, {s.tile_rows}, {m}, 0>" for m in (1, 0)])
+    kname, traffic = measured_traffic(args.workload, stage2_kernel_names(False, s.tile_rows))
 
     cpu = None
     if args.cpu_steps > 0:

@@ -148,6 +148,19 @@ def test_cli_failure_before_solver_keeps_output_dir(model_dir, tmp_path):
     assert (out / "keep.txt").exists()
 
 
+def test_bench_roofline_kernels_are_in_the_committed_profiles():
+    """bench.py reads the measured HBM traffic of its roofline kernel from the
+    committed rocprofv3 summaries by kernel name; the names must still match."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for solid, workload in ((False, "seafloor2001"), (True, "solid_topo2001")):
+        name, traffic = bench.measured_traffic(workload, bench.stage2_kernel_names(solid, 8))
+        assert name is not None, workload
+        assert traffic > 0
+
+
 def test_main_debug_usage():
     if not os.path.exists(msfdm.BIN_PATH):
         pytest.skip("bin/main_debug not built")
