@@ -248,8 +248,9 @@ __constant__ float FILT_W[7] = {   // GAUSS_W7_PML, src/elements.cu:30-33
 // into LDS (tile column x0 at LDS column XOFF); cells outside [0,nrows) x [0,P)
 // read as 0.  The 64 interior columns move as 16-byte vectors (x0, the pitch P
 // and every field offset are multiples of 64 floats), the 2 x 3 halo columns as
-// single floats.
-template <int NF, int TZ>
+// single floats.  INB: every cell of the tile is known to lie inside the array
+// (an interior tile), so the bounds tests are compiled out.
+template <int NF, int TZ, bool INB = false>
 __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const float *__restrict__ S,
                                           const long long *off, int f0, int z0, int x0, int nrows, int P) {
     constexpr int LZ = TZ + 2 * HALO;
@@ -269,7 +270,7 @@ __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const
             const int r = e / (TX / 4), q = e - r * (TX / 4);
             const int gz = z0 - HALO + r;
             v[f][i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (e < NV && gz >= 0 && gz < nrows)
+            if (e < NV && (INB || (gz >= 0 && gz < nrows)))
                 v[f][i] = *reinterpret_cast<const float4 *>(src + (long long)gz * P + x0 + 4 * q);
         }
 #pragma unroll
@@ -279,7 +280,7 @@ __device__ __forceinline__ void load_tile(float (*sh)[TZ + 2 * HALO][LXP], const
             const int gz = z0 - HALO + r;
             const int gx = x0 + (h < HALO ? h - HALO : TX + (h - HALO));
             hv[f][i] = 0.0f;
-            if (e < NH && gz >= 0 && gz < nrows && gx >= 0 && gx < P) hv[f][i] = src[(long long)gz * P + gx];
+            if (e < NH && (INB || (gz >= 0 && gz < nrows && gx >= 0 && gx < P))) hv[f][i] = src[(long long)gz * P + gx];
         }
     }
 #pragma unroll
@@ -539,13 +540,13 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     float amp = 0.0f, pxa = 0.f, pxb = 0.f, pxc = 0.f;
     const float *const cp[2] = {c.rhof_inv, c.Kf};
     RowPre<3, 2> nxt;
-    if (active) {
+    if (INT || active) {   // an interior tile has every column active
         amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
         const long long a0 = (long long)min(z0 + ty, nzf - 1) * P + ix;
         prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_FVX, a0, a0);
     }
-    load_tile<3, TZ>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
+    load_tile<3, TZ, INT>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
     __syncthreads();
     if constexpr (STAGE >= 2 && !INT) {
         const int lrow = (nzf - 1) - (z0 - HALO);   // tile row of the interface (the tile starts inside the fluid)
@@ -560,7 +561,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             __syncthreads();
         }
     }
-    if (!active) return;
+    if (!INT && !active) return;
     const int cc = tx + XOFF;
     const float (*shp)[LXP] = sh[2];
     const float (*shvx)[LXP] = sh[0];
@@ -570,7 +571,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     for (int j = 0; j < TZ / 4; ++j) {
         const int lz = ty + 4 * j;
         const int iz = z0 + lz;
-        if (iz > nzf - 1) break;
+        if (!INT && iz > nzf - 1) break;   // (an interior tile ends 4 rows above the interface)
         const RowPre<3, 2> cur = nxt;
         if (j + 1 < TZ / 4) {
             const long long an = (long long)min(iz + 4, nzf - 1) * P + ix;
@@ -717,7 +718,8 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             rk<STAGE>(U, T2, T3, O, g.off[MSFDM_FZEZ] + ai, Ez, kEz, ct);
         }
         // ---- source: 5x5 Gaussian-weighted Ricker on p_k (include/tools_func.cuh:21-37)
-        {
+        // (never in an interior tile, whose rows then form one basic block)
+        if constexpr (!INT) {
             const int sxo = ix - ctl.srcx, szo = iz - ctl.srcz;
             if (sxo >= -2 && sxo <= 2 && szo >= -2 && szo <= 2 && iz <= nzf - 4)
                 kp += amp * SRC_W[szo + 2][sxo + 2];
@@ -774,14 +776,14 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     float amp = 0.0f;
     const float *const cp[5] = {c.C11, c.C13, c.C33, c.C55, c.rhos_inv};
     RowPre<5, 5> nxt;
-    if (active) {
+    if (INT || active) {   // an interior tile has every column active
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
         if constexpr (SOLO)
             amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
         const long long a0 = (long long)min(z0 + ty, nzs - 4) * P + ix;
         prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_SVX, a0, moff + a0);
     }
-    load_tile<5, TZ>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
+    load_tile<5, TZ, INT>(sh, S, g.off, MSFDM_SVX, z0, x0, nzs, P);
     __syncthreads();
     if constexpr (STAGE >= 2 && !SOLO && !INT) {
         if (z0 == 0) {   // the band holding solid row 0 (tile row HALO)
@@ -794,7 +796,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             __syncthreads();
         }
     }
-    if (!active) return;
+    if (!INT && !active) return;
     const int cc = tx + XOFF;
     const float (*svx)[LXP] = sh[0];
     const float (*svz)[LXP] = sh[1];
@@ -807,7 +809,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     for (int j = 0; j < TZ / 4; ++j) {
         const int lz = ty + 4 * j;
         const int iz = z0 + lz;
-        if (iz > nzs - 4) break;
+        if (!INT && iz > nzs - 4) break;   // (an interior tile ends above the bottom strip)
         const RowPre<5, 5> cur = nxt;
         if (j + 1 < TZ / 4) {
             const long long an = (long long)min(iz + 4, nzs - 4) * P + ix;
@@ -999,7 +1001,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         }
         // ---- source: 5x5 Gaussian-weighted Ricker on both normal stresses
         // (solid only, src/pml_fs_kernels.cu:22-27, include/tools_func.cuh:21-37)
-        if constexpr (SOLO) {
+        if constexpr (SOLO && !INT) {
             const int sxo = ix - ctl.srcx, szo = iz - ctl.srcz;
             if (sxo >= -2 && sxo <= 2 && szo >= -2 && szo <= 2) {
                 const float w = amp * SRC_W[szo + 2][sxo + 2];
@@ -1007,7 +1009,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 ksz += w;
             }
         }
-        if (!SOLO && iz == 0) {
+        if (!SOLO && !INT && iz == 0) {
             constexpr int par = STAGE == 0 ? 0 : step_par<DIR, STAGE>();
             float *kk = kif + (long long)kif_set(STAGE, par) * P;
             kk[3 * (long long)P + ix] = kvx;
@@ -1081,8 +1083,12 @@ __device__ __forceinline__ int2 xcd_tile(int b, int nbx, int nbf, int nbs, int n
 // Interior tiles: every column outside the side strips, every row at least 3
 // rows from a surface, the interface or a closed top, outside the zeta strips,
 // and (fluid) the loaded rows clear of the interface row, so no rebuild either.
+// A tile under the source's 5 x 5 footprint is not interior either.
 __device__ __forceinline__ bool cols_interior(const Geo &g, int x0) {
     return x0 >= 3 + g.npd && x0 + TX - 1 <= g.nx - 4 - g.npd;
+}
+__device__ __forceinline__ bool clear_of_source(const Ctl &ctl, int x0, int z0, int TZ) {
+    return ctl.srcx + 2 < x0 || ctl.srcx - 2 > x0 + TX - 1 || ctl.srcz + 2 < z0 || ctl.srcz - 2 > z0 + TZ - 1;
 }
 template <int FT>
 __device__ __forceinline__ bool fluid_interior(const Geo &g, int x0, int z0, int TZ) {
@@ -1111,7 +1117,7 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
     const int bx = tb.x, band = tb.y;
     constexpr int TOP = CPML ? TOP_IFACE_CPML : TOP_IFACE;
     if (band < nbz_f) {
-        if (fluid_interior<CPML>(g, bx * TX, 3 + band * TZ, TZ))
+        if (fluid_interior<CPML>(g, bx * TX, 3 + band * TZ, TZ) && clear_of_source(ctl, bx * TX, 3 + band * TZ, TZ))
             fluid_body<DIR, STAGE, TZ, MET, CPML, FIN, 1>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
                                                           cprev, bx, band, fin);
         else
@@ -1142,7 +1148,7 @@ free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const flo
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
     const int band = tile / nbx, bx = tile - band * nbx;
-    if (solid_interior<TOP>(g, bx * TX, 3 + band * TZ, TZ))
+    if (solid_interior<TOP>(g, bx * TX, 3 + band * TZ, TZ) && clear_of_source(ctl, bx * TX, 3 + band * TZ, TZ))
         solid_body<DIR, STAGE, TZ, MET, TOP, 0, 1>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f,
                                                    bx, band);
     else
@@ -1242,7 +1248,7 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     const float *S = s.buf[rS];
     float *O = s.buf[rO];
     if (tl.y < tl.w) {
-        if (fluid_interior<0>(s.g, tl.z * TX, 3 + tl.y * TZ, TZ))
+        if (fluid_interior<0>(s.g, tl.z * TX, 3 + tl.y * TZ, TZ) && clear_of_source(s.ctl, tl.z * TX, 3 + tl.y * TZ, TZ))
             fluid_body<DIR, STAGE, TZ, 2, 0, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
                                                      jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
         else
