@@ -241,8 +241,8 @@ __constant__ float SRC_W[5][5] = {   // src/elements.cu:34-40
     {0.0219f, 0.0983f, 0.1621f, 0.0983f, 0.0219f},
     {0.0133f, 0.0596f, 0.0983f, 0.0596f, 0.0133f},
     {0.0030f, 0.0133f, 0.0219f, 0.0133f, 0.0030f}};
-__constant__ float FILT_W[7] = {   // GAUSS_W7_PML, src/elements.cu:30-33
-    1.1e-13f, 1.678e-06f, 0.03419f, 0.93163f, 0.03419f, 1.678e-06f, 1.1e-13f};
+// GAUSS_W7_PML, src/elements.cu:30-33 (compile-time constants of iface_filter)
+constexpr float FILT_W[7] = {1.1e-13f, 1.678e-06f, 0.03419f, 0.93163f, 0.03419f, 1.678e-06f, 1.1e-13f};
 
 // Load NF fields of the stage state (rows z0-3 .. z0+TZ+2, columns x0-3 .. x0+66)
 // into LDS (tile column x0 at LDS column XOFF); cells outside [0,nrows) x [0,P)
@@ -397,14 +397,22 @@ __device__ __forceinline__ void iface_pre(const Geo &g, const Coef &c, const flo
 __device__ __forceinline__ void iface_filter(const float *pre, int pitch, int t, int ix, int nx, float2 n,
                                              float o[8]) {
 #pragma clang fp contract(off)
+    // taps outside the grid (klo..khi clips them) weigh 0; the 7 taps are
+    // unrolled with constant weights, so no tap waits on a memory load
     const int klo = max(-3, 3 - ix), khi = min(3, nx - 4 - ix);
+    float w[7];
     float wsum = 0.0f;
-    for (int kk = klo; kk <= khi; ++kk) wsum += FILT_W[kk + 3];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        w[q] = (q - 3 >= klo && q - 3 <= khi) ? FILT_W[q] : 0.0f;
+        wsum += w[q];
+    }
     float v[8];
 #pragma unroll
     for (int f = 0; f < 8; ++f) {
         float s = 0.0f;
-        for (int kk = klo; kk <= khi; ++kk) s += FILT_W[kk + 3] * pre[f * pitch + t + kk];
+#pragma unroll
+        for (int q = 0; q < 7; ++q) s += w[q] * pre[f * pitch + t + q - 3];
         v[f] = s / wsum;
     }
     const float n1 = n.x, n2 = n.y;
