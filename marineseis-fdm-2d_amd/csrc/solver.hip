@@ -192,8 +192,8 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     }
     const int nzf = s->d.nz_fluid;
     const int nbf = (nzf - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
-    int nfi = 0;   // fluid bands whose loaded rows (z0 - 3 .. z0 + TZ + 2, z0 = 3 + band TZ) hold row nzf - 1
-    for (int b = 0; b < nbf; ++b) nfi += (nzf - 1) - b * TZ < TZ + 2 * D::HALO;
+    int nfi = 0;   // fluid bands whose loaded rows (z0 - 3 .. z0 + TZ + 2, z0 = D::fluid_z0) hold row nzf - 1
+    for (int b = 0; b < nbf; ++b) nfi += (nzf - 1) - (nzf - (nbf - b) * TZ) < TZ + 2 * D::HALO;
     // each XCD: one eighth of the interface-band, the other fluid and the other solid tiles (D::xcd_tile)
     const int grid = 8 * ((nbx * (nfi + 1) + 7) / 8 + (nbx * (nbf - nfi) + 7) / 8 + (nbx * (nbs - 1) + 7) / 8);
     if (s->geo.fcpml)   // top PML in the water; metric form decided at run time

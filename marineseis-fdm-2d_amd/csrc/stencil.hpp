@@ -521,6 +521,14 @@ __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO]
     }
 }
 
+// First row of fluid band `band` of TZ-row tiles: the bands end at the interface
+// row nzf-1, so only the top band (holding the surface rows) is partial and a
+// single band holds the interface row; rows above 3 are not updated.
+__device__ __forceinline__ int fluid_z0(const Geo &g, int band, int TZ) {
+    const int nbf = (g.nzf - 3 + TZ - 1) / TZ;
+    return g.nzf - (nbf - band) * TZ;
+}
+
 // ---------------------------------------------------------------------------
 // Fluid stage: rows 3..nzf-1, columns 3..nx-4.  FT = 0: pressure-release sea
 // surface on row 3 (PmlFs::FluidSolid); FT = 1: a top PML in the water on rows
@@ -540,7 +548,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
            float ct, float amp_direct, float cprev, int bx, int bz, float (*fin)[FW] = nullptr) {
     // sh[0..2] = vx, vz, p of the stage state
     const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    const int x0 = bx * TX, z0 = 3 + bz * TZ;
+    const int x0 = bx * TX, z0 = fluid_z0(g, bz, TZ);
     const int P = g.P, nx = g.nx, nzf = g.nzf;
     const int ix = x0 + tx;
     const bool active = ix >= 3 && ix <= nx - 4;
@@ -551,7 +559,7 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
     if (INT || active) {   // an interior tile has every column active
         amp = (STAGE == 0) ? amp_direct : ctl.amp[4 * (ctl.base[0] + jstep) + (STAGE > 0 ? STAGE - 1 : 0)];
         if (scol >= 0) { pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix]; }
-        const long long a0 = (long long)min(z0 + ty, nzf - 1) * P + ix;
+        const long long a0 = (long long)min(max(z0 + ty, 3), nzf - 1) * P + ix;
         prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_FVX, a0, a0);
     }
     load_tile<3, TZ, INT>(sh, S, g.off, MSFDM_FVX, z0, x0, nzf, P);
@@ -582,9 +590,10 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         if (!INT && iz > nzf - 1) break;   // (an interior tile ends 4 rows above the interface)
         const RowPre<3, 2> cur = nxt;
         if (j + 1 < TZ / 4) {
-            const long long an = (long long)min(iz + 4, nzf - 1) * P + ix;
+            const long long an = (long long)min(max(iz + 4, 3), nzf - 1) * P + ix;
             prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_FVX, an, an);
         }
+        if (!INT && iz < 3) continue;   // above the first updated row (top band)
         const int r = lz + HALO;
         const long long gi = (long long)iz * P + ix;
         const float xx = cur.xx, xz = cur.xz, zx = cur.zx, zz = cur.zz;
@@ -1125,7 +1134,8 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
     const int bx = tb.x, band = tb.y;
     constexpr int TOP = CPML ? TOP_IFACE_CPML : TOP_IFACE;
     if (band < nbz_f) {
-        if (fluid_interior<CPML>(g, bx * TX, 3 + band * TZ, TZ) && clear_of_source(ctl, bx * TX, 3 + band * TZ, TZ))
+        const int z0 = fluid_z0(g, band, TZ);
+        if (fluid_interior<CPML>(g, bx * TX, z0, TZ) && clear_of_source(ctl, bx * TX, z0, TZ))
             fluid_body<DIR, STAGE, TZ, MET, CPML, FIN, 1>(sh, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
                                                           cprev, bx, band, fin);
         else
@@ -1256,7 +1266,8 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     const float *S = s.buf[rS];
     float *O = s.buf[rO];
     if (tl.y < tl.w) {
-        if (fluid_interior<0>(s.g, tl.z * TX, 3 + tl.y * TZ, TZ) && clear_of_source(s.ctl, tl.z * TX, 3 + tl.y * TZ, TZ))
+        const int z0 = fluid_z0(s.g, tl.y, TZ);
+        if (fluid_interior<0>(s.g, tl.z * TX, z0, TZ) && clear_of_source(s.ctl, tl.z * TX, z0, TZ))
             fluid_body<DIR, STAGE, TZ, 2, 0, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
                                                      jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
         else
