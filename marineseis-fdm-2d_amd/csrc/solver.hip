@@ -175,11 +175,10 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     const int nbx = (s->d.nx - 4) / D::TX + 1;
     if (s->solid_only) {   // rows 3..nzs-4 of the solid-only grid
         const int nb = (s->d.nz_solid - 6 + TZ - 1) / TZ;
-        const int ntiles = nbx * nb;
-        const int grid = 8 * ((ntiles + 7) / 8);
+        const int grid = 8 * ((nbx + 7) / 8 + (nbx * (nb - 1) + 7) / 8);   // D::xcd_tile with no fluid bands
 #define MSFDM_FREE_LAUNCH(MET, TOP)                                                                          \
     D::free_stage_kernel<DIR, STAGE, TZ, MET, TOP><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, \
-                                                                                    T3, O, j, ct, amp, nbx, ntiles)
+                                                                                    T3, O, j, ct, amp, nbx, nb)
         if (s->top == D::TOP_FREE) {
             if (s->coef.vert) MSFDM_FREE_LAUNCH(1, D::TOP_FREE);
             else MSFDM_FREE_LAUNCH(0, D::TOP_FREE);

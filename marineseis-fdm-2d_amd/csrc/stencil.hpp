@@ -1160,12 +1160,14 @@ template <int DIR, int STAGE, int TZ, int MET, int TOP>
 __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
 free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
                   const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O, int jstep,
-                  float ct, float amp_direct, int nbx, int ntiles) {
+                  float ct, float amp_direct, int nbx, int nb) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
-    const int per_xcd = gridDim.x >> 3;
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= ntiles) return;
-    const int band = tile / nbx, bx = tile - band * nbx;
+    // band 0 (the surface rows, the dearest tiles) spread over the XCDs and
+    // dispatched first, then contiguous runs of the other bands (xcd_tile with
+    // no fluid bands)
+    const int2 tb = xcd_tile(blockIdx.x, nbx, 0, nb, 0);
+    if (tb.y < 0) return;
+    const int bx = tb.x, band = tb.y;
     if (solid_interior<TOP>(g, bx * TX, 3 + band * TZ, TZ) && clear_of_source(ctl, bx * TX, 3 + band * TZ, TZ))
         solid_body<DIR, STAGE, TZ, MET, TOP, 0, 1>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f,
                                                    bx, band);
