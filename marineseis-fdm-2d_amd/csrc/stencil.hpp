@@ -827,8 +827,14 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
         const int lz = ty + 4 * j;
         const int iz = z0 + lz;
         if (!INT && iz > nzs - 4) break;   // (an interior tile ends above the bottom strip)
-        const RowPre<5, 5> cur = nxt;
-        if (j + 1 < TZ / 4) {
+        // Boundary tiles load a row's inputs at the row rather than one row
+        // ahead: the boundary path sets the kernel's register budget, and the
+        // prefetched row would stay live through all of its branches.
+        RowPre<5, 5> cur = nxt;
+        if constexpr (!INT) {
+            const long long ac = (long long)min(iz, nzs - 4) * P + ix;
+            if (j > 0) prefetch_row<STAGE, MET>(cur, c, cp, U, g.off, MSFDM_SVX, ac, moff + ac);
+        } else if (j + 1 < TZ / 4) {
             const long long an = (long long)min(iz + 4, nzs - 4) * P + ix;
             prefetch_row<STAGE, MET>(nxt, c, cp, U, g.off, MSFDM_SVX, an, moff + an);
         }
