@@ -547,7 +547,10 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
            const float *__restrict__ T3, float *__restrict__ O, float *__restrict__ kif, int jstep,
            float ct, float amp_direct, float cprev, int bx, int bz, float (*fin)[FW] = nullptr) {
     // sh[0..2] = vx, vz, p of the stage state
-    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    // a wave holds one row of the tile at a time: ty (and every row test) is
+    // uniform per wave, which readfirstlane tells the compiler, so the row
+    // branches are scalar branches rather than exec-masked ones
+    const int tx = threadIdx.x & (TX - 1), ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
     const int x0 = bx * TX, z0 = fluid_z0(g, bz, TZ);
     const int P = g.P, nx = g.nx, nzf = g.nzf;
     const int ix = x0 + tx;
@@ -780,7 +783,10 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
            float *__restrict__ O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
            int bz, float (*fin)[FW] = nullptr) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
-    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+    // a wave holds one row of the tile at a time: ty (and every row test) is
+    // uniform per wave, which readfirstlane tells the compiler, so the row
+    // branches are scalar branches rather than exec-masked ones
+    const int tx = threadIdx.x & (TX - 1), ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
     constexpr bool SOLO = TOP == TOP_FREE || TOP == TOP_PLAIN;   // a solid-only grid
     const int x0 = bx * TX, z0 = (SOLO ? 3 : 0) + bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
