@@ -55,6 +55,37 @@ def test_rhs_matches_oracle_on_random_state(preset, direction, tile_rows):
         assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
 
 
+@pytest.mark.parametrize("water_depth", [110.0, 115.0, 120.0])
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("tile_rows", [8, 16])
+def test_rhs_fluid_band_heights(water_depth, direction, tile_rows):
+    """The fluid bands end at the interface row, so the top band (with the
+    surface rows) holds 1..tile_rows rows depending on nz_fluid: 26, 27 and 28
+    fluid rows give top bands of 7, 8 and 1 rows at 8-row tiles (7, 8 and 9 at
+    16).  The raw RHS matches the oracle on every cell for each."""
+    m = modelgen.preset("cut201", water_depth=water_depth)
+    print(f"[rhs bands] nz_fluid={m.nz_fluid} dir={direction} tile_rows={tile_rows}", flush=True)
+    o = pyoracle.Oracle(m)
+    s = msfdm.Solver(m)
+    s.tile_rows = tile_rows
+    st = _random_state(m, 21 + direction)
+    for name, a in st.items():
+        a32 = a.astype(np.float32)
+        o.set(name, a32.astype(np.float64))
+        s.set(name, a32)
+    o.rhs(direction, 0.731)
+    s.eval_rhs(direction, 0.731)
+    for name in msfdm.FIELDS:
+        ko = o.get_k(name)
+        kg = s.get_rhs(name).astype(np.float64)
+        scale = np.abs(ko).max()
+        if scale == 0:
+            assert np.abs(kg).max() == 0, name
+            continue
+        err = np.abs(kg - ko).max() / scale
+        assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
+
+
 def _general_metrics(m, seed=1):
     """Copy of m with xi_x, xi_z varying in space (exercises the general-metric
     path; the oracle reads the same arrays)."""
