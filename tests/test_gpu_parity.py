@@ -55,14 +55,16 @@ def test_rhs_matches_oracle_on_random_state(preset, direction, tile_rows):
         assert err < 2e-5, f"{name}: max err {err:.3e} (scale {scale:.3e})"
 
 
-@pytest.mark.parametrize("water_depth", [110.0, 115.0, 120.0])
+@pytest.mark.parametrize("water_depth", [20.0, 25.0, 110.0, 115.0, 120.0])
 @pytest.mark.parametrize("direction", [0, 1])
 @pytest.mark.parametrize("tile_rows", [8, 16])
 def test_rhs_fluid_band_heights(water_depth, direction, tile_rows):
     """The fluid bands end at the interface row, so the top band (with the
     surface rows) holds 1..tile_rows rows depending on nz_fluid: 26, 27 and 28
     fluid rows give top bands of 7, 8 and 1 rows at 8-row tiles (7, 8 and 9 at
-    16).  The raw RHS matches the oracle on every cell for each."""
+    16); 8 and 9 fluid rows (the thinnest water layers accepted) give a single
+    band that starts above the grid (rows -8.. at 16-row tiles).  The raw RHS
+    matches the oracle on every cell for each."""
     m = modelgen.preset("cut201", water_depth=water_depth)
     print(f"[rhs bands] nz_fluid={m.nz_fluid} dir={direction} tile_rows={tile_rows}", flush=True)
     o = pyoracle.Oracle(m)
