@@ -11,6 +11,12 @@ after the surface row's p_k = 0):
 Each shot runs 300 steps; records within 1e-4 relative L2 of the FP64 oracle
 (exactly zero where the oracle's are zero).
 
+Source footprints across tile boundaries: a tile under the 5x5 footprint
+leaves the interior path (which carries no source term), so a footprint that
+straddles a column-tile or row-band boundary inside the interior of the 401 x
+201 grid must still be applied in full (pressure after 150 steps and records
+after 300, same tolerance).
+
 Size: the 4001 x 1501 two-scale seafloor grid (the largest workload) for 40
 steps against the oracle, same tolerance.
 """
@@ -78,3 +84,28 @@ def test_largest_grid_matches_oracle():
         e = rel_l2(a, b)
         print(f"  {name}: rel L2 {e:.3e}", flush=True)
         assert e <= 1e-4, (name, e)
+
+
+@pytest.mark.parametrize("tile_rows", [8, 16])
+def test_source_footprint_across_interior_tiles(tile_rows):
+    m = modelgen.preset("flat401", nt=300)
+    # column 127 | 128 is a tile boundary; with the fluid bands ending at row
+    # nz_f - 1 = 43, rows 19 | 20 separate two interior 8-row bands and rows
+    # 11 | 12 two 16-row bands (the lower one interior)
+    shots = [(127, 20), (127, 12), (191, 28)]
+    m.src_x = [x for x, _ in shots]
+    m.src_z = [z for _, z in shots]
+    for shot in range(len(shots)):
+        print(f"[footprint] tile_rows={tile_rows} shot at (ix, iz) = {shots[shot]}", flush=True)
+        o = pyoracle.Oracle(m)
+        s = msfdm.Solver(m)
+        s.tile_rows = tile_rows
+        o.reset(shot); o.run(150)
+        s.reset(shot); s.run(150)
+        e = rel_l2(s.get("FP"), o.get("FP"))
+        assert e <= 1e-4, (shot, e)
+        o.run(150)
+        s.run(150)
+        for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), s.records(), o.records()):
+            e = rel_l2(a, b)
+            assert e <= 1e-4, (shot, name, e)
