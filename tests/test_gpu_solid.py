@@ -220,3 +220,31 @@ def test_solid_cli_record_rows(kind, row, tmp_path):
     o.reset(0)
     o.run(60)
     assert rel_l2(vz, o.records()[0]) <= RECORD_TOL
+
+
+@pytest.mark.parametrize("tile_rows", [8, 16])
+def test_solid_source_footprint_across_interior_tiles(tile_rows):
+    """A tile under the source's 5x5 footprint leaves the interior path, which
+    carries no source term: footprints across the column-tile boundary 127 | 128
+    and the band boundary at row 19 (bands start at rows 3 + k tile_rows) inside
+    the interior of the 301 x 161 grid are applied in full."""
+    m = modelgen.preset("solid_flat301", nt=300)
+    shots = [(127, 19), (191, 35)]
+    m.src_x = [x for x, _ in shots]
+    m.src_z = [z for _, z in shots]
+    for shot in range(len(shots)):
+        print(f"[solid footprint] tile_rows={tile_rows} shot at (ix, iz) = {shots[shot]}", flush=True)
+        o = pyoracle.Oracle(m)
+        s = msfdm.Solver(m)
+        s.tile_rows = tile_rows
+        o.reset(shot); o.run(150)
+        s.reset(shot); s.run(150)
+        for f in ("SVZ", "SSX"):
+            e = rel_l2(s.get(f), o.get(f))
+            assert e <= 1e-4, (shot, f, e)
+        o.run(150)
+        s.run(150)
+        (ro,) = o.records()
+        (rg,) = s.records()
+        e = rel_l2(rg, ro)
+        assert e <= 1e-4, (shot, e)
