@@ -171,6 +171,27 @@ def test_flat401_full_record_matches_oracle():
         assert e <= RECORD_TOL, f"{name}: rel L2 {e:.3e}"
 
 
+def test_seafloor2001_full_grid_matches_oracle():
+    """The 2001x1001 body-fitted seafloor workload at its full size: 160 steps
+    (the source sits 4 cells above the seafloor, so the wave crosses the
+    interface, the records' rows and the random solid), 40 records, and every
+    field of the final state."""
+    m = modelgen.preset("seafloor2001", nt=160)
+    print(f"[records] seafloor2001 {m.nx}x{m.nz} 160 steps", flush=True)
+    o, s = _run_both(m, m.nt)
+    ro, rg = o.records(), s.records()
+    assert rg[0].shape == (40, 2001)
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), rg, ro):
+        assert np.abs(b).max() > 0, name
+        e = rel_l2(a, b)
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+    for name in ("FVX", "FVZ", "FP", "SVX", "SVZ", "SSX", "SSZ", "SSXZ"):
+        e = rel_l2(s.get(name), o.get(name))
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= 10 * RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+
+
 def test_second_shot_and_reset():
     m = modelgen.preset("tiny_curvi")
     o, s = _run_both(m, 120, isrc=1)
