@@ -291,3 +291,32 @@ def test_state_roundtrip_layout():
             assert np.array_equal(b[z0:z0 + m.npd], a[z0:z0 + m.npd])
         else:
             assert np.array_equal(a, b)
+
+
+def test_seafloor2001_whole_record_matches_oracle(request):
+    """The benchmark workload as the benchmark runs it: 2001x1001, all 6000 steps
+    (1500 records), against the FP64 oracle.  About 12e9 oracle cell updates, so
+    it runs only with --full-length."""
+    if not request.config.getoption("--full-length"):
+        pytest.skip("runs with --full-length (about 6 minutes of oracle time on 16 cores)")
+    m = modelgen.preset("seafloor2001")
+    print(f"[records] seafloor2001 {m.nx}x{m.nz} {m.nt} steps", flush=True)
+    s = msfdm.Solver(m)
+    s.reset(0)
+    s.run(m.nt)
+    o = pyoracle.Oracle(m)
+    o.reset(0)
+    for done in range(0, m.nt, 500):
+        o.run(min(500, m.nt - done))
+        print(f"  oracle at step {min(done + 500, m.nt)}", flush=True)
+    s.sync()
+    ro, rg = o.records(), s.records()
+    assert rg[0].shape == (1500, 2001)
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), rg, ro):
+        e = rel_l2(a, b)
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+    for name in ("FVX", "FVZ", "FP", "SVX", "SVZ", "SSX", "SSZ", "SSXZ"):
+        e = rel_l2(s.get(name), o.get(name))
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= 10 * RECORD_TOL, f"{name}: rel L2 {e:.3e}"
