@@ -61,8 +61,19 @@ constexpr double DRP_1 = +1.23300;
 constexpr double DRP_2 = -0.33340;
 constexpr double DRP_3 = +0.04168;
 
-// Interface-row filter taps (src/elements.cu:30-33).
-constexpr double W7[7] = {1.1e-13, 1.678e-06, 0.03419, 0.93163, 0.03419, 1.678e-06, 1.1e-13};
+// Interface-row filter taps (src/elements.cu:30-33), the reference's float values.
+constexpr double W7[7] = {double(1.1e-13f), double(1.678e-06f), double(0.03419f), double(0.93163f),
+                          double(0.03419f), double(1.678e-06f), double(1.1e-13f)};
+// The tap sum the filter divides by, accumulated in float over k = klo..khi as
+// the reference does (src/pml_fs_kernels.cu:474-477).  Its rounding is part of
+// the algorithm: for the seven interior taps the float sum lies 1.8e-8 below
+// the exact one, so every step scales the interface rows by 1 + 1.8e-8, about
+// 1.1e-4 over 6000 steps.  An exact sum would drift from the reference by that much.
+double filter_wsum(int klo, int khi) {
+    float s = 0.0f;
+    for (int k = klo; k <= khi; ++k) s += (float)W7[k + 3];
+    return s;
+}
 
 // 5x5 source spreading weights (src/elements.cu:34-40).
 constexpr double SRC_W[5][5] = {
@@ -762,8 +773,7 @@ void filter(Oracle &o, double *const *Ud) {
     std::vector<double> out(8 * (size_t)nx, 0.0);
     for (int ix = le; ix <= ri; ++ix) {
         const int klo = std::max(-3, le - ix), khi = std::min(3, ri - ix);
-        double wsum = 0.0;
-        for (int k = klo; k <= khi; ++k) wsum += W7[k + 3];
+        const double wsum = filter_wsum(klo, khi);
         double v[8];
         for (int f = 0; f < 8; ++f) {
             double s = 0.0;
@@ -1254,8 +1264,7 @@ void filter_adj(Oracle &o, double *const *A) {
         for (int i = 0; i < 8; ++i) yb[i] = *iface_ptr(o, A, i, ix);
         for (int j = 0; j < 8; ++j) { vb[j] = 0; for (int i = 0; i < 8; ++i) vb[j] += M[i][j] * yb[i]; }
         const int klo = std::max(-3, le - ix), khi = std::min(3, ri - ix);
-        double wsum = 0.0;
-        for (int k = klo; k <= khi; ++k) wsum += W7[k + 3];
+        const double wsum = filter_wsum(klo, khi);
         for (int f = 0; f < 8; ++f)
             for (int k = klo; k <= khi; ++k) out[(size_t)f * nx + ix + k] += W7[k + 3] / wsum * vb[f];
     }

@@ -248,3 +248,32 @@ def test_solid_source_footprint_across_interior_tiles(tile_rows):
         (rg,) = s.records()
         e = rel_l2(rg, ro)
         assert e <= 1e-4, (shot, e)
+
+
+def test_solid_topo2001_whole_record_matches_oracle(request):
+    """The solid-only benchmark workload as the benchmark runs it: 2001x1001
+    under the sinusoidal free surface, all 6000 steps (1500 surface records),
+    against the FP64 oracle.  Runs only with --full-length."""
+    if not request.config.getoption("--full-length"):
+        pytest.skip("runs with --full-length (about 6 minutes of oracle time on 16 cores)")
+    m = modelgen.preset("solid_topo2001")
+    print(f"[solid records] solid_topo2001 {m.nx}x{m.nz} {m.nt} steps", flush=True)
+    s = msfdm.Solver(m)
+    s.reset(0)
+    s.run(m.nt)
+    o = pyoracle.Oracle(m)
+    o.reset(0)
+    for done in range(0, m.nt, 500):
+        o.run(min(500, m.nt - done))
+        print(f"  oracle at step {min(done + 500, m.nt)}", flush=True)
+    s.sync()
+    (ro,) = o.records()
+    (rg,) = s.records()
+    assert rg.shape == (1500, 2001)
+    err = rel_l2(rg, ro)
+    print(f"  vz(row 3) rel L2 {err:.3e}", flush=True)
+    assert err <= RECORD_TOL
+    for name in ("SVX", "SVZ", "SSX", "SSZ", "SSXZ"):
+        e = rel_l2(s.get(name), o.get(name))
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= RECORD_TOL, name

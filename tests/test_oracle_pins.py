@@ -297,3 +297,39 @@ def test_oracle_step_invariants():
     mf = _flat_model()
     n1f, n2f = pyoracle.Oracle(mf).normals()
     assert np.all(n1f == 0.0) and np.all(n2f == 1.0)
+
+
+def test_filter_divides_by_the_float_tap_sum():
+    """The reference sums the seven float filter taps in float and divides by
+    that sum (src/elements.cu:30-33, src/pml_fs_kernels.cu:474-487).  The float
+    sum rounds 1.8e-8 below the exact sum of the taps, so the filter has a DC
+    gain of 1 + 1.8e-8 per step: 1.1e-4 over a 6000-step run.  On a flat
+    interface with a state that already meets the interface conditions and is
+    constant along x, the filter can only apply that gain.  Dividing by the
+    exact sum (gain 1) fails the test."""
+    taps = np.array([1.1e-13, 1.678e-06, 0.03419, 0.93163, 0.03419, 1.678e-06, 1.1e-13], dtype=np.float32)
+    fsum = np.float32(0.0)
+    for w in taps:                      # the reference's left-to-right float accumulation
+        fsum = np.float32(fsum + w)
+    gain = math.fsum(float(w) for w in taps) / float(fsum)
+    assert 1.5e-8 < gain - 1.0 < 2.5e-8
+
+    m = modelgen.build_model(nx=72, nz=56, h=5.0, water_depth=100.0, npd=8, fpeak=25.0, dt=4e-4, nt=60,
+                             src_x=[36], src_height=10.0)   # flat seafloor: normal (0, +-1)
+    o = pyoracle.Oracle(m)
+    rf = m.nz_fluid - 1
+    # v_n equal on both sides, p = -sigma_zz, sigma_xz = 0; the tangential parts are free
+    iface = {"FVX": 0.3, "FVZ": 0.7, "FP": 2.0, "SVX": -0.4, "SVZ": 0.7, "SSX": 1.5, "SSZ": -2.0, "SSXZ": 0.0}
+    for i, n in enumerate(pyoracle.FIELDS):
+        a = np.zeros(o.shape(i))
+        if n in iface:
+            a[rf if n.startswith("F") else 0, :] = iface[n]
+        o.set(n, a)
+    o.filter()
+    inner = slice(6, m.nx - 6)          # all seven taps inside columns 3..nx-4
+    for n, v in iface.items():
+        row = o.get(n)[rf if n.startswith("F") else 0, inner]
+        if v == 0.0:
+            assert np.abs(row).max() < 1e-15, n
+        else:
+            assert np.allclose(row / v, gain, rtol=0, atol=1e-13), (n, row[0] / v - 1.0)
