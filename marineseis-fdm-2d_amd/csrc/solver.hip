@@ -482,14 +482,16 @@ extern "C" int msfdm_solver_create(const msfdm_model *m, int device, msfdm_solve
     ptrs[np++] = c.pzaf = upload(pa, nZf);
     ptrs[np++] = c.pzbf = upload(pb, nZf);
     ptrs[np++] = c.pzcf = upload(pc, nZf);
-    // Interface normal (src/elements.cu:892-905) in double (zero on a solid-only grid).
+    // Interface normal (src/elements.cu:892-905; zero on a solid-only grid),
+    // computed in float as the reference does, so that it holds the same
+    // float2 bit for bit: every interface projection applies its n1^2 + n2^2.
     std::vector<float> nif(2 * (size_t)nx, 0.0f);
     for (int ix = 0; ix < nx && !s->solid_only; ++ix) {
         const size_t gi = (size_t)(nzf - 1) * nx + ix;
-        const double zx = m->a[MSFDM_A_ZE_X][gi], zz = m->a[MSFDM_A_ZE_Z][gi];
-        const double n = std::sqrt(zx * zx + zz * zz);
-        nif[2 * ix] = (float)(zx / n);
-        nif[2 * ix + 1] = (float)(zz / n);
+        const float zx = m->a[MSFDM_A_ZE_X][gi], zz = m->a[MSFDM_A_ZE_Z][gi];
+        const float n = std::sqrt(zx * zx + zz * zz);
+        nif[2 * ix] = zx / n;
+        nif[2 * ix + 1] = zz / n;
     }
     // vertical grid columns: xi_z == 0 and xi_x constant everywhere
     {

@@ -1399,11 +1399,16 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
     // Interface normal (src/elements.cu:892-905), or the free-surface ALM.
     o->nifx.assign(o->nx, 0.0); o->nifz.assign(o->nx, 0.0);
     if (o->solid_only && o->bdy == 2) build_alm(*o);
+    // The reference builds the normal in float and keeps it as float2, so
+    // n1^2 + n2^2 differs from 1 by a rounding fixed per column.  Every
+    // interface projection applies that factor, so the restatement takes the
+    // same float normal.
     for (int ix = 0; ix < o->nx && !o->solid_only; ++ix) {
         const size_t g = (size_t)(o->nzf - 1) * o->nx + ix;
-        const double s = std::sqrt(o->zex[g] * o->zex[g] + o->zez[g] * o->zez[g]);
-        o->nifx[ix] = o->zex[g] / s;
-        o->nifz[ix] = o->zez[g] / s;
+        const float zx = (float)o->zex[g], zz = (float)o->zez[g];
+        const float s = std::sqrt(zx * zx + zz * zz);
+        o->nifx[ix] = zx / s;
+        o->nifz[ix] = zz / s;
     }
     for (int i = 0; i < d->nsrc; ++i) { o->srcx.push_back(d->src_x[i]); o->srcz.push_back(d->src_z[i]); }
     for (int f = 0; f < NFIELD; ++f) {

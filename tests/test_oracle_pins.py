@@ -292,7 +292,16 @@ def test_oracle_step_invariants():
     assert np.abs(vnf - vns)[c].max() <= 1e-12 * np.abs(vns).max()
     sxx, szz, sxz = o.get("SSX")[0], o.get("SSZ")[0], o.get("SSXZ")[0]
     snn = n1 * n1 * sxx + 2 * n1 * n2 * sxz + n2 * n2 * szz
-    assert np.abs(p[nzf - 1] + snn)[c].max() <= 1e-12 * np.abs(snn).max()
+    # p = -sigma_nn holds up to the float normal's |n|^2 - 1 (about 1e-7): the
+    # stresses are rebuilt from sigma_nn along n, so their sigma_nn carries |n|^4
+    dn = np.abs(n1 * n1 + n2 * n2 - 1.0)[c].max()
+    assert np.abs(p[nzf - 1] + snn)[c].max() <= (4 * dn + 1e-12) * np.abs(snn).max()
+    # the normal is the reference's float computation (src/elements.cu:892-905):
+    # float components, a float square root and float quotients
+    zx, zz = m.arrays["ze_x"][nzf - 1], m.arrays["ze_z"][nzf - 1]
+    s32 = np.sqrt(zx * zx + zz * zz, dtype=np.float32)
+    assert np.array_equal(n1, (zx / s32).astype(np.float64)) and np.array_equal(n2, (zz / s32).astype(np.float64))
+    assert np.abs(n1 * n1 + n2 * n2 - 1.0).max() > 1e-9      # not renormalised in double
     # flat models have the vertical normal exactly
     mf = _flat_model()
     n1f, n2f = pyoracle.Oracle(mf).normals()
