@@ -293,6 +293,26 @@ def test_state_roundtrip_layout():
             assert np.array_equal(a, b)
 
 
+def test_curved_seafloor_long_run_matches_oracle():
+    """6000 steps (2000 records) on the small curved-seafloor, random-solid grid.
+    Over long runs a curved interface is where the reference's float interface
+    normal and its float filter normalisation show; the solver and the oracle
+    both carry them."""
+    m = modelgen.preset("tiny_curvi", nt=6000)
+    print(f"[records] tiny_curvi {m.nx}x{m.nz} 6000 steps", flush=True)
+    o, s = _run_both(m, m.nt)
+    ro, rg = o.records(), s.records()
+    assert rg[0].shape == (2000, 120)
+    for name, a, b in zip(("solid_vx", "solid_vz", "fluid_p"), rg, ro):
+        e = rel_l2(a, b)
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+    for name in ("FVX", "FVZ", "FP", "SVX", "SVZ", "SSX", "SSZ", "SSXZ"):
+        e = rel_l2(s.get(name), o.get(name))
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= 10 * RECORD_TOL, f"{name}: rel L2 {e:.3e}"
+
+
 def test_seafloor2001_whole_record_matches_oracle(request):
     """The benchmark workload as the benchmark runs it: 2001x1001, all 6000 steps
     (1500 records), against the FP64 oracle.  About 12e9 oracle cell updates, so
