@@ -36,7 +36,9 @@ class _Desc(C.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # MSFDM_ORACLE_LIB names another build of oracle.cpp (the mutation test
+        # builds copies with planted mistakes and runs the pins against them).
+        path = os.environ.get("MSFDM_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
         L = C.CDLL(path)
