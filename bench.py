@@ -6,10 +6,14 @@ record) over the whole grid of the workload.  Default workload: the 2001 x 1001
 body-fitted seafloor model (z = 500 + 40 sin(2 pi x / 800) m, 5 % random solid,
 20 Hz, dt = 0.25 ms, 30-cell CFS-PML), K = 6000 steps = one full shot record.
 
-Prints one JSON line: value = Gcell-updates/s (nx * nz * K / wall time, inputs
-resident in HBM), plus the roofline of the dominant kernel (the solid stage
-kernel, HIP-event timed on the solver's stream) and the FP64 CPU baseline on a
-bounded sample of the same workload.
+Prints one JSON line: value = Gcell-updates/s (nx * nz * K / wall time of the
+hipGraph-replayed run, inputs resident in HBM), the roofline of the dominant
+kernel (the RK stage-2 kernel over both media, its mean duration over the K
+steps of the same run with every launch timed by start / stop events its own
+dispatch stamps), the FP64 CPU baseline on bounded samples of the same
+workload (all host cores and one core), and the shots/s of the batched
+forward+adjoint gradient and of the batched forward, each beside the same shots
+run one at a time.
 """
 from __future__ import annotations
 
@@ -25,25 +29,35 @@ sys.path.insert(0, os.path.join(ROOT, "marineseis-fdm-2d_amd"))
 HBM_PEAK_GBPS = 8000.0
 
 
-def cpu_baseline(model, nsteps: int):
-    """FP64 oracle (OpenMP over rows) timed on `nsteps` steps of the same model."""
+def _oracle_rate(model, nsteps: int, threads: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    threads = os.cpu_count() or 1
-    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-    if cap > 0:
-        threads = min(threads, cap)
     o = pyoracle.Oracle(model, threads=threads)
     o.reset(0)
     o.run(2)   # first-touch / page-in outside the timed sample
     t0 = time.perf_counter()
     o.run(nsteps)
     dt = time.perf_counter() - t0
-    cells = model.nx * model.nz
-    return {"value": cells * nsteps / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{nsteps} FP64 RK4 steps of the same {model.nx}x{model.nz} model "
-                      f"(after 2 untimed steps), {dt:.2f} s wall"}
+    return model.nx * model.nz * nsteps / dt / 1e9, dt
+
+
+def cpu_baseline(model, nsteps: int, nsteps_1core: int):
+    """FP64 oracle (OpenMP over rows) timed on `nsteps` steps of the same model
+    with every host core the run may use, and on `nsteps_1core` steps with one."""
+    threads = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if cap > 0:
+        threads = min(threads, cap)
+    v, dt = _oracle_rate(model, nsteps, threads)
+    out = {"value": round(v, 5), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+           "sample": f"{nsteps} FP64 RK4 steps of the same {model.nx}x{model.nz} model "
+                     f"(after 2 untimed steps), {dt:.2f} s wall"}
+    if nsteps_1core > 0:
+        v1, dt1 = _oracle_rate(model, nsteps_1core, 1)
+        out["single_core"] = {"value": round(v1, 5), "unit": "Gcell-updates/s", "cores": 1,
+                              "sample": f"{nsteps_1core} FP64 RK4 steps of the same model on one thread, "
+                                        f"{dt1:.2f} s wall"}
+    return out
 
 
 def stage2_kernel_names(solid: bool, tile_rows: int):
@@ -73,7 +87,7 @@ def measured_traffic(workload: str, kernels):
     return None, None
 
 
-def bench_batch(args):
+def bench_batch(args, nsteps=None):
     """64 shot windows of uneven size: one batched run vs the same shots one at a
     time with the single-shot solver, both on `args.batch_steps` steps per shot
     (shots/s scaled to the full 4000-step record)."""
@@ -81,7 +95,7 @@ def bench_batch(args):
     from msfdm import modelgen
     models = modelgen.batch_shots(n=args.batch_shots, nt=4000)
     cells = sum(m.nx * m.nz for m in models)
-    nsteps = args.batch_steps
+    nsteps = nsteps or args.batch_steps
     b = msfdm.Batch(models)
     best = b.autotune(64)
     b.reset()
@@ -113,7 +127,7 @@ def bench_batch(args):
            "sequential_Gcell_per_s": round(cells * nsteps / t_seq / 1e9, 3),
            "tile_rows": best, "steps_timed": nsteps, "shots": len(models), "total_cells": cells,
            "note": f"timed {nsteps} of 4000 steps per shot after 64 warm-up steps; shots/s scaled x{scale:g}"}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_gradient(args):
@@ -181,7 +195,7 @@ def bench_gradient(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_shots(args):
+def bench_shots(args, workload=None, steps=None, nseq=None):
     """Multi-shot gradient: n shots of one model (sources spread evenly over the
     middle 80 % of the line), all shots of a wave stepped together
     (msfdm_gradient_shots) against the same shots one at a time (msfdm_gradient),
@@ -192,11 +206,12 @@ def bench_shots(args):
     import msfdm
     from msfdm import modelgen
     n = args.shots
-    nx = modelgen.preset(args.shots_workload, nt=1).nx
+    workload = workload or args.shots_workload
+    nx = modelgen.preset(workload, nt=1).nx
     src = [int(round(x)) for x in np.linspace(0.1 * (nx - 1), 0.9 * (nx - 1), n)]
-    model = modelgen.preset(args.shots_workload, src_x=src, **({"nt": args.shots_record} if args.shots_record else {}))
+    model = modelgen.preset(workload, src_x=src, **({"nt": args.shots_record} if args.shots_record else {}))
     full = model.nt
-    steps = args.shots_steps or full
+    steps = steps or args.shots_steps or full
     s = msfdm.Solver(model)
     s.reset(0); s.run(steps); s.sync()
     nrec = s.num_records()
@@ -208,7 +223,7 @@ def bench_shots(args):
     t0 = time.perf_counter()
     g, wave = s.gradient_shots(list(range(n)), steps, res)
     t_batch = time.perf_counter() - t0
-    k = min(n, args.shots_seq)
+    k = min(n, nseq or args.shots_seq)
     idx = [int(i) for i in np.linspace(0, n - 1, k)]
     t_seq, ref = 0.0, None
     for i in idx:
@@ -221,7 +236,8 @@ def bench_shots(args):
     par = {a: float(np.linalg.norm(gsub[a] - ref[a]) / np.linalg.norm(ref[a])) for a in msfdm._lib.GRADS}
     scale = full / steps
     hbm = 288e9
-    out = {"metric": f"multi-shot fwd+adjoint gradient shots/s ({args.shots_workload}, {full}-step records)",
+    s.close()
+    out = {"metric": f"multi-shot fwd+adjoint gradient shots/s ({workload}, {full}-step records)",
            "value": round(n / (t_batch * scale), 4), "unit": "shots/s",
            "sequential_value": round(n / (t_seq * scale), 4),
            "speedup_vs_sequential": round(t_seq / t_batch, 3),
@@ -233,7 +249,7 @@ def bench_shots(args):
            "rel_l2_batched_vs_sum_of_single": par,
            "data": "synthetic residuals (seeded normal), generated model",
            "note": (f"timed {steps} of {full} steps; shots/s scaled x{scale:g}" if scale != 1 else "full records")}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_solid(args):
@@ -333,6 +349,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="seafloor2001")
     ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=10, help="steps of the one-thread CPU sample (0 = skip)")
+    ap.add_argument("--no-shots", action="store_true",
+                    help="default mode: skip the batched-gradient and batched-forward shots/s legs")
     ap.add_argument("--kernel-reps", type=int, default=100)
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16],
                     help="tile height of the stage kernels (0 = the library's autotuner)")
@@ -364,13 +383,13 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}))
         return
     if args.mode == "batch":
-        bench_batch(args)
+        print(json.dumps(bench_batch(args)), flush=True)
         return
     if args.mode == "gradient":
         bench_gradient(args)
         return
     if args.mode == "shots":
-        bench_shots(args)
+        print(json.dumps(bench_shots(args)), flush=True)
         return
     if args.mode == "solid":
         bench_solid(args)
@@ -400,19 +419,39 @@ def main():
     value = cells * args.steps / wall / 1e9
     bytes_step = s.bytes_per_step()
 
-    # dominant kernel: the RK stage kernel over both media (stage-2 variant)
-    reps = args.kernel_reps
-    t_stage = s.time_kernel(0, reps)
-    t_stage4 = s.time_kernel(1, reps)
-    t_fin = s.time_kernel(2, reps)
-    b_stage = s.kernel_bytes(0)
+    # dominant kernel: the RK stage kernel over both media.  Its duration is
+    # taken inside the step sequence: the same shot again (reset, warm-up), the
+    # same K steps with direct launches, every launch timed by start / stop
+    # events its own dispatch stamps (no launch gaps, no repeat of one launch
+    # over cache-resident inputs).
+    s.reset(0)
+    s.run(args.warmup)
+    s.sync()
+    prof = s.profile_run(args.steps)
+    kms, nl = prof["kernel_ms"], prof["launches"]
+    stage_ms = [kms[i] / max(nl[i], 1) for i in range(4)]
+    t_stage = stage_ms[1]
+    b_stage, b_stage4 = s.kernel_bytes(0), s.kernel_bytes(1)
+    tile_rows = s.tile_rows
     achieved = b_stage / (t_stage * 1e-3) / 1e9
+    # the same launch repeated back to back (inputs partly cache resident)
+    reps = args.kernel_reps
+    t_rep2 = s.time_kernel(0, reps)
+    t_rep4 = s.time_kernel(1, reps)
 
-    kname, traffic = measured_traffic(args.workload, stage2_kernel_names(False, s.tile_rows))
+    kname, traffic = measured_traffic(args.workload, stage2_kernel_names(False, tile_rows))
 
     cpu = None
     if args.cpu_steps > 0:
-        cpu = cpu_baseline(model, args.cpu_steps)
+        cpu = cpu_baseline(model, args.cpu_steps, args.cpu_steps_1core)
+    s.close()
+    shots = None
+    if not args.no_shots:
+        shots = {
+            "gradient_flat401_32": bench_shots(args, workload="flat401", steps=2000, nseq=4),
+            "gradient_twoscale4001_32": bench_shots(args, workload="twoscale4001", steps=40, nseq=2),
+            "forward_batch64": bench_batch(args, nsteps=200),
+        }
 
     out = {
         "metric": "Gcell-updates/s (plain FP32 fluid-solid solver, 1 GPU)",
@@ -430,13 +469,17 @@ def main():
         "config": {
             "workload": f"{args.workload}: {model.nx}x{model.nz} grid (fluid {model.nz_fluid} + solid "
                         f"{model.nz_solid} rows), npd={model.npd}, dt={model.dt}, fpeak={model.fpeak}",
-            "nx": model.nx, "nz": model.nz, "time_steps": args.steps,
+            "nx": model.nx, "nz": model.nz, "time_steps": args.steps, "tile_rows": tile_rows,
             "hbm_GBps_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9, 1),
             "hbm_frac_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
             "bytes_per_step_alg": bytes_step,
-            "kernel_ms": {"stage_kernel_stage2": round(t_stage, 5), "stage_kernel_stage4": round(t_stage4, 5),
-                          "iface_final": round(t_fin, 5)},
-            "stage4_GBps_alg": round(s.kernel_bytes(1) / (t_stage4 * 1e-3) / 1e9, 1),
+            "kernel_ms_in_sequence": {"stage1": round(stage_ms[0], 5), "stage2": round(stage_ms[1], 5),
+                                      "stage3": round(stage_ms[2], 5), "stage4": round(stage_ms[3], 5),
+                                      "iface_final": round(kms[4] / max(nl[4], 1), 5)},
+            "launches_in_sequence": nl,
+            "ms_per_step_profiled": round(prof["wall_ms"] / args.steps, 5),
+            "kernel_ms_isolated_repeat": {"stage2": round(t_rep2, 5), "stage4": round(t_rep4, 5)},
+            "stage4_GBps_alg": round(b_stage4 / (stage_ms[3] * 1e-3) / 1e9, 1),
         },
         "roofline": {
             "bound": "hbm",
@@ -445,10 +488,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
-            "kernel": f"{kname or 'stage_kernel'} (RK stage 2, F direction, fluid + solid tiles)",
+            "kernel": f"{kname or 'stage_kernel'} (RK stage 2, both directions, fluid + solid tiles)",
             "alg_bytes_per_launch": b_stage,
+            "timing": f"mean of the {nl[1]} stage-2 launches of a {args.steps}-step run, dispatch-stamped events",
         },
         "cpu_baseline": cpu,
+        "shots": shots,
     }
     print(json.dumps(out), flush=True)
 

@@ -163,6 +163,15 @@ int msfdm_set_graph(msfdm_solver *s, int enable);
 int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms);
 /* Time nsteps of msfdm_run between HIP events on the solver's stream. */
 int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
+/* Step nsteps with the launch sequence of msfdm_run, each launch as a direct
+ * launch carrying start / stop events stamped by its own dispatch
+ * (hipExtLaunchKernelGGL), so kernel durations exclude the gaps between
+ * launches.  kernel_ms[6] / launches[6] (either may be NULL): total ms and
+ * launch count per kernel class: 0..3 = RK stages 1..4, 4 = the interface
+ * finish after a launch sequence, 5 = the step counter.  wall_ms: the whole
+ * run between two stream events.  The state advances as msfdm_run's would.
+ * No reference counterpart (measurement). */
+int msfdm_profile_run(msfdm_solver *s, int nsteps, float *kernel_ms, int *launches, float *wall_ms);
 /* Algorithmic HBM bytes of one time step (state + coefficients + PML memory). */
 int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
 /* Algorithmic HBM bytes of one launch timed by msfdm_time_kernel(which): an
@@ -198,7 +207,13 @@ int msfdm_grid_metrics(const double *x, const double *z, int nz, int nx, int dev
 /* The adjoint, multi-shot and batched entry points, and msfdm_update_media,
  * serve fluid-solid models with the pressure-release surface (calc_type 1,
  * boundary_type 2); with any other model they return MSFDM_EINVAL. */
-/* Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
+/* Residual contract: each residual plane is exactly nrec * nx floats, row-major
+ * [nrec][nx] over every grid column (PML and frozen columns included), with
+ * nrec = (nsteps - 1) / gap + 1 and gap = (int)(1e-3 / dt + 0.5), the record
+ * count of an nsteps-step run.  The library reads that many floats from each
+ * plane and cannot check the caller's buffer sizes; the Python binding refuses
+ * any other shape (a PML-cut gather, a shorter record).
+ * Gradient of J = sum_r <res_r, rec_r> (res = dJ/d(records), three [nrec][nx]
  * arrays for solid vx, solid vz, fluid p as returned by msfdm_get_records) with
  * respect to the model arrays K, rho_f ([nz_f][nx]) and C11, C13, C33, C55,
  * rho_s ([nz_s][nx]), all doubles, for shot isrc and nsteps steps from a zero

@@ -6,6 +6,7 @@
 // record sampling of src/main.cu:245-295.  One stream, explicit ordering (no
 // cross-stream races), records kept on the device.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -75,6 +76,13 @@ struct msfdm_solver {
     int adj_every = 0, adj_nsteps = 0, adj_isrc = 0;
     bool solid_only = false;         // grid.calc_type = 0: a solid grid
     int top = D::TOP_IFACE;          // top kind of the solid stage: interface, free surface or plain
+    // In-sequence kernel timing (msfdm_profile_run): while `on`, every launch of
+    // the stepper carries a start / stop event pair stamped by its own dispatch.
+    struct KProf {
+        bool on = false;
+        std::vector<hipEvent_t> ev;  // 2 per launch, kept for reuse
+        std::vector<int> cls;        // kernel class of each launch of the current run
+    } prof;
 };
 
 namespace {
@@ -122,6 +130,32 @@ void pml_profile(int n, int npd, int npow, bool lo, bool hi, double damp0, doubl
         pb[i] = (float)(dp / be);
         pc[i] = (float)(1.0 / be);
     }
+}
+
+// Kernel classes of msfdm_profile_run: RK stages 1..4 (0..3), the interface
+// finish (4), the step counter (5).
+constexpr int KCLASS_IFACE = 4, KCLASS_OTHER = 5, NKCLASS = 6;
+
+// Launch `kernel` on the solver's stream; during a profiled run with a start /
+// stop event pair recorded by the dispatch itself (hipExtLaunchKernelGGL), so
+// the events time the kernel and not the gaps between launches.
+template <class F, class... A>
+void klaunch(msfdm_solver *s, int cls, F kernel, dim3 grid, dim3 block, A... args) {
+    if (!s->prof.on) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, args...);
+        return;
+    }
+    const size_t i = s->prof.cls.size();
+    while (s->prof.ev.size() < 2 * i + 2) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {   // out of events: launch untimed
+            hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, args...);
+            return;
+        }
+        s->prof.ev.push_back(e);
+    }
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s->stream, s->prof.ev[2 * i], s->prof.ev[2 * i + 1], 0, args...);
+    s->prof.cls.push_back(cls);
 }
 
 int grow_tables(msfdm_solver *s, int steps_needed) {
@@ -176,9 +210,9 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     if (s->solid_only) {   // rows 3..nzs-4 of the solid-only grid
         const int nb = (s->d.nz_solid - 6 + TZ - 1) / TZ;
         const int grid = 8 * ((nbx + 7) / 8 + (nbx * (nb - 1) + 7) / 8);   // D::xcd_tile with no fluid bands
-#define MSFDM_FREE_LAUNCH(MET, TOP)                                                                          \
-    D::free_stage_kernel<DIR, STAGE, TZ, MET, TOP><<<grid, D::NTHR, 0, s->stream>>>(s->geo, s->coef, s->ctl, S, U, T2, \
-                                                                                    T3, O, j, ct, amp, nbx, nb)
+#define MSFDM_FREE_LAUNCH(MET, TOP)                                                                              \
+    klaunch(s, STAGE - 1, D::free_stage_kernel<DIR, STAGE, TZ, MET, TOP>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, \
+            s->ctl, S, U, T2, T3, O, j, ct, amp, nbx, nb)
         if (s->top == D::TOP_FREE) {
             if (s->coef.vert) MSFDM_FREE_LAUNCH(1, D::TOP_FREE);
             else MSFDM_FREE_LAUNCH(0, D::TOP_FREE);
@@ -195,15 +229,16 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     for (int b = 0; b < nbf; ++b) nfi += (nzf - 1) - (nzf - (nbf - b) * TZ) < TZ + 2 * D::HALO;
     // each XCD: one eighth of the interface-band, the other fluid and the other solid tiles (D::xcd_tile)
     const int grid = 8 * ((nbx * (nfi + 1) + 7) / 8 + (nbx * (nbf - nfi) + 7) / 8 + (nbx * (nbs - 1) + 7) / 8);
+    const int cls = STAGE > 0 ? STAGE - 1 : KCLASS_OTHER;
     if (s->geo.fcpml)   // top PML in the water; metric form decided at run time
-        D::stage_kernel<DIR, STAGE, TZ, 2, 1, FIN><<<grid, D::NTHR, 0, s->stream>>>(
-            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
+        klaunch(s, cls, D::stage_kernel<DIR, STAGE, TZ, 2, 1, FIN>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, s->ctl,
+                S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
     else if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
-        D::stage_kernel<DIR, STAGE, TZ, 1, 0, FIN><<<grid, D::NTHR, 0, s->stream>>>(
-            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
+        klaunch(s, cls, D::stage_kernel<DIR, STAGE, TZ, 1, 0, FIN>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, s->ctl,
+                S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
     else
-        D::stage_kernel<DIR, STAGE, TZ, 0, 0, FIN><<<grid, D::NTHR, 0, s->stream>>>(
-            s->geo, s->coef, s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
+        klaunch(s, cls, D::stage_kernel<DIR, STAGE, TZ, 0, 0, FIN>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, s->ctl,
+                S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
 }
 // One RK stage; cprev is the previous stage's weight (its interface rows are
 // rebuilt by this stage's tiles).  Stage 4 adds the interface / filter / record
@@ -220,8 +255,8 @@ void launch_stage(msfdm_solver *s, const float *S, float *O, int j, float ct, fl
     if constexpr (STAGE == 4) {
         if (fin_next) return;
         const float dt = s->d.dt;
-        D::iface_final<DIR><<<(s->d.nx + D::FO - 1) / D::FO, D::FB, 0, s->stream>>>(s->geo, s->coef, s->ctl, O, s->kif,
-                                                                                    dt / 2, dt, ct, j);
+        klaunch(s, KCLASS_IFACE, D::iface_final<DIR>, dim3((s->d.nx + D::FO - 1) / D::FO), dim3(D::FB), s->geo, s->coef,
+                s->ctl, O, (const float *)s->kif, dt / 2, dt, ct, j);
     } else if constexpr (STAGE == 0) {
         D::iface_rhs<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, O, s->kif);
     }
@@ -249,7 +284,7 @@ void enqueue_steps(msfdm_solver *s, int it0, int n) {
         if ((it0 + j) & 1) enqueue_step_dir<1>(s, j, j == 0, j == n - 1);
         else enqueue_step_dir<0>(s, j, j == 0, j == n - 1);
     }
-    D::advance_base<<<1, 64, 0, s->stream>>>(s->base, n);
+    klaunch(s, KCLASS_OTHER, D::advance_base, dim3(1), dim3(64), s->base, n);
 }
 
 int build_graph(msfdm_solver *s) {
@@ -345,6 +380,8 @@ void free_adjoint(msfdm_solver *s) {
 
 void free_all(msfdm_solver *s) {
     drop_graph(s);
+    for (hipEvent_t e : s->prof.ev) (void)hipEventDestroy(e);
+    s->prof.ev.clear();
     free_adjoint(s);
     for (auto &b : s->buf) if (b) { (void)hipFree(b); b = nullptr; }
     for (void *p : {(void *)s->kbuf, (void *)s->kif, (void *)s->coefmem, (void *)s->amp, (void *)s->base, (void *)s->rec})
@@ -802,6 +839,56 @@ extern "C" int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = t;
+    return MSFDM_OK;
+}
+
+// The launch sequence of msfdm_run (GRAPH_STEPS-step sequences where msfdm_run
+// replays its graph, single steps at odd step indices) with direct launches,
+// each stamped by its own dispatch.  kernel_ms[c] / launches[c]: total
+// milliseconds and launch count of kernel class c (RK stages 1..4, interface
+// finish, other); wall_ms: the whole run between two stream events.
+extern "C" int msfdm_profile_run(msfdm_solver *s, int nsteps, float *kernel_ms, int *launches, float *wall_ms) {
+    clear_error();
+    if (!s || nsteps < 0) return set_error(MSFDM_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(s->device));
+    if (int rc = grow_tables(s, s->host_it + nsteps)) return rc;
+    hipEvent_t w0, w1;
+    HIPCHK(hipEventCreate(&w0));
+    HIPCHK(hipEventCreate(&w1));
+    s->prof.cls.clear();
+    s->prof.on = true;
+    (void)hipEventRecord(w0, s->stream);
+    int left = nsteps;
+    while (left > 0) {
+        int n = std::min(left, GRAPH_STEPS);
+        if (s->use_graph) n = (s->host_it & 1) ? 1 : (left >= GRAPH_STEPS ? GRAPH_STEPS : std::min(left, GRAPH_STEPS));
+        enqueue_steps(s, s->host_it, n);
+        s->host_it += n;
+        left -= n;
+    }
+    (void)hipEventRecord(w1, s->stream);
+    s->prof.on = false;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(w1));
+    float wall = 0.f;
+    HIPCHK(hipEventElapsedTime(&wall, w0, w1));
+    (void)hipEventDestroy(w0);
+    (void)hipEventDestroy(w1);
+    double tot[NKCLASS] = {0, 0, 0, 0, 0, 0};
+    int cnt[NKCLASS] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = 0; i < s->prof.cls.size(); ++i) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, s->prof.ev[2 * i], s->prof.ev[2 * i + 1]));
+        tot[s->prof.cls[i]] += t;
+        cnt[s->prof.cls[i]] += 1;
+    }
+    for (int c = 0; c < NKCLASS; ++c) {
+        if (kernel_ms) kernel_ms[c] = (float)tot[c];
+        if (launches) launches[c] = cnt[c];
+    }
+    if (wall_ms) *wall_ms = wall;
+    s->nrec_done = s->host_it > 0 ? (s->host_it - 1) / s->gap + 1 : 0;
+    if (s->solid_only && s->ctl.rec_row < 0) s->nrec_done = 0;
     return MSFDM_OK;
 }
 

@@ -151,43 +151,31 @@ __device__ __forceinline__ float mac_dz(const float (*t)[LXP], int r, int c, int
 // floats per field and step; the rounding of U' is no larger than that of the
 // accumulator form.  s is the stage state at the cell (U in stage 1, T4 in
 // stage 4); STAGE 0 writes the raw k.
-#ifndef MSFDM_NT_STORE
-#define MSFDM_NT_STORE 0   // 1: stage outputs bypass the caches (non-temporal stores)
-#endif
-__device__ __forceinline__ void store_out(float *p, float v) {
-#if MSFDM_NT_STORE
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
+// Stage 4 writes U' over U^n (O == U): each cell of U is read by the thread that
+// writes it, before the write, and by no other thread of the launch, so U and O
+// carry no __restrict__.
 template <int STAGE>
-__device__ __forceinline__ void rk(const float *__restrict__ U, const float *__restrict__ T2,
-                                   const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
-                                   float k, float ct) {
+__device__ __forceinline__ void rk(const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
+                                   float *O, long long i, float s, float k, float ct) {
     if constexpr (STAGE == 0) {
         O[i] = k;
     } else if constexpr (STAGE == 1) {
-        store_out(O + i, s + ct * k);
+        O[i] = s + ct * k;
     } else if constexpr (STAGE == 4) {
         const float u = U[i];
-        store_out(O + i, u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k);
+        O[i] = u + ((T2[i] - u) + 2.0f * (T3[i] - u) + (s - u)) * (1.0f / 3.0f) + ct * k;
     } else {
-        store_out(O + i, U[i] + ct * k);
+        O[i] = U[i] + ct * k;
     }
 }
 
-// rk with U[i] already in a register (loaded ahead of the row's arithmetic);
-// stage 4 loads it here unless MSFDM_PREFETCH_U4.
-#ifndef MSFDM_PREFETCH_U4
-#define MSFDM_PREFETCH_U4 0
-#endif
+// rk with U[i] already in a register (loaded ahead of the row's arithmetic in
+// stages 2 and 3; stage 4 loads it here).
 template <int STAGE>
-__device__ __forceinline__ constexpr bool prefetch_u() { return STAGE == 2 || STAGE == 3 || (STAGE == 4 && MSFDM_PREFETCH_U4); }
+__device__ __forceinline__ constexpr bool prefetch_u() { return STAGE == 2 || STAGE == 3; }
 template <int STAGE>
-__device__ __forceinline__ float rk_u(const float *__restrict__ U, const float *__restrict__ T2,
-                                      const float *__restrict__ T3, float *__restrict__ O, long long i, float s,
-                                      float k, float ct, float u) {
+__device__ __forceinline__ float rk_u(const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
+                                      float *O, long long i, float s, float k, float ct, float u) {
     float v;
     if constexpr (STAGE == 0) {
         v = k;
@@ -201,7 +189,7 @@ __device__ __forceinline__ float rk_u(const float *__restrict__ U, const float *
     } else {
         v = u + ct * k;
     }
-    store_out(O + i, v);
+    O[i] = v;
     return v;
 }
 
@@ -217,7 +205,7 @@ struct RowPre {
 
 template <int STAGE, int MET, int NF, int NC>
 __device__ __forceinline__ void prefetch_row(RowPre<NF, NC> &p, const Coef &c, const float *const (&cp)[NC],
-                                             const float *__restrict__ U, const long long *off, int f0,
+                                             const float *U, const long long *off, int f0,
                                              long long ai, long long gi) {
     xi_metrics<MET>(c, gi, p.xx, p.xz);
     p.zx = c.zex[gi];
@@ -451,10 +439,12 @@ __device__ __forceinline__ void iface_record(const Ctl &ctl, int it, int nx, int
 // step's side-buffer sets (iface_pre on the 76 columns x0-6 .. x0+69, then
 // iface_filter); the tile that owns the row (wb) stores its 64 columns of fields
 // f0 .. f0+NF-1 to U^n, and the solid one writes the previous step's record.
+// As in rebuild_interface_row, other blocks may load the row before or after
+// that store and replace it in LDS with the same bits they compute themselves.
 constexpr int FW = TX + 4 * HALO;   // pre-filter columns of a tile
 template <int DIR, int NF, int TZ>
 __device__ __forceinline__ void finish_interface_row(float (*sh)[TZ + 2 * HALO][LXP], float (*pre)[FW], const Geo &g,
-                                                     const Coef &c, const Ctl &ctl, const float *__restrict__ S,
+                                                     const Coef &c, const Ctl &ctl, const float *S,
                                                      const float *__restrict__ kif, int lrow, int f0, int x0, bool wb,
                                                      float dt, int jstep) {
     const int nx = g.nx;
@@ -497,10 +487,14 @@ __device__ __forceinline__ void finish_interface_row(float (*sh)[TZ + 2 * HALO][
 // buffers are complete for later readers (the RK4 combination of stage 4 uses
 // the same values through iface_final, the adjoint reads the stored states).
 // Other blocks of the same launch may load that row before or after the store;
-// they overwrite it in LDS either way.
+// they overwrite it in LDS either way.  This is race-free only because every
+// block rebuilds bit-identical values from the same side-buffer inputs with the
+// same arithmetic (interface_bc is compiled without contraction): a block that
+// loads the stored row and one that loads the stale row end with the same LDS
+// contents.  The bitwise chunked-run tests guard this.
 template <int DIR, int STAGE, int NF, int TZ>
 __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
-                                                      const float *__restrict__ S, const float *__restrict__ kif,
+                                                      const float *S, const float *__restrict__ kif,
                                                       int lrow, int f0, int x0, float cprev, bool wb) {
     const long long P = g.P;
     constexpr int par = step_par<DIR, STAGE>();
@@ -543,8 +537,8 @@ __device__ __forceinline__ int fluid_z0(const Geo &g, int band, int TZ) {
 template <int DIR, int STAGE, int TZ, int MET, int FT = 0, int FIN = 0, int INT = 0>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
-           const float *__restrict__ S, const float *__restrict__ U, const float *__restrict__ T2,
-           const float *__restrict__ T3, float *__restrict__ O, float *__restrict__ kif, int jstep,
+           const float *S, const float *U, const float *__restrict__ T2,
+           const float *__restrict__ T3, float *O, float *__restrict__ kif, int jstep,
            float ct, float amp_direct, float cprev, int bx, int bz, float (*fin)[FW] = nullptr) {
     // sh[0..2] = vx, vz, p of the stage state
     // a wave holds one row of the tile at a time: ty (and every row test) is
@@ -778,9 +772,9 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2, TOP_IFACE_CPML = 3;
 template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE, int FIN = 0, int INT = 0>
 __device__ __forceinline__ void
-solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
-           const float *__restrict__ U, const float *__restrict__ T2, const float *__restrict__ T3,
-           float *__restrict__ O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
+solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *S,
+           const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
+           float *O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
            int bz, float (*fin)[FW] = nullptr) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     // a wave holds one row of the tile at a time: ty (and every row test) is
@@ -1082,9 +1076,6 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 // solid tiles, each as a contiguous run of row bands, so every XCD finds the
 // halo rows of its neighbours in its own L2 and gets the same mix of cheap,
 // dear and slow tiles.
-#ifndef MSFDM_STAGE_WAVES
-#define MSFDM_STAGE_WAVES 0   // minimum waves per SIMD requested for stage_kernel (0 = compiler choice)
-#endif
 // Tile (bx, band) of block b, band -1 past the end of a run; bands 0..nbf-1
 // are fluid, nbf..nbf+nbs-1 solid.
 __device__ __forceinline__ int2 xcd_tile(int b, int nbx, int nbf, int nbs, int nfi) {
@@ -1133,9 +1124,9 @@ __device__ __forceinline__ bool solid_interior(const Geo &g, int x0, int z0, int
 // FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
 // and record (finish_interface_row); cprev is then the time step.
 template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
-stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
-             const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O,
+__global__ void __launch_bounds__(NTHR)
+stage_kernel(Geo g, Coef c, Ctl ctl, const float *S, const float *U,
+             const float *__restrict__ T2, const float *__restrict__ T3, float *O,
              float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
              int nbz_s, int nfi) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
@@ -1169,9 +1160,9 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *_
 // of rows 3..nzs-4, the same XCD-aware block order.  Stage 4 also records vz
 // of the receiver row.
 template <int DIR, int STAGE, int TZ, int MET, int TOP>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
-free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, const float *__restrict__ U,
-                  const float *__restrict__ T2, const float *__restrict__ T3, float *__restrict__ O, int jstep,
+__global__ void __launch_bounds__(NTHR)
+free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *S, const float *U,
+                  const float *__restrict__ T2, const float *__restrict__ T3, float *O, int jstep,
                   float ct, float amp_direct, int nbx, int nb) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
     // band 0 (the surface rows, the dearest tiles) spread over the XCDs and
@@ -1266,7 +1257,7 @@ struct Shot {
 
 // tiles[t] = (shot, band, bx, fluid bands of that shot); FIN as in stage_kernel.
 template <int DIR, int STAGE, int TZ, int FIN = 0>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MSFDM_STAGE_WAVES)))
+__global__ void __launch_bounds__(NTHR)
 batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
                    int jstep, float ct, float cprev) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");

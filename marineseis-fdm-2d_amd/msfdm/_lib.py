@@ -39,7 +39,7 @@ EXPORTS = [
     "msfdm_synchronize", "msfdm_step_index", "msfdm_num_records", "msfdm_get_records",
     "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
     "msfdm_set_tile_rows", "msfdm_get_tile_rows",
-    "msfdm_time_kernel", "msfdm_time_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
+    "msfdm_time_kernel", "msfdm_time_run", "msfdm_profile_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
     "msfdm_run_cli", "msfdm_autotune", "msfdm_batch_create", "msfdm_batch_destroy",
     "msfdm_batch_reset", "msfdm_batch_run", "msfdm_batch_synchronize", "msfdm_batch_size",
     "msfdm_batch_shot", "msfdm_batch_set_tile_rows", "msfdm_batch_get_tile_rows",
@@ -104,6 +104,7 @@ def lib():
         "msfdm_get_tile_rows": ([vp, ip], i),
         "msfdm_time_kernel": ([vp, i, i, fp], i),
         "msfdm_time_run": ([vp, i, fp], i),
+        "msfdm_profile_run": ([vp, i, fp, ip, fp], i),
         "msfdm_bytes_per_step": ([vp, C.POINTER(C.c_double)], i),
         "msfdm_kernel_bytes": ([vp, i, C.POINTER(C.c_double)], i),
         "msfdm_run_cli": ([C.c_char_p, C.c_char_p], i),
@@ -156,6 +157,31 @@ def _f32(a):
 
 def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def record_gap(dt: float) -> int:
+    """Record cadence in steps, (int)(1e-3 / dt + 0.5) with dt as float32 (src/main.cu:92)."""
+    return max(1, int(1e-3 / float(np.float32(dt)) + 0.5))
+
+
+def num_records(nsteps: int, dt: float) -> int:
+    """Records taken by an nsteps-step run (steps 0, gap, 2 gap, ... < nsteps)."""
+    return (int(nsteps) - 1) // record_gap(dt) + 1 if nsteps > 0 else 0
+
+
+def check_residuals(res, nsteps: int, dt: float, nx: int):
+    """The three record residuals (vx, vz, p) as float32 [nrec][nx] arrays, nrec
+    the record count of an nsteps-step run; any other shape raises ValueError
+    (the C side reads nrec * nx floats of each plane)."""
+    want = (num_records(nsteps, dt), int(nx))
+    if len(res) != 3:
+        raise ValueError(f"three residual planes (vx, vz, p) expected, got {len(res)}")
+    rs = [_f32(r) for r in res]
+    for name, r in zip(("vx", "vz", "p"), rs):
+        if r.shape != want:
+            raise ValueError(f"residual {name}: shape {r.shape}, expected {want} "
+                             f"(records of a {nsteps}-step run over every grid column)")
+    return rs
 
 
 def grid_metrics(x, z, device: int = 0) -> dict:
@@ -236,6 +262,10 @@ class Solver:
         self.nx, self.nzf, self.nzs = d.nx, (0 if self.solid_only else d.nz_fluid), d.nz_solid
         self.nt, self.dt = d.nt, d.dt
         self._owner = _owner      # a Batch owns the handles of its shots
+        # adjoint state: the checkpoints of the last adjoint_forward (token bumped
+        # by every call that replaces them or the medium they were taken in)
+        self._adj_token = 0
+        self._adj_nsteps = 0
         if _handle is not None:
             self.h = _handle
             return
@@ -330,18 +360,35 @@ class Solver:
         unknown = set(keep) - set(GRADS)
         if unknown:
             raise KeyError(f"unknown medium arrays {sorted(unknown)}")
+        for n, v in keep.items():
+            want = ((self.nzf if GRADS.index(n) < 2 else self.nzs), self.nx)
+            if v.shape != want:
+                raise ValueError(f"{n}: shape {v.shape}, expected {want}")
         ptrs = [(_fptr(keep[n]) if n in keep else None) for n in GRADS]
+        self._adj_token += 1
         check(lib().msfdm_update_media(self.h, *ptrs))
 
     # --- adjoint ------------------------------------------------------------
     def _grad_arrays(self):
         return {n: np.zeros(((self.nzf if k < 2 else self.nzs), self.nx)) for k, n in enumerate(GRADS)}
 
+    def _residuals(self, res, nsteps: int):
+        return check_residuals(res, nsteps, self.dt, self.nx)
+
+    @property
+    def adjoint_token(self) -> int:
+        """Changes whenever the adjoint checkpoints or the medium are replaced."""
+        return self._adj_token
+
     def adjoint_forward(self, isrc: int, nsteps: int, every: int = 0):
+        self._adj_token += 1
         check(lib().msfdm_adjoint_forward(self.h, int(isrc), int(nsteps), int(every)))
+        self._adj_nsteps = int(nsteps)
 
     def adjoint_backward(self, res):
-        rs = [_f32(r) for r in res]
+        if self._adj_nsteps <= 0:
+            raise RuntimeError("adjoint_backward needs a preceding adjoint_forward")
+        rs = self._residuals(res, self._adj_nsteps)
         g = self._grad_arrays()
         dp = C.POINTER(C.c_double)
         check(lib().msfdm_adjoint_backward(self.h, *[_fptr(r) for r in rs],
@@ -350,8 +397,10 @@ class Solver:
 
     def gradient(self, isrc: int, nsteps: int, res, every: int = 0):
         """Gradient of sum_r <res_r, rec_r> w.r.t. the model arrays (dict of float64)."""
-        rs = [_f32(r) for r in res]
+        rs = self._residuals(res, nsteps)
         g = self._grad_arrays()
+        self._adj_token += 1
+        self._adj_nsteps = int(nsteps)
         dp = C.POINTER(C.c_double)
         check(lib().msfdm_gradient(self.h, int(isrc), int(nsteps), int(every), *[_fptr(r) for r in rs],
                                    *[g[n].ctypes.data_as(dp) for n in GRADS]))
@@ -362,9 +411,10 @@ class Solver:
         of a wave stepped together.  res[i] = (res_vx, res_vz, res_p) of shot i, each
         [nrec][nx].  Returns (dict of float64 gradients, shots per wave)."""
         isrc = np.ascontiguousarray(shots, dtype=np.int32)
-        r = np.ascontiguousarray(np.stack([np.stack([_f32(a) for a in rr]) for rr in res]), dtype=np.float32)
-        if r.shape[0] != isrc.size:
+        if len(res) != isrc.size:
             raise ValueError("one residual triple per shot")
+        r = np.ascontiguousarray(np.stack([np.stack(self._residuals(rr, nsteps)) for rr in res]), dtype=np.float32)
+        self._adj_token += 1
         g = self._grad_arrays()
         dp = C.POINTER(C.c_double)
         wave = C.c_int()
@@ -402,6 +452,16 @@ class Solver:
         v = C.c_float()
         check(lib().msfdm_time_run(self.h, int(nsteps), C.byref(v)))
         return v.value
+
+    def profile_run(self, nsteps: int) -> dict:
+        """Step nsteps as run() does, every launch timed by its own dispatch:
+        {"kernel_ms": [6] totals (RK stages 1-4, interface finish, other),
+        "launches": [6] counts, "wall_ms": the whole run}."""
+        ms = (C.c_float * 6)()
+        n = (C.c_int * 6)()
+        wall = C.c_float()
+        check(lib().msfdm_profile_run(self.h, int(nsteps), ms, n, C.byref(wall)))
+        return {"kernel_ms": list(ms), "launches": list(n), "wall_ms": wall.value}
 
     def kernel_bytes(self, which: int) -> float:
         v = C.c_double()

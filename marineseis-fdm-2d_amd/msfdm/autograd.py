@@ -21,6 +21,12 @@ _ORDER = GRADS   # K, rho_f, C11, C13, C33, C55, rho_s
 
 
 class SeismogramFunction(torch.autograd.Function):
+    """forward: checkpointed run of shot isrc in the given medium -> records;
+    backward: the reverse sweep.  One solver may serve several nodes (shots,
+    media) of a graph: each node keeps its shot, step count and medium, and its
+    backward first redoes its own checkpointed forward when the solver's
+    checkpoints or medium have been replaced since (Solver.adjoint_token)."""
+
     @staticmethod
     def forward(ctx, solver: Solver, isrc: int, nsteps: int, every: int, *media):
         arrays = {n: m.detach().cpu().numpy().astype(np.float32) for n, m in zip(_ORDER, media)}
@@ -28,6 +34,9 @@ class SeismogramFunction(torch.autograd.Function):
         solver.adjoint_forward(isrc, nsteps, every)
         recs = solver.records()
         ctx.solver = solver
+        ctx.args = (int(isrc), int(nsteps), int(every))
+        ctx.arrays = arrays
+        ctx.token = solver.adjoint_token
         ctx.dtypes = [m.dtype for m in media]
         ctx.devices = [m.device for m in media]
         dev = media[0].device
@@ -36,7 +45,12 @@ class SeismogramFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_vx, g_vz, g_p):
         res = [g.detach().cpu().numpy().astype(np.float32) for g in (g_vx, g_vz, g_p)]
-        grads = ctx.solver.adjoint_backward(res)
+        solver = ctx.solver
+        if solver.adjoint_token != ctx.token:   # another node (or call) replaced this node's forward
+            solver.update_media(**ctx.arrays)
+            solver.adjoint_forward(*ctx.args)
+            ctx.token = solver.adjoint_token
+        grads = solver.adjoint_backward(res)
         out = [torch.from_numpy(grads[n]).to(device=d, dtype=t) for n, d, t in zip(_ORDER, ctx.devices, ctx.dtypes)]
         return (None, None, None, None, *out)
 

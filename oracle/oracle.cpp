@@ -135,6 +135,11 @@ struct Oracle {
     int it = 0;
     // state sets (reference MemManager: _d, _b, _tmp, _k)
     std::vector<double> U[NFIELD], Bacc[NFIELD], T[NFIELD], K[NFIELD];
+    // Cells of each field the RHS can write, as [begin, end) runs along rows:
+    // every cell of a state field, the strip cells of a PML memory field (its k
+    // is written nowhere else and stays 0 outside, so the RK combination there
+    // leaves U unchanged and is skipped).
+    std::vector<std::pair<size_t, size_t>> runs[NFIELD];
     // ghost fluxes [3][nx] x/z components (src/pml_fs_kernels.cu:385-426)
     std::vector<double> Gfx, Gfz, Gsx, Gsz, Gfsx, Gfsz;
     // records
@@ -518,9 +523,21 @@ void fl_so_boundary(Oracle &o, double *const *Kd, int dir) {
     }
 }
 
+// Fields a grid never drives: the fluid zeta memory outside the fluid-solid
+// CPML grid (src/elements.cu:866-873: no fluid zeta PML with the
+// pressure-release surface).  Their k, stage values and state stay 0, so the
+// stepper skips them.
+inline bool driven(const Oracle &o, int f) { return o.fsize(f) > 0 && (f < FZVX || o.fcpml()); }
+void zero_runs(const Oracle &o, int f, double *a) {
+    const auto &r = o.runs[f];
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < r.size(); ++q)
+        for (size_t i = r[q].first; i < r[q].second; ++i) a[i] = 0.0;
+}
+
 // linear_operate + fl_so_apply (src/pml_fs_kernels.cu:533-660): k = BC(L(S) + src).
 void rhs(Oracle &o, double *const *S, double *const *Kd, int dir, double amp) {
-    for (int f = 0; f < NFIELD; ++f) std::fill(Kd[f], Kd[f] + o.fsize(f), 0.0);
+    for (int f = 0; f < NFIELD; ++f) zero_runs(o, f, Kd[f]);
     build_ghosts(o, S);
     velocity_fluid(o, S, Kd, dir);
     stress_fluid(o, S, Kd, dir);
@@ -758,7 +775,7 @@ void add_source_solid(Oracle &o, double *const *Kd, double amp) {
 // linear_operate (src/pml_fs_kernels.cu:29-103, src/cpml_kernels.cu,
 // src/dirichlet_kernels.cu:12-56): k = L(S) + src.
 void rhs_free(Oracle &o, double *const *S, double *const *Kd, int dir, double amp) {
-    for (int f = 0; f < NFIELD; ++f) std::fill(Kd[f], Kd[f] + o.fsize(f), 0.0);
+    for (int f = 0; f < NFIELD; ++f) zero_runs(o, f, Kd[f]);
     if (o.bdy == 2) build_ghosts_free(o, S);
     velocity_solid_free(o, S, Kd, dir);
     stress_solid_free(o, S, Kd, dir);
@@ -820,15 +837,66 @@ double stage_amp(const Oracle &o, int it, int stage) {
     return ricker(o, t);
 }
 
-void axpy(std::vector<double> &dst, const std::vector<double> &a, double c, const std::vector<double> &k) {
+// dst = a + c k, dst += c k, dst = a over the runs of field f.
+void axpy(const Oracle &o, int f, std::vector<double> &dst, const std::vector<double> &a, double c,
+          const std::vector<double> &k) {
+    const auto &r = o.runs[f];
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < r.size(); ++q)
+        for (size_t i = r[q].first; i < r[q].second; ++i) dst[i] = a[i] + c * k[i];
+}
+void self_axpy(const Oracle &o, int f, std::vector<double> &dst, double c, const std::vector<double> &k) {
+    const auto &r = o.runs[f];
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < r.size(); ++q)
+        for (size_t i = r[q].first; i < r[q].second; ++i) dst[i] += c * k[i];
+}
+void axpy_all(std::vector<double> &dst, const std::vector<double> &a, double c, const std::vector<double> &k) {
     const size_t n = dst.size();
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; ++i) dst[i] = a[i] + c * k[i];
 }
-void self_axpy(std::vector<double> &dst, double c, const std::vector<double> &k) {
+void self_axpy_all(std::vector<double> &dst, double c, const std::vector<double> &k) {
     const size_t n = dst.size();
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; ++i) dst[i] += c * k[i];
+}
+void copy_runs(const Oracle &o, int f, std::vector<double> &dst, const std::vector<double> &a) {
+    const auto &r = o.runs[f];
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < r.size(); ++q)
+        for (size_t i = r[q].first; i < r[q].second; ++i) dst[i] = a[i];
+}
+
+// The runs of every field (see Oracle::runs): whole rows of the state fields;
+// the xi-strip columns of the xi memory (every row); the zeta-strip rows of the
+// zeta memory (every column).  Supersets of the cells the RHS functions write.
+void build_runs(Oracle &o) {
+    const size_t nx = o.nx;
+    for (int f = 0; f < NFIELD; ++f) {
+        auto &r = o.runs[f];
+        r.clear();
+        if (!driven(o, f)) continue;
+        const int rows = (int)(o.fsize(f) / nx);
+        const bool xi = (f >= FXVX && f <= FXEZ) || (f >= SXVX && f <= SXEZ);
+        const bool ze = (f >= SZVX && f <= SZEZ) || f >= FZVX;
+        for (int iz = 0; iz < rows; ++iz) {
+            const size_t b = (size_t)iz * nx;
+            if (!xi && !ze) { r.emplace_back(b, b + nx); continue; }
+            if (ze) {
+                const bool on = f >= FZVX ? (o.fcpml() && iz >= 3 && iz <= o.npd + 2) : o.zpml(iz);
+                if (on) r.emplace_back(b, b + nx);
+                continue;
+            }
+            for (int ix = 0; ix < o.nx;) {
+                if (!in_side(o, ix)) { ++ix; continue; }
+                int e = ix;
+                while (e < o.nx && in_side(o, e)) ++e;
+                r.emplace_back(b + ix, b + e);
+                ix = e;
+            }
+        }
+    }
 }
 
 // One time step it: RK4_reset (src/elements.cu:345-383), update_RK4
@@ -836,8 +904,8 @@ void self_axpy(std::vector<double> &dst, double c, const std::vector<double> &k)
 void step(Oracle &o) {
     double *S[NFIELD], *Kd[NFIELD], *Ud[NFIELD];
     for (int f = 0; f < NFIELD; ++f) {
-        o.Bacc[f] = o.U[f];
-        o.T[f] = o.U[f];
+        copy_runs(o, f, o.Bacc[f], o.U[f]);
+        copy_runs(o, f, o.T[f], o.U[f]);
         S[f] = o.T[f].data();
         Kd[f] = o.K[f].data();
         Ud[f] = o.U[f].data();
@@ -852,10 +920,10 @@ void step(Oracle &o) {
         else rhs(o, S, Kd, dirs[s], stage_amp(o, o.it, s));
         for (int f = 0; f < NFIELD; ++f) {
             if (s < 3) {
-                axpy(o.T[f], o.U[f], ct[s], o.K[f]);
-                self_axpy(o.Bacc[f], cb[s], o.K[f]);
+                axpy(o, f, o.T[f], o.U[f], ct[s], o.K[f]);
+                self_axpy(o, f, o.Bacc[f], cb[s], o.K[f]);
             } else {
-                axpy(o.U[f], o.Bacc[f], dt / 6, o.K[f]);
+                axpy(o, f, o.U[f], o.Bacc[f], dt / 6, o.K[f]);
             }
         }
     }
@@ -1300,8 +1368,8 @@ void step_keep(Oracle &o, Field20 &T2, Field20 &T3, Field20 &T4) {
         rhs(o, S, Kd, dirs[s], stage_amp(o, o.it, s));
         const double cb = s == 0 || s == 3 ? dt / 6 : dt / 3;
         for (int f = 0; f < NFIELD; ++f) {
-            if (s < 3) axpy((*outs[s])[f], o.U[f], ct[s], o.K[f]);
-            self_axpy(o.Bacc[f], cb, o.K[f]);
+            if (s < 3) axpy_all((*outs[s])[f], o.U[f], ct[s], o.K[f]);
+            self_axpy_all(o.Bacc[f], cb, o.K[f]);
         }
     }
     for (int f = 0; f < NFIELD; ++f) o.U[f] = o.Bacc[f];
@@ -1420,6 +1488,7 @@ void *orc_create(const orc_desc *d, const float *K, const float *rhof, const flo
     for (auto *g : {&o->Gfx, &o->Gfz, &o->Gsx, &o->Gsz, &o->Gfsx, &o->Gfsz}) g->assign(3 * (size_t)o->nx, 0.0);
     for (int f = 0; f < NFIELD; ++f) { o->Ain[f].assign(o->fsize(f), 0.0); o->Aout[f].assign(o->fsize(f), 0.0); }
     for (int g = 0; g < 7; ++g) o->G[g].assign((size_t)(g < 2 ? o->nzf : o->nzs) * o->nx, 0.0);
+    build_runs(*o);
     return o;
 }
 

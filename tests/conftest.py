@@ -11,12 +11,6 @@ for p in (PKG, os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 
-def pytest_addoption(parser):
-    parser.addoption("--full-length", action="store_true", default=False,
-                     help="also run the parity tests that step a full-size workload for its whole "
-                          "record length against the FP64 oracle (minutes of CPU time each)")
-
-
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libmsfdm.so")
 

@@ -72,3 +72,28 @@ def test_velocity_gradient_matches_central_differences():
         an = sum(float(np.sum(g[n] * dv)) for n, dv in zip(names, d))
         print(f"  fd {fd:.6e} adjoint {an:.6e}", flush=True)
         assert an == pytest.approx(fd, rel=2e-2)
+
+
+def test_two_shot_loss_through_one_solver_equals_sum_of_single_shot_gradients():
+    """Two nodes of one graph on one solver: the second forward replaces the
+    first one's checkpoints, so the first node's backward must redo its own
+    forward (shot 0) before its reverse sweep."""
+    m = _small(src_x=[24, 48])
+    s = msfdm.Solver(m)
+    rng = np.random.default_rng(12)
+    nrec = msfdm._lib.num_records(m.nt, m.dt)
+    res = [[rng.standard_normal((nrec, m.nx)) for _ in range(3)] for _ in range(2)]
+    media = {n: torch.tensor(m.arrays[n].astype(np.float64), requires_grad=True) for n in msfdm._lib.GRADS}
+    rec0 = ag.records(s, media, isrc=0, nsteps=m.nt)
+    rec1 = ag.records(s, media, isrc=1, nsteps=m.nt)
+    loss = sum(torch.sum(r.double() * torch.from_numpy(q)) for r, q in zip(rec0, res[0])) + \
+        sum(torch.sum(r.double() * torch.from_numpy(q)) for r, q in zip(rec1, res[1]))
+    loss.backward()
+    s2 = msfdm.Solver(m)
+    g0 = s2.gradient(0, m.nt, [q.astype(np.float32) for q in res[0]])
+    g1 = s2.gradient(1, m.nt, [q.astype(np.float32) for q in res[1]])
+    for n in msfdm._lib.GRADS:
+        want = g0[n] + g1[n]
+        got = media[n].grad.numpy()
+        assert np.linalg.norm(got - want) <= 1e-6 * np.linalg.norm(want), n
+        assert np.linalg.norm(g0[n]) > 0 and np.linalg.norm(g1[n]) > 0, n

@@ -313,12 +313,10 @@ def test_curved_seafloor_long_run_matches_oracle():
         assert e <= 10 * RECORD_TOL, f"{name}: rel L2 {e:.3e}"
 
 
-def test_seafloor2001_whole_record_matches_oracle(request):
+def test_seafloor2001_whole_record_matches_oracle():
     """The benchmark workload as the benchmark runs it: 2001x1001, all 6000 steps
-    (1500 records), against the FP64 oracle.  About 12e9 oracle cell updates, so
-    it runs only with --full-length."""
-    if not request.config.getoption("--full-length"):
-        pytest.skip("runs with --full-length (about 6 minutes of oracle time on 16 cores)")
+    (1500 records), against the FP64 oracle (about 12e9 oracle cell updates,
+    1.5-2 minutes on 16 host cores)."""
     m = modelgen.preset("seafloor2001")
     print(f"[records] seafloor2001 {m.nx}x{m.nz} {m.nt} steps", flush=True)
     s = msfdm.Solver(m)

@@ -250,12 +250,10 @@ def test_solid_source_footprint_across_interior_tiles(tile_rows):
         assert e <= 1e-4, (shot, e)
 
 
-def test_solid_topo2001_whole_record_matches_oracle(request):
+def test_solid_topo2001_whole_record_matches_oracle():
     """The solid-only benchmark workload as the benchmark runs it: 2001x1001
     under the sinusoidal free surface, all 6000 steps (1500 surface records),
-    against the FP64 oracle.  Runs only with --full-length."""
-    if not request.config.getoption("--full-length"):
-        pytest.skip("runs with --full-length (about 6 minutes of oracle time on 16 cores)")
+    against the FP64 oracle (1.5-2 minutes on 16 host cores)."""
     m = modelgen.preset("solid_topo2001")
     print(f"[solid records] solid_topo2001 {m.nx}x{m.nz} {m.nt} steps", flush=True)
     s = msfdm.Solver(m)

@@ -214,3 +214,66 @@ def test_random_field_statistics_and_determinism():
     for lag in (10, 20):
         r = np.corrcoef(big[:, :-lag].ravel(), big[:, lag:].ravel())[0, 1]
         assert abs(r - np.exp(-(lag / 20.0) ** 2)) < 0.06, lag
+
+
+def test_reference_json_template_loads(tmp_path, monkeypatch):
+    """The reference's own configuration template (json_template.json, kept
+    unchanged as tests/golden/json_template.json) loads through the C ABI:
+    every key lands where include/json_config.hpp puts it, the CWD-relative
+    <f4 files of the sizes it names are read in the reference layout."""
+    doc = json.load(open(os.path.join(ROOT, "tests", "golden", "json_template.json")))
+    g = doc["grid"]
+    nx, nzs, nzf = g["solid"]["nx"], g["solid"]["nz"], g["fluid"]["nz"]
+    nz = nzf + nzs - 1
+    rng = np.random.default_rng(3)
+    shapes = {**{k: (nzs, nx) for k in ("C11", "C13", "C33", "C55", "rho")}}
+    files = {}
+    for sec, rows in (("solid", nzs), ("fluid", nzf), ("jacob", nz)):
+        for key, rel in g[sec].items():
+            if key == "nx" or key == "nz":
+                continue
+            a = rng.uniform(1.0, 2.0, (rows, nx)).astype("<f4")
+            path = tmp_path / rel
+            path.parent.mkdir(parents=True, exist_ok=True)
+            a.tofile(path)
+            files[(sec, key)] = a
+    jpath = tmp_path / "json_template.json"
+    json.dump(doc, open(jpath, "w"))
+    monkeypatch.chdir(tmp_path)
+    hm = msfdm.HostModel.from_json(str(jpath))
+    d = hm.desc()
+    assert (d.nx, d.nz_fluid, d.nz_solid, d.calc_type, d.boundary_type) == (501, 201, 381, 1, 2)
+    assert (d.nt, d.snapshot, d.npd, d.npow) == (30001, 200, 20, 2)
+    for got, want in ((d.fpeak, 30.0), (d.dt, 1e-4), (d.thickness, 47.5), (d.vp_max, 3200.0), (d.rc, 1e-3),
+                      (d.beta0, 5.0)):
+        assert got == np.float32(want)
+    assert (d.nsrc, d.src_x[0], d.src_z[0]) == (1, 250, 60)
+    names = {("solid", "C11"): "C11", ("solid", "C13"): "C13", ("solid", "C33"): "C33", ("solid", "C55"): "C55",
+             ("solid", "rho"): "rho_s", ("fluid", "K"): "K", ("fluid", "rho"): "rho_f", ("jacob", "J"): "J",
+             ("jacob", "xi_x"): "xi_x", ("jacob", "xi_z"): "xi_z", ("jacob", "ze_x"): "ze_x", ("jacob", "ze_z"): "ze_z"}
+    for k, name in names.items():
+        assert np.array_equal(hm.array(name), files[k]), name
+    hm.close()
+
+
+def test_record_count_follows_the_oracle_cadence():
+    import pyoracle
+    for dt, nt in ((5e-4, 2000), (2.5e-4, 6000), (2e-4, 8000), (4e-4, 37), (1e-4, 30001), (3e-3, 10)):
+        m = modelgen.build_model(nx=40, nz=40, h=5.0, water_depth=60.0, npd=5, fpeak=20.0, dt=dt, nt=nt, src_x=[20])
+        o = pyoracle.Oracle(m)
+        assert msfdm._lib.record_gap(dt) == o.gap, dt
+        assert msfdm._lib.num_records(nt, dt) == (nt - 1) // o.gap + 1
+
+
+def test_residual_shapes_are_checked():
+    """A residual that is not [nrec][nx] of the run (a PML-cut gather, a shorter
+    record, two planes) is refused before any device call."""
+    check = msfdm._lib.check_residuals
+    nrec = msfdm._lib.num_records(600, 5e-4)
+    assert nrec == 300
+    ok = [np.zeros((nrec, 201), np.float64) for _ in range(3)]
+    out = check(ok, 600, 5e-4, 201)
+    assert all(r.dtype == np.float32 and r.shape == (nrec, 201) for r in out)
+    for bad in ([np.zeros((nrec, 161))] * 3, [np.zeros((nrec - 1, 201))] * 3, ok[:2], [np.zeros(nrec * 201)] * 3):
+        with pytest.raises(ValueError):
+            check(bad, 600, 5e-4, 201)
