@@ -8,13 +8,16 @@
 //                     k[i] = sum_j a_ij S[j] is run in reverse,
 //                     Sbar[j] += a_ij qbar[i]; stencil contributions are
 //                     gathered in an LDS accumulator covering the tile and its
-//                     3-cell halo (ghost-flux reflections included) and flushed
-//                     with one global atomic add per cell (one LDS copy per
-//                     wave, x-direction taps gathered across lanes: no LDS
-//                     atomics); PML memory terms
-//                     and the model-gradient terms dk/dm . qbar (evaluated on
-//                     the forward stage state S) are pointwise and need no
-//                     atomics.  Gradients accumulate in FP64.
+//                     3-cell halo (ghost-flux reflections included) and added
+//                     to the output (one LDS copy per wave, x-direction taps
+//                     gathered across lanes: no LDS atomics).  The tiles run in
+//                     four launches, one per colour (tile column parity x band
+//                     parity): the accumulators of one colour never overlap, so
+//                     the flush is a plain read-modify-write and the colours
+//                     add in a fixed order -- repeated runs give the same bits.
+//                     PML memory terms and the model-gradient terms dk/dm . qbar
+//                     (evaluated on the forward stage state S) are pointwise.
+//                     Gradients accumulate in FP64.
 //   adj_filter        Filter^T on the interface rows (out of place, LDS).
 // The per-column 8x8 interface BC and filter maps are transposed by probing
 // the forward maps with unit vectors.
@@ -562,12 +565,13 @@ __device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, c
     }
 }
 
-// Zero the per-wave LDS accumulators, run the medium body, flush the tile and
-// its halo with one global atomic add per cell.  Wave w owns tile rows
-// w*RW .. w*RW+RW-1 and accumulates into its own copy covering rows
-// w*RW-3 .. w*RW+RW+2; no other wave touches the copy and each cell of it has
-// one writing lane, so every LDS update is a plain read-modify-write.  The
-// flush adds the overlapping copies in wave order.
+// Zero the per-wave LDS accumulators, run the medium body, add the tile and
+// its halo to the output.  Wave w owns tile rows w*RW .. w*RW+RW-1 and
+// accumulates into its own copy covering rows w*RW-3 .. w*RW+RW+2; no other
+// wave touches the copy and each cell of it has one writing lane, so every LDS
+// update is a plain read-modify-write.  The flush adds the overlapping copies
+// in wave order; no other block of the launch (one tile colour) covers the
+// same output cells.
 template <int NF, int TZ>
 __device__ __forceinline__ void acc_zero(WaveAcc<TZ> *acc) {
     constexpr int RW = TZ / 4, N = (RW + 2 * HALO) * LXW;
@@ -593,33 +597,63 @@ __device__ __forceinline__ void acc_flush(WaveAcc<TZ> *acc, float *TB, const lon
                 if (lr >= 0 && lr < RW + 2 * HALO) v += acc[w].v[f][lr][cc];
             }
             const int gz = z0 - HALO + r, gx = x0 - HALO + cc;
-            if (v != 0.0f && gz >= 0 && gz < nrows && gx >= 0 && gx < P) atomicAdd(dst + (long long)gz * P + gx, v);
+            if (v != 0.0f && gz >= 0 && gz < nrows && gx >= 0 && gx < P) dst[(long long)gz * P + gx] += v;
         }
     }
 }
 
+// Tiles of colour `color` (bit 0: tile column parity, bit 1: band parity within
+// its medium).  Two tiles of one colour are at least one tile apart in x (64
+// columns) and in z (TZ >= 8 rows), more than the 2 x 3-cell halos of their
+// accumulators.
+__host__ __device__ __forceinline__ int adj_color_tiles(int nbx, int nbz_f, int nbz_s, int color) {
+    const int px = color & 1, pz = color >> 1;
+    return ((nbx - px + 1) / 2) * ((nbz_f - pz + 1) / 2 + (nbz_s - pz + 1) / 2);
+}
+
 template <int DIR, int TZ>
 __global__ void __launch_bounds__(NTHR)
-adj_stage_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int ntiles) {
+adj_stage_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int nbz_s, int color) {
+    static_assert(TZ >= 2 * HALO, "tiles of one colour must not share accumulator cells");
     __shared__ WaveAcc<TZ> acc[4];
     const AdjIn a = adj_of_shot(a0, blockIdx.y);
     const Grad gr = grad_of_shot(gr0, blockIdx.y);
     const int per_xcd = gridDim.x >> 3;
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= ntiles) return;
-    const int band = tile / nbx, bx = tile - band * nbx;
-    if (band < nbz_f) {
+    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= adj_color_tiles(nbx, nbz_f, nbz_s, color)) return;
+    const int px = color & 1, pz = color >> 1;
+    const int nbxc = (nbx - px + 1) / 2, nfc = (nbz_f - pz + 1) / 2;
+    const int bc = t / nbxc, bx = 2 * (t - bc * nbxc) + px;
+    if (bc < nfc) {
+        const int band = 2 * bc + pz;
         acc_zero<3, TZ>(acc);
         __syncthreads();
         fluid_adj_body<DIR, TZ>(acc, g, c, a, gr, bx, band);
         __syncthreads();
         acc_flush<3, TZ>(acc, a.TB, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);
     } else {
+        const int band = 2 * (bc - nfc) + pz;
         acc_zero<5, TZ>(acc);
         __syncthreads();
-        solid_adj_body<DIR, TZ>(acc, g, c, a, gr, bx, band - nbz_f);
+        solid_adj_body<DIR, TZ>(acc, g, c, a, gr, bx, band);
         __syncthreads();
-        acc_flush<5, TZ>(acc, a.TB, g.off, MSFDM_SVX, (band - nbz_f) * TZ, bx * TX, g.nzs, g.P);
+        acc_flush<5, TZ>(acc, a.TB, g.off, MSFDM_SVX, band * TZ, bx * TX, g.nzs, g.P);
+    }
+}
+
+// Compensated (Kahan) FP64 accumulation of segment partial sums: total += part
+// with the running compensation comp, then part = 0.  Over a gradient of N
+// steps the partial sums of one checkpoint segment (about sqrt(N) steps, four
+// stages each) are plain FP64; the totals over the segments carry the
+// compensation, so their rounding does not grow with the number of segments.
+__global__ void kahan_add_kernel(double *__restrict__ total, double *__restrict__ comp, double *__restrict__ part,
+                                 long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double y = part[i] - comp[i];
+        const double t = total[i] + y;
+        comp[i] = (t - total[i]) - y;
+        total[i] = t;
+        part[i] = 0.0;
     }
 }
 

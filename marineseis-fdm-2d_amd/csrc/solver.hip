@@ -69,7 +69,9 @@ struct msfdm_solver {
     // adjoint state: lambda, lambda', accumulator, two Tbar buffers, interface qbar
     float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float *qif = nullptr;
-    double *gmem = nullptr;          // 7 gradient arrays, [rows][P] doubles
+    double *gmem = nullptr;          // 7 gradient arrays, [rows][P] doubles, three times:
+                                     // segment partial sums, Kahan totals, compensations
+    long long gfl = 0;               // doubles of one set of 7 arrays
     D::Grad grad{};
     std::vector<float *> ckpt;       // checkpoints of the full state every adj_every steps
     std::vector<float *> seg;        // per-step U^n, T2, T3, T4 of one segment
@@ -969,18 +971,28 @@ struct BatchView {
     int *base;
     float dt;
     hipStream_t stream;
+    bool vert;   // every shot has vertical grid columns: the MET = 1 kernels
 };
+bool all_vertical(const std::vector<msfdm_solver *> &shots) {
+    for (const msfdm_solver *s : shots)
+        if (!s->coef.vert) return false;
+    return true;
+}
 BatchView view_of(const msfdm_batch *b) {
     return BatchView{b->d_shots, {b->d_tiles[0], b->d_tiles[1]}, {b->ntiles[0], b->ntiles[1]}, b->tz,
-                     (int)b->shots.size(), b->maxnx, b->base, b->dt, b->stream};
+                     (int)b->shots.size(), b->maxnx, b->base, b->dt, b->stream, all_vertical(b->shots)};
 }
 
 template <int DIR, int STAGE, int TZ, int FIN>
 void batch_launch_media(const BatchView &b, int rS, int rO, int j, float ct, float cprev) {
     const int k = TZ == 8 ? 0 : 1;
     const int grid = 8 * ((b.ntiles[k] + 7) / 8);
-    D::batch_stage_kernel<DIR, STAGE, TZ, FIN><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k], rS, rO,
-                                                                                j, ct, cprev);
+    if (b.vert)
+        D::batch_stage_kernel<DIR, STAGE, TZ, 1, FIN><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k],
+                                                                                       rS, rO, j, ct, cprev);
+    else
+        D::batch_stage_kernel<DIR, STAGE, TZ, 2, FIN><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k],
+                                                                                       rS, rO, j, ct, cprev);
 }
 // As launch_stage: FIN, a stage 1 that finishes the previous step (cprev = dt);
 // fin_next, stage 4 leaves its interface rows to the next step's stage 1.
@@ -1266,8 +1278,9 @@ int ensure_adjoint(msfdm_solver *s) {
     if (!s->gmem) {
         const long long nF = (long long)s->d.nz_fluid * P, nS = (long long)s->d.nz_solid * P;
         const long long total = 2 * nF + 5 * nS;
-        HIPCHK(hipMalloc(&s->gmem, total * sizeof(double)));
-        HIPCHK(hipMemset(s->gmem, 0, total * sizeof(double)));
+        s->gfl = total;
+        HIPCHK(hipMalloc(&s->gmem, 3 * total * sizeof(double)));
+        HIPCHK(hipMemset(s->gmem, 0, 3 * total * sizeof(double)));
         double *p = s->gmem;
         s->grad.K = p; p += nF;
         s->grad.rho_f = p; p += nF;
@@ -1284,14 +1297,26 @@ int ensure_adjoint(msfdm_solver *s) {
 // The adjoint stage kernel always runs 8-row tiles (one LDS accumulator copy
 // per wave fits three blocks per CU; 16-row tiles would fit two).
 constexpr int ADJ_TZ = 8;
+// The four tile colours of an adjoint stage, one launch each, in a fixed order.
+template <int DIR>
+void launch_adj_colors(hipStream_t st, const D::Geo &g, const D::Coef &c, const D::AdjIn &a, const D::Grad &gr,
+                       int nx, int nzf, int nzs, int nshot) {
+    constexpr int TZ = ADJ_TZ;
+    const int nbx = (nx - 4) / D::TX + 1;
+    const int nbf = (nzf - 3 + TZ - 1) / TZ, nbs = (nzs - 3 + TZ - 1) / TZ;
+    for (int color = 0; color < 4; ++color) {
+        const int nt = D::adj_color_tiles(nbx, nbf, nbs, color);
+        D::adj_stage_kernel<DIR, TZ><<<dim3(8 * ((nt + 7) / 8), nshot), D::NTHR, 0, st>>>(g, c, a, gr, nbx, nbf, nbs,
+                                                                                          color);
+    }
+}
 template <int DIR>
 void launch_adj_media(msfdm_solver *s, const D::AdjIn &a) {
-    constexpr int TZ = ADJ_TZ;
-    const int nbx = (s->d.nx - 4) / D::TX + 1;
-    const int nbf = (s->d.nz_fluid - 3 + TZ - 1) / TZ, nbs = (s->d.nz_solid - 3 + TZ - 1) / TZ;
-    const int ntiles = nbx * (nbf + nbs);
-    D::adj_stage_kernel<DIR, TZ><<<8 * ((ntiles + 7) / 8), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, s->grad, nbx, nbf,
-                                                                                   ntiles);
+    launch_adj_colors<DIR>(s->stream, s->geo, s->coef, a, s->grad, s->d.nx, s->d.nz_fluid, s->d.nz_solid, 1);
+}
+// total += segment partial sums (Kahan), partial sums = 0; n doubles from p.
+void kahan_fold(hipStream_t st, double *part, long long n) {
+    D::kahan_add_kernel<<<(int)std::min<long long>((n + 255) / 256, 8192), 256, 0, st>>>(part + n, part + 2 * n, part, n);
 }
 
 // One adjoint stage: TB = L_dir^T B_dir^T (clam LP + cprev TBp) at forward state S.
@@ -1397,8 +1422,7 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
     HIPCHK(hipMemcpy(res + (size_t)nrec * nx, res_vz, (size_t)nrec * nx * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(res + 2 * (size_t)nrec * nx, res_p, (size_t)nrec * nx * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(s->adj[A_LAM], 0, bytes, s->stream));
-    const long long P = s->geo.P;
-    HIPCHK(hipMemsetAsync(s->gmem, 0, (2 * (long long)s->d.nz_fluid * P + 5 * (long long)s->d.nz_solid * P) * sizeof(double), s->stream));
+    HIPCHK(hipMemsetAsync(s->gmem, 0, 3 * s->gfl * sizeof(double), s->stream));
     int rc = MSFDM_OK;
     for (int seg = (int)s->ckpt.size() - 1; seg >= 0 && rc == MSFDM_OK; --seg) {
         const int it0 = seg * every, it1 = std::min(nsteps, it0 + every);
@@ -1425,6 +1449,7 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
             const int j = it - it0;
             rc = adj_step(s, it, s->seg[4 * j], s->seg[4 * j + 1], s->seg[4 * j + 2], s->seg[4 * j + 3]);
         }
+        kahan_fold(s->stream, s->gmem, s->gfl);   // this segment's sums into the compensated totals
     }
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(res);
@@ -1435,10 +1460,11 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
     D::set_base<<<1, 64, 0, s->stream>>>(s->base, nsteps);
     HIPCHK(hipStreamSynchronize(s->stream));
     const int nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
-    if ((rc = copy_grad_out(s, s->grad.K, nzf, gK)) || (rc = copy_grad_out(s, s->grad.rho_f, nzf, grho_f)) ||
-        (rc = copy_grad_out(s, s->grad.C11, nzs, gC11)) || (rc = copy_grad_out(s, s->grad.C13, nzs, gC13)) ||
-        (rc = copy_grad_out(s, s->grad.C33, nzs, gC33)) || (rc = copy_grad_out(s, s->grad.C55, nzs, gC55)) ||
-        (rc = copy_grad_out(s, s->grad.rho_s, nzs, grho_s)))
+    const long long T = s->gfl;   // the totals follow the partial sums
+    if ((rc = copy_grad_out(s, s->grad.K + T, nzf, gK)) || (rc = copy_grad_out(s, s->grad.rho_f + T, nzf, grho_f)) ||
+        (rc = copy_grad_out(s, s->grad.C11 + T, nzs, gC11)) || (rc = copy_grad_out(s, s->grad.C13 + T, nzs, gC13)) ||
+        (rc = copy_grad_out(s, s->grad.C33 + T, nzs, gC33)) || (rc = copy_grad_out(s, s->grad.C55 + T, nzs, gC55)) ||
+        (rc = copy_grad_out(s, s->grad.rho_s + T, nzs, grho_s)))
         return rc;
     return MSFDM_OK;
 }
@@ -1550,8 +1576,8 @@ ShotsPlan plan_shots(const msfdm_solver *s, int nsteps, int every) {
     p.every = every;
     p.nseg = (nsteps + every - 1) / every;
     p.bufs = 6 + p.nseg + 4LL * every;
-    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl) + sizeof(double) * (size_t)p.gfl +
-                 (size_t)every * sizeof(D::Shot);
+    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl) +
+                 3 * sizeof(double) * (size_t)p.gfl + (size_t)every * sizeof(D::Shot);
     p.shared = sizeof(double) * (size_t)p.gfl + ((size_t)64 << 20);
     return p;
 }
@@ -1563,7 +1589,8 @@ struct ShotsWave {
     ShotsPlan p;
     int kc = 0;                        // shots per wave (slab capacity)
     float *fmem = nullptr;             // all float slabs
-    double *gmem = nullptr;            // kc per-shot gradient sets, then the total
+    double *gmem = nullptr;            // kc per-shot gradient sets three times (segment partial sums,
+                                       // Kahan totals, compensations), then the sum over shots
     float *U = nullptr, *ckpt = nullptr, *seg = nullptr, *kif = nullptr, *qif = nullptr, *res = nullptr;
     float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double *grad = nullptr, *total = nullptr;
@@ -1587,10 +1614,10 @@ int wave_alloc(ShotsWave &w) {
     const long long nfl = kc * (p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl);
     if (hipMalloc(&w.fmem, nfl * sizeof(float)) != hipSuccess)
         return set_error(MSFDM_EDEVICE, "out of device memory for the multi-shot gradient slabs");
-    if (hipMalloc(&w.gmem, (kc + 1) * p.gfl * sizeof(double)) != hipSuccess)
+    if (hipMalloc(&w.gmem, (3 * kc + 1) * p.gfl * sizeof(double)) != hipSuccess)
         return set_error(MSFDM_EDEVICE, "out of device memory for the multi-shot gradients");
     HIPCHK(hipMemsetAsync(w.fmem, 0, nfl * sizeof(float), s->stream));
-    HIPCHK(hipMemsetAsync(w.gmem, 0, (kc + 1) * p.gfl * sizeof(double), s->stream));
+    HIPCHK(hipMemsetAsync(w.gmem, 0, (3 * kc + 1) * p.gfl * sizeof(double), s->stream));
     float *f = w.fmem;
     w.U = f; f += kc * p.bf;
     for (auto &a : w.adj) { a = f; f += kc * p.bf; }
@@ -1600,7 +1627,7 @@ int wave_alloc(ShotsWave &w) {
     w.qif = f; f += kc * 8 * p.P;
     w.res = f;
     w.grad = w.gmem;
-    w.total = w.gmem + kc * p.gfl;
+    w.total = w.gmem + 3 * kc * p.gfl;
     HIPCHK(hipMalloc(&w.tables, (size_t)p.every * kc * sizeof(D::Shot)));
     const int tz = s->tz;
     w.nbx = (s->d.nx - 4) / D::TX + 1;
@@ -1630,6 +1657,7 @@ BatchView wave_view(const ShotsWave &w, int kk, int slot) {
     v.base = w.s->base;
     v.dt = w.s->d.dt;
     v.stream = w.s->stream;
+    v.vert = w.s->coef.vert != 0;   // every shot of a wave shares the solver's grid
     return v;
 }
 
@@ -1658,9 +1686,7 @@ void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, floa
     a.qstride = 8 * w.p.P;
     (void)hipMemsetAsync(TB, 0, (size_t)kk * w.p.bf * sizeof(float), s->stream);
     D::adj_iface_q<DIR><<<dim3((s->d.nx + 127) / 128, kk), 128, 0, s->stream>>>(s->geo, s->coef, a, w.qif);
-    const int grid = 8 * ((w.nta_shot + 7) / 8);
-    D::adj_stage_kernel<DIR, ADJ_TZ><<<dim3(grid, kk), D::NTHR, 0, s->stream>>>(s->geo, s->coef, a, wave_grad(w), w.nbx,
-                                                                                 w.nbfa, w.nta_shot);
+    launch_adj_colors<DIR>(s->stream, s->geo, s->coef, a, wave_grad(w), s->d.nx, s->d.nz_fluid, s->d.nz_solid, kk);
 }
 
 // Reverse of step `it` for the kk shots of the wave (segment slot j holds the
@@ -1719,7 +1745,7 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
     HIPCHK(hipMemsetAsync(w.U, 0, sbytes, s->stream));
     HIPCHK(hipMemsetAsync(w.adj[W_LAM], 0, sbytes, s->stream));
     HIPCHK(hipMemsetAsync(w.kif, 0, (size_t)kk * D::KIF_ROWS * p.P * sizeof(float), s->stream));
-    HIPCHK(hipMemsetAsync(w.grad, 0, (size_t)kk * p.gfl * sizeof(double), s->stream));
+    HIPCHK(hipMemsetAsync(w.grad, 0, 3 * (size_t)w.kc * p.gfl * sizeof(double), s->stream));
     // checkpointed forward
     D::set_base<<<1, 64, 0, s->stream>>>(s->base, 0);
     for (int it = 0; it < nsteps; ++it) {
@@ -1744,10 +1770,13 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
                                                                                        it / gap, p.bf, p.rfl);
             wave_adj_step(w, kk, it, it - it0);
         }
+        // every shot's segment sums into its compensated totals
+        D::kahan_add_kernel<<<(int)std::min<long long>((kk * p.gfl + 255) / 256, 8192), 256, 0, s->stream>>>(
+            w.grad + w.kc * p.gfl, w.grad + 2 * w.kc * p.gfl, w.grad, kk * p.gfl);
         HIPCHK(hipGetLastError());
     }
-    D::grad_sum_kernel<<<(int)std::min<long long>((p.gfl + 255) / 256, 8192), 256, 0, s->stream>>>(w.total, w.grad, p.gfl,
-                                                                                                  p.gfl, kk);
+    D::grad_sum_kernel<<<(int)std::min<long long>((p.gfl + 255) / 256, 8192), 256, 0, s->stream>>>(
+        w.total, w.grad + w.kc * p.gfl, p.gfl, p.gfl, kk);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
 }

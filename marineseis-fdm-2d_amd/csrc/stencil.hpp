@@ -1256,7 +1256,9 @@ struct Shot {
 };
 
 // tiles[t] = (shot, band, bx, fluid bands of that shot); FIN as in stage_kernel.
-template <int DIR, int STAGE, int TZ, int FIN = 0>
+// MET as in stage_kernel: 1 when every shot of the launch has vertical grid
+// columns (xi_x a constant, xi_z = 0), 2 to decide per shot at run time.
+template <int DIR, int STAGE, int TZ, int MET, int FIN = 0>
 __global__ void __launch_bounds__(NTHR)
 batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
                    int jstep, float ct, float cprev) {
@@ -1273,19 +1275,19 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
     if (tl.y < tl.w) {
         const int z0 = fluid_z0(s.g, tl.y, TZ);
         if (fluid_interior<0>(s.g, tl.z * TX, z0, TZ) && clear_of_source(s.ctl, tl.z * TX, z0, TZ))
-            fluid_body<DIR, STAGE, TZ, 2, 0, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
-                                                     jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
+            fluid_body<DIR, STAGE, TZ, MET, 0, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
+                                                       jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
         else
-            fluid_body<DIR, STAGE, TZ, 2, 0, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
-                                                     jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
+            fluid_body<DIR, STAGE, TZ, MET, 0, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O, s.kif,
+                                                       jstep, ct, 0.0f, cprev, tl.z, tl.y, fin);
     } else {
         const int sb = tl.y - tl.w;
         if (solid_interior<TOP_IFACE>(s.g, tl.z * TX, sb * TZ, TZ))
-            solid_body<DIR, STAGE, TZ, 2, TOP_IFACE, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
-                                                             s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
+            solid_body<DIR, STAGE, TZ, MET, TOP_IFACE, FIN, 1>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
+                                                               s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
         else
-            solid_body<DIR, STAGE, TZ, 2, TOP_IFACE, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
-                                                             s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
+            solid_body<DIR, STAGE, TZ, MET, TOP_IFACE, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
+                                                               s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
     }
 }
 

@@ -121,6 +121,10 @@ def test_gradient_matches_oracle(curvi):
     gs2 = s.gradient(0, nsteps, res, every=0)
     for n in msfdm._lib.GRADS:
         assert rel_l2(gs2[n], gs[n]) <= 1e-6, n
+    # and a repeated gradient gives the same bits
+    gs3 = s.gradient(0, nsteps, res, every=8)
+    for n in msfdm._lib.GRADS:
+        assert np.array_equal(gs3[n], gs[n]), n
 
 
 def test_gradient_matches_gpu_central_differences():
