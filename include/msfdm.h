@@ -290,6 +290,23 @@ int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows);
 int msfdm_dispersion_image(const float *gather, int nt, int ntr, const double *offsets, double fs,
                            const double *velocities, int nc, int device, double *freqs, double *amp);
 
+/* Allocation-free variant of msfdm_dispersion_image.  No reference counterpart:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and bit-identical results, over caller-owned DEVICE buffers of the
+ * current HIP device: d_gather [nt][ntr] float, d_offsets [ntr] and
+ * d_velocities [nc] doubles, d_amp [nf][nc] doubles out (nf = nt / 2 + 1;
+ * the frequencies are f * fs / nt).  The twiddle table and the phase spectrum
+ * live in a static device buffer of MSFDM_DISP_SCRATCH_BYTES per GPU, held
+ * until the process ends; 256 * ceil(16 nt / 256) + 16 nf ntr bytes beyond it
+ * return MSFDM_EINVAL.  Work is queued on `stream` (a hipStream_t; NULL = the
+ * null stream) and the call returns without waiting.  Calls on one GPU share
+ * that buffer: calls that overlap in time on one GPU (other streams or threads)
+ * race.  Offsets and velocities are not validated (they stay on the device). */
+#define MSFDM_DISP_SCRATCH_BYTES (32u << 20)
+int msfdm_dispersion_image_noalloc(const float *d_gather, int nt, int ntr, const double *d_offsets, double fs,
+                                   const double *d_velocities, int nc, double *d_amp, void *stream);
+
 /* --- Process boundary ---------------------------------------------------- */
 
 /* The whole bin/main_debug run: argument guards, output-dir reset, shot loop,

@@ -10,11 +10,46 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "model.hpp"
 
+// Scratch of msfdm_dispersion_image_noalloc: the twiddle table (nt complex
+// doubles, padded to 256 bytes) followed by the phase spectrum (nf x ntr complex
+// doubles).  Each GPU the library runs on holds one copy until the process
+// ends.  In host code `scratch` names a host-side stand-in: its device address
+// on each GPU comes from hipGetSymbolAddress and is kept in scratch_dev.
+static __device__ __align__(256) unsigned char scratch[MSFDM_DISP_SCRATCH_BYTES];
+
 namespace {
+
+using msfdm::set_error;
+constexpr size_t DISP_SCRATCH_BYTES = MSFDM_DISP_SCRATCH_BYTES;
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned char *scratch_dev[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device,
+// with that device current).
+int scratch_of_current_device(unsigned char **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(MSFDM_EDEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return set_error(MSFDM_EDEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(MSFDM_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = scratch_dev[dev];
+    return MSFDM_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
 __global__ void twiddle_kernel(double2 *w, int n) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,5 +149,36 @@ extern "C" int msfdm_dispersion_image(const float *gather, int nt, int ntr, cons
         for (int f = 0; f < nf; ++f) freqs[f] = (double)f * fstep;
     (void)hipFree(dg); (void)hipFree(dw); (void)hipFree(dP);
     (void)hipFree(dx); (void)hipFree(dv); (void)hipFree(da);
+    return MSFDM_OK;
+}
+
+// The same three kernels over caller-owned device buffers, with the twiddle
+// table and the phase spectrum in the library's static scratch: no device
+// memory is allocated or freed.  Every intermediate cell is written before it is
+// read, so nothing in the scratch needs clearing between calls.
+extern "C" int msfdm_dispersion_image_noalloc(const float *d_gather, int nt, int ntr, const double *d_offsets,
+                                              double fs, const double *d_velocities, int nc, double *d_amp,
+                                              void *stream) {
+    clear_error();
+    if (!d_gather || !d_offsets || !d_velocities || !d_amp) return set_error(MSFDM_EINVAL, "null argument");
+    if (nt < 2 || ntr < 1 || nc < 1) return set_error(MSFDM_EINVAL, "need nt >= 2, ntr >= 1 and nc >= 1");
+    if (!(fs > 0.0) || !std::isfinite(fs)) return set_error(MSFDM_EINVAL, "fs must be a positive finite value");
+    const int nf = nt / 2 + 1;
+    const size_t tw_bytes = align256((size_t)nt * sizeof(double2));
+    const size_t need = tw_bytes + (size_t)nf * (size_t)ntr * sizeof(double2);
+    if (need > DISP_SCRATCH_BYTES)
+        return set_error(MSFDM_EINVAL, "gather too large for the allocation-free scratch: " + std::to_string(need) +
+                                           " bytes needed, " + std::to_string(DISP_SCRATCH_BYTES) + " available");
+    unsigned char *base = nullptr;
+    if (int rc = scratch_of_current_device(&base)) return rc;
+    double2 *dw = reinterpret_cast<double2 *>(base);
+    double2 *dP = reinterpret_cast<double2 *>(base + tw_bytes);
+    const double fstep = 1.0 / ((double)nt * (1.0 / fs));   // numpy.fft.rfftfreq spacing
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    twiddle_kernel<<<(nt + 255) / 256, 256, 0, st>>>(dw, nt);
+    phase_spectrum_kernel<<<dim3((ntr + 255) / 256, nf), 256, 0, st>>>(d_gather, nt, ntr, nf, dw, dP);
+    stack_kernel<<<dim3((nc + 255) / 256, nf), 256, 0, st>>>(dP, d_offsets, d_velocities, nf, ntr, nc, fstep, d_amp);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(MSFDM_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
     return MSFDM_OK;
 }

@@ -47,9 +47,12 @@ EXPORTS = [
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
     "msfdm_update_media", "msfdm_grid_metrics", "msfdm_gradient_shots", "msfdm_gradient_shots_plan",
-    "msfdm_dispersion_image",
+    "msfdm_dispersion_image", "msfdm_dispersion_image_noalloc",
 ]
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
+
+# Size of the static device scratch of msfdm_dispersion_image_noalloc (MSFDM_DISP_SCRATCH_BYTES).
+DISP_SCRATCH_BYTES = 32 << 20
 
 
 class MsfdmError(RuntimeError):
@@ -133,6 +136,7 @@ def lib():
         "msfdm_gradient_shots_plan": ([vp, i, i, C.POINTER(C.c_double), ip], i),
         "msfdm_dispersion_image": ([fp, i, i, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double), i, i,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)], i),
+        "msfdm_dispersion_image_noalloc": ([vp, i, i, vp, C.c_double, vp, i, vp, vp], i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
