@@ -42,6 +42,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "msfdm.h"
 
 namespace msfdm {
@@ -151,12 +153,17 @@ __device__ __forceinline__ float mac_dz(const float (*t)[LXP], int r, int c, int
 // floats per field and step; the rounding of U' is no larger than that of the
 // accumulator form.  s is the stage state at the cell (U in stage 1, T4 in
 // stage 4); STAGE 0 writes the raw k.
-// Stage 4 writes U' over U^n (O == U): each cell of U is read by the thread that
-// writes it, before the write, and by no other thread of the launch, so U and O
-// carry no __restrict__.
+// U^n (U) and the stage output (O) are distinct buffers in stages 0-3 and carry
+// __restrict__ there.  Stage 4 writes U' over U^n (O == U): each cell of U is
+// read by the thread that writes it, before the write, and by no other thread
+// of the launch; the pointers are plain, and the stage-4 cell update (rk4_cell)
+// loads every input of a cell before its first store.
+template <int STAGE> using UPtr = std::conditional_t<STAGE == 4, const float *, const float *__restrict__>;
+template <int STAGE> using OPtr = std::conditional_t<STAGE == 4, float *, float *__restrict__>;
+
 template <int STAGE>
-__device__ __forceinline__ void rk(const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
-                                   float *O, long long i, float s, float k, float ct) {
+__device__ __forceinline__ void rk(UPtr<STAGE> U, const float *__restrict__ T2, const float *__restrict__ T3,
+                                   OPtr<STAGE> O, long long i, float s, float k, float ct) {
     if constexpr (STAGE == 0) {
         O[i] = k;
     } else if constexpr (STAGE == 1) {
@@ -174,8 +181,8 @@ __device__ __forceinline__ void rk(const float *U, const float *__restrict__ T2,
 template <int STAGE>
 __device__ __forceinline__ constexpr bool prefetch_u() { return STAGE == 2 || STAGE == 3; }
 template <int STAGE>
-__device__ __forceinline__ float rk_u(const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
-                                      float *O, long long i, float s, float k, float ct, float u) {
+__device__ __forceinline__ float rk_u(UPtr<STAGE> U, const float *__restrict__ T2, const float *__restrict__ T3,
+                                      OPtr<STAGE> O, long long i, float s, float k, float ct, float u) {
     float v;
     if constexpr (STAGE == 0) {
         v = k;
@@ -193,6 +200,26 @@ __device__ __forceinline__ float rk_u(const float *U, const float *__restrict__ 
     return v;
 }
 
+// Stage 4 of NF fields at one cell (the classical combination of rk, same
+// expression): all U^n, T2 and T3 values are loaded before the first store.
+template <int NF>
+__device__ __forceinline__ void rk4_cell(const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
+                                         float *O, const long long (&idx)[NF], const float (&s)[NF],
+                                         const float (&k)[NF], float ct, float (&v)[NF]) {
+    float u[NF], a[NF], b[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        u[f] = U[idx[f]];
+        a[f] = T2[idx[f]];
+        b[f] = T3[idx[f]];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        v[f] = u[f] + ((a[f] - u[f]) + 2.0f * (b[f] - u[f]) + (s[f] - u[f])) * (1.0f / 3.0f) + ct * k[f];
+        O[idx[f]] = v[f];
+    }
+}
+
 // Pointwise inputs of one row of a thread, loaded one row ahead of their use so
 // that their latency overlaps the previous row's arithmetic (and, for the first
 // row, the tile load).  u: U^n of the row's state fields (stages 2-4).
@@ -205,7 +232,7 @@ struct RowPre {
 
 template <int STAGE, int MET, int NF, int NC>
 __device__ __forceinline__ void prefetch_row(RowPre<NF, NC> &p, const Coef &c, const float *const (&cp)[NC],
-                                             const float *U, const long long *off, int f0,
+                                             UPtr<STAGE> U, const long long *off, int f0,
                                              long long ai, long long gi) {
     xi_metrics<MET>(c, gi, p.xx, p.xz);
     p.zx = c.zex[gi];
@@ -444,7 +471,7 @@ __device__ __forceinline__ void iface_record(const Ctl &ctl, int it, int nx, int
 constexpr int FW = TX + 4 * HALO;   // pre-filter columns of a tile
 template <int DIR, int NF, int TZ>
 __device__ __forceinline__ void finish_interface_row(float (*sh)[TZ + 2 * HALO][LXP], float (*pre)[FW], const Geo &g,
-                                                     const Coef &c, const Ctl &ctl, const float *S,
+                                                     const Coef &c, const Ctl &ctl, const float *__restrict__ S,
                                                      const float *__restrict__ kif, int lrow, int f0, int x0, bool wb,
                                                      float dt, int jstep) {
     const int nx = g.nx;
@@ -494,7 +521,7 @@ __device__ __forceinline__ void finish_interface_row(float (*sh)[TZ + 2 * HALO][
 // contents.  The bitwise chunked-run tests guard this.
 template <int DIR, int STAGE, int NF, int TZ>
 __device__ __forceinline__ void rebuild_interface_row(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
-                                                      const float *S, const float *__restrict__ kif,
+                                                      const float *__restrict__ S, const float *__restrict__ kif,
                                                       int lrow, int f0, int x0, float cprev, bool wb) {
     const long long P = g.P;
     constexpr int par = step_par<DIR, STAGE>();
@@ -537,8 +564,8 @@ __device__ __forceinline__ int fluid_z0(const Geo &g, int band, int TZ) {
 template <int DIR, int STAGE, int TZ, int MET, int FT = 0, int FIN = 0, int INT = 0>
 __device__ __forceinline__ void
 fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl,
-           const float *S, const float *U, const float *__restrict__ T2,
-           const float *__restrict__ T3, float *O, float *__restrict__ kif, int jstep,
+           const float *__restrict__ S, UPtr<STAGE> U, const float *__restrict__ T2,
+           const float *__restrict__ T3, OPtr<STAGE> O, float *__restrict__ kif, int jstep,
            float ct, float amp_direct, float cprev, int bx, int bz, float (*fin)[FW] = nullptr) {
     // sh[0..2] = vx, vz, p of the stage state
     // a wave holds one row of the tile at a time: ty (and every row test) is
@@ -751,9 +778,16 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             }
             if constexpr (STAGE != 0) continue;
         }
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct, cur.u[0]);
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct, cur.u[1]);
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct, cur.u[2]);
+        if constexpr (STAGE == 4) {
+            const long long idx[3] = {g.off[MSFDM_FVX] + gi, g.off[MSFDM_FVZ] + gi, g.off[MSFDM_FP] + gi};
+            const float sv[3] = {shvx[r][cc], shvz[r][cc], shp[r][cc]}, kv[3] = {kvx, kvz, kp};
+            float v[3];
+            rk4_cell<3>(U, T2, T3, O, idx, sv, kv, ct, v);
+        } else {
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVX] + gi, shvx[r][cc], kvx, ct, cur.u[0]);
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FVZ] + gi, shvz[r][cc], kvz, ct, cur.u[1]);
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_FP] + gi, shp[r][cc], kp, ct, cur.u[2]);
+        }
     }
 }
 
@@ -772,9 +806,9 @@ fluid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
 constexpr int TOP_IFACE = 0, TOP_FREE = 1, TOP_PLAIN = 2, TOP_IFACE_CPML = 3;
 template <int DIR, int STAGE, int TZ, int MET, int TOP = TOP_IFACE, int FIN = 0, int INT = 0>
 __device__ __forceinline__ void
-solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *S,
-           const float *U, const float *__restrict__ T2, const float *__restrict__ T3,
-           float *O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
+solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const Ctl &ctl, const float *__restrict__ S,
+           UPtr<STAGE> U, const float *__restrict__ T2, const float *__restrict__ T3,
+           OPtr<STAGE> O, float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int bx,
            int bz, float (*fin)[FW] = nullptr) {
     // sh[0..4] = vx, vz, sx, sz, sxz of the stage state
     // a wave holds one row of the tile at a time: ty (and every row test) is
@@ -1049,8 +1083,22 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
             }
             if constexpr (STAGE != 0) continue;
         }
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cur.u[0]);
-        const float vz_new = rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cur.u[1]);
+        float vz_new;
+        if constexpr (STAGE == 4) {
+            const long long idx[5] = {g.off[MSFDM_SVX] + si, g.off[MSFDM_SVZ] + si, g.off[MSFDM_SSX] + si,
+                                      g.off[MSFDM_SSZ] + si, g.off[MSFDM_SSXZ] + si};
+            const float sv[5] = {svx[r][cc], svz[r][cc], ssx[r][cc], ssz[r][cc], ssxz[r][cc]};
+            const float kv[5] = {kvx, kvz, ksx, ksz, ksxz};
+            float v[5];
+            rk4_cell<5>(U, T2, T3, O, idx, sv, kv, ct, v);
+            vz_new = v[1];
+        } else {
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVX] + si, svx[r][cc], kvx, ct, cur.u[0]);
+            vz_new = rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SVZ] + si, svz[r][cc], kvz, ct, cur.u[1]);
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct, cur.u[2]);
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cur.u[3]);
+            rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cur.u[4]);
+        }
         if constexpr (SOLO && STAGE == 4) {
             // receiver row (3 with the free surface, 30 with CPML), vz after the
             // step (src/main.cu:297-322), plane 1 of the record buffer
@@ -1060,9 +1108,7 @@ solid_body(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c, const C
                 if (slot < ctl.rec_cap) ctl.rec[(long long)ctl.rec_cap * nx + (long long)slot * nx + ix] = vz_new;
             }
         }
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSX] + si, ssx[r][cc], ksx, ct, cur.u[2]);
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSZ] + si, ssz[r][cc], ksz, ct, cur.u[3]);
-        rk_u<STAGE>(U, T2, T3, O, g.off[MSFDM_SSXZ] + si, ssxz[r][cc], ksxz, ct, cur.u[4]);
+        (void)vz_new;
     }
 }
 
@@ -1121,12 +1167,17 @@ __device__ __forceinline__ bool solid_interior(const Geo &g, int x0, int z0, int
            z0 + TZ - 1 < g.nzs - 3 - g.npd;
 }
 
+// Resident waves per SIMD of the fluid-solid stage kernels: at most 4 for stage
+// 4, the stage that streams the most bytes (its registers would allow 5, which
+// measured slower), the compiler's choice otherwise.
+template <int STAGE> __host__ __device__ constexpr int stage_max_waves() { return STAGE == 4 ? 4 : 8; }
+
 // FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
 // and record (finish_interface_row); cprev is then the time step.
 template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
-__global__ void __launch_bounds__(NTHR)
-stage_kernel(Geo g, Coef c, Ctl ctl, const float *S, const float *U,
-             const float *__restrict__ T2, const float *__restrict__ T3, float *O,
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(1, stage_max_waves<STAGE>())))
+stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
+             const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O,
              float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
              int nbz_s, int nfi) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
@@ -1161,8 +1212,8 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *S, const float *U,
 // of the receiver row.
 template <int DIR, int STAGE, int TZ, int MET, int TOP>
 __global__ void __launch_bounds__(NTHR)
-free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *S, const float *U,
-                  const float *__restrict__ T2, const float *__restrict__ T3, float *O, int jstep,
+free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
+                  const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O, int jstep,
                   float ct, float amp_direct, int nbx, int nb) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];
     // band 0 (the surface rows, the dearest tiles) spread over the XCDs and
@@ -1259,7 +1310,7 @@ struct Shot {
 // MET as in stage_kernel: 1 when every shot of the launch has vertical grid
 // columns (xi_x a constant, xi_z = 0), 2 to decide per shot at run time.
 template <int DIR, int STAGE, int TZ, int MET, int FIN = 0>
-__global__ void __launch_bounds__(NTHR)
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(1, stage_max_waves<STAGE>())))
 batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
                    int jstep, float ct, float cprev) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
