@@ -163,15 +163,16 @@ int msfdm_set_graph(msfdm_solver *s, int enable);
 int msfdm_time_kernel(msfdm_solver *s, int which, int reps, float *ms);
 /* Time nsteps of msfdm_run between HIP events on the solver's stream. */
 int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
-/* Step nsteps with the launch sequence of msfdm_run, each launch as a direct
- * launch carrying start / stop events stamped by its own dispatch
- * (hipExtLaunchKernelGGL), so kernel durations exclude the gaps between
- * launches.  kernel_ms[6] / launches[6] (either may be NULL): total ms and
- * launch count per kernel class: 0..3 = RK stages 1..4, 4 = the interface
- * finish after a launch sequence, 5 = the step counter.  wall_ms: the whole
- * run between two stream events.  The state advances as msfdm_run's would.
- * No reference counterpart (measurement). */
-int msfdm_profile_run(msfdm_solver *s, int nsteps, float *kernel_ms, int *launches, float *wall_ms);
+/* Step nsteps with the launch sequence of msfdm_run as direct launches; the
+ * launches of the kernel classes in class_mask (bit c = class c) carry start /
+ * stop events stamped by their own dispatch (hipExtLaunchKernelGGL), so their
+ * durations exclude the gaps between launches.  Classes: 0..3 = RK stages
+ * 1..4, 4 = the interface finish after a launch sequence, 5 = the step
+ * counter.  kernel_ms[6] / launches[6] (either may be NULL): total ms and
+ * count of the timed launches per class.  wall_ms: the whole run between two
+ * stream events.  The state advances as msfdm_run's would.  No reference
+ * counterpart (measurement). */
+int msfdm_profile_run(msfdm_solver *s, int nsteps, int class_mask, float *kernel_ms, int *launches, float *wall_ms);
 /* Algorithmic HBM bytes of one time step (state + coefficients + PML memory). */
 int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
 /* Algorithmic HBM bytes of one launch timed by msfdm_time_kernel(which): an

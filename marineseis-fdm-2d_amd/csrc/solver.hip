@@ -82,8 +82,9 @@ struct msfdm_solver {
     // the stepper carries a start / stop event pair stamped by its own dispatch.
     struct KProf {
         bool on = false;
-        std::vector<hipEvent_t> ev;  // 2 per launch, kept for reuse
-        std::vector<int> cls;        // kernel class of each launch of the current run
+        int mask = 0;                // kernel classes timed (bit c: class c)
+        std::vector<hipEvent_t> ev;  // 2 per timed launch, kept for reuse
+        std::vector<int> cls;        // kernel class of each timed launch of the current run
     } prof;
 };
 
@@ -143,14 +144,17 @@ constexpr int KCLASS_IFACE = 4, KCLASS_OTHER = 5, NKCLASS = 6;
 // the events time the kernel and not the gaps between launches.
 template <class F, class... A>
 void klaunch(msfdm_solver *s, int cls, F kernel, dim3 grid, dim3 block, A... args) {
-    if (!s->prof.on) {
+    if (!s->prof.on || !((s->prof.mask >> cls) & 1)) {
         hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, args...);
         return;
     }
     const size_t i = s->prof.cls.size();
     while (s->prof.ev.size() < 2 * i + 2) {
+        // timing-only events: without the system-scope fence the stamped
+        // dispatch ends with the device-scope release of every other launch
+        // (no L2 write-back / invalidate inside the timed kernel or the next)
         hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) {   // out of events: launch untimed
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {   // out of events: launch untimed
             hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, args...);
             return;
         }
@@ -845,13 +849,16 @@ extern "C" int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms) {
 }
 
 // The launch sequence of msfdm_run (GRAPH_STEPS-step sequences where msfdm_run
-// replays its graph, single steps at odd step indices) with direct launches,
-// each stamped by its own dispatch.  kernel_ms[c] / launches[c]: total
-// milliseconds and launch count of kernel class c (RK stages 1..4, interface
-// finish, other); wall_ms: the whole run between two stream events.
-extern "C" int msfdm_profile_run(msfdm_solver *s, int nsteps, float *kernel_ms, int *launches, float *wall_ms) {
+// replays its graph, single steps at odd step indices) with direct launches;
+// the launches of the classes in class_mask are stamped by their own dispatch.
+// kernel_ms[c] / launches[c]: total milliseconds and count of the timed
+// launches of kernel class c (RK stages 1..4, interface finish, other);
+// wall_ms: the whole run between two stream events.
+extern "C" int msfdm_profile_run(msfdm_solver *s, int nsteps, int class_mask, float *kernel_ms, int *launches,
+                                 float *wall_ms) {
     clear_error();
     if (!s || nsteps < 0) return set_error(MSFDM_EINVAL, "bad argument");
+    s->prof.mask = class_mask;
     HIPCHK(hipSetDevice(s->device));
     if (int rc = grow_tables(s, s->host_it + nsteps)) return rc;
     hipEvent_t w0, w1;

@@ -107,7 +107,7 @@ def lib():
         "msfdm_get_tile_rows": ([vp, ip], i),
         "msfdm_time_kernel": ([vp, i, i, fp], i),
         "msfdm_time_run": ([vp, i, fp], i),
-        "msfdm_profile_run": ([vp, i, fp, ip, fp], i),
+        "msfdm_profile_run": ([vp, i, i, fp, ip, fp], i),
         "msfdm_bytes_per_step": ([vp, C.POINTER(C.c_double)], i),
         "msfdm_kernel_bytes": ([vp, i, C.POINTER(C.c_double)], i),
         "msfdm_run_cli": ([C.c_char_p, C.c_char_p], i),
@@ -457,14 +457,16 @@ class Solver:
         check(lib().msfdm_time_run(self.h, int(nsteps), C.byref(v)))
         return v.value
 
-    def profile_run(self, nsteps: int) -> dict:
-        """Step nsteps as run() does, every launch timed by its own dispatch:
-        {"kernel_ms": [6] totals (RK stages 1-4, interface finish, other),
-        "launches": [6] counts, "wall_ms": the whole run}."""
+    def profile_run(self, nsteps: int, classes=range(6)) -> dict:
+        """Step nsteps as run() does, the launches of the kernel classes in
+        `classes` timed by their own dispatch: {"kernel_ms": [6] totals (RK
+        stages 1-4, interface finish, other), "launches": [6] counts of timed
+        launches, "wall_ms": the whole run}."""
         ms = (C.c_float * 6)()
         n = (C.c_int * 6)()
         wall = C.c_float()
-        check(lib().msfdm_profile_run(self.h, int(nsteps), ms, n, C.byref(wall)))
+        mask = sum(1 << int(c) for c in classes)
+        check(lib().msfdm_profile_run(self.h, int(nsteps), mask, ms, n, C.byref(wall)))
         return {"kernel_ms": list(ms), "launches": list(n), "wall_ms": wall.value}
 
     def kernel_bytes(self, which: int) -> float:

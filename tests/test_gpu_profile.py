@@ -28,3 +28,10 @@ def test_profile_run_equals_run(preset):
     assert n[4] == (0 if b.solid_only else 3) and n[5] == 3
     assert all(t > 0 for t in p["kernel_ms"][:4])
     assert p["wall_ms"] >= 0.99 * sum(p["kernel_ms"])
+    # only the requested classes are timed; the stepping is the same
+    c = msfdm.Solver(m)
+    c.reset(0)
+    q = c.profile_run(70, classes=[1])
+    assert q["launches"] == [0, 70, 0, 0, 0, 0] and q["kernel_ms"][1] > 0
+    for x, y in zip(a.records(), c.records()):
+        assert np.array_equal(x, y)
