@@ -182,8 +182,10 @@ int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
  * updated cell. */
 int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
 
-/* Tile-shape autotuner: times nsteps steps for each tile height, keeps the
- * fastest and resets the shot (no reference counterpart). */
+/* Tile-shape autotuner: times nsteps steps for each tile height (8, 16 rows)
+ * three times, the heights interleaved, keeps the height with the fastest
+ * single timing and resets the shot (no reference counterpart).
+ * msfdm_batch_autotune does the same for a batch. */
 int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
 
 /* Replace the medium arrays of an existing solver (each [rows][nx] float32 in

@@ -3,21 +3,26 @@
 // The forward stage is k = B_dir(L_dir(S) + src) (interface BC B, RHS L of
 // include/pml_fs_kernels.cuh:333-827).  One adjoint stage takes
 // kbar = clam * LP + cprev * TBprev (the RK4 stage weights), applies B^T on the
-// interface rows (adj_iface_q) and scatters L^T(qbar) into TB:
-//   adj_stage_kernel  per 64 x TZ tile, every forward expression
-//                     k[i] = sum_j a_ij S[j] is run in reverse,
-//                     Sbar[j] += a_ij qbar[i]; stencil contributions are
-//                     gathered in an LDS accumulator covering the tile and its
-//                     3-cell halo (ghost-flux reflections included) and added
-//                     to the output (one LDS copy per wave, x-direction taps
-//                     gathered across lanes: no LDS atomics).  The tiles run in
-//                     four launches, one per colour (tile column parity x band
-//                     parity): the accumulators of one colour never overlap, so
-//                     the flush is a plain read-modify-write and the colours
-//                     add in a fixed order -- repeated runs give the same bits.
-//                     PML memory terms and the model-gradient terms dk/dm . qbar
-//                     (evaluated on the forward stage state S) are pointwise.
-//                     Gradients accumulate in FP64.
+// interface rows (adj_iface_q) and forms TB = L^T(qbar) in two launches:
+//   adj_prod_kernel    per source cell (64 x TZ tiles, the forward state S of
+//                      the tile and its 3-cell halo in LDS): every forward
+//                      expression k[i] = c(i) D(S)[i] (D a one-sided stencil
+//                      along x or z) has the transpose
+//                      Sbar[j] += w * c(i) qbar[i] over the stencil's cells j;
+//                      the kernel evaluates the pointwise products c(i) qbar[i]
+//                      of all of a cell's stencils (the "adjoint products") and
+//                      stores them, together with the pointwise parts of the
+//                      transpose (PML memory) and the model-gradient terms
+//                      dk/dm . qbar, which accumulate in FP64.
+//   adj_gather_kernel  per output cell (64 x TZ tiles, the products of the
+//                      tile and its 3-cell halo in LDS): Sbar[j] = the sum of
+//                      w * product over every source cell whose stencil holds
+//                      j, with the weights of that source row's stencil -- the
+//                      DRP taps, the 4-2 / 2-2 closures, and the ghost-flux
+//                      rows of the interface and free surface with their
+//                      reflections.
+// Every output cell has one writer and sums its terms in a fixed order, so
+// repeated runs give the same bits; nothing needs zeroing between stages.
 //   adj_filter        Filter^T on the interface rows (out of place, LDS).
 // The per-column 8x8 interface BC and filter maps are transposed by probing
 // the forward maps with unit vectors.
@@ -50,62 +55,55 @@ template <int DIR> __device__ __forceinline__ float drp_w(int q) {   // q = 0..4
     }
 }
 
-// Adjoint of mac_dz: add(offset, weight*g) for the distance-dependent zeta operator.
-template <int DIR, class Add>
-__device__ __forceinline__ void mac_dz_adj(int dist, bool solid_side, float g, Add add) {
-    if (dist >= 3) {
-        constexpr int o = lo_off<DIR>();
-#pragma unroll
-        for (int q = 0; q < 5; ++q) add(o + q, drp_w<DIR>(q) * g);
-        return;
-    }
+// Weight of row offset m in the DRP stencil of direction DIR (0 outside it).
+template <int DIR> __device__ __forceinline__ float drp_at(int m) {
+    const int q = m - lo_off<DIR>();
+    return (q >= 0 && q <= 4) ? drp_w<DIR>(q) : 0.0f;
+}
+
+// Weight of row offset m in the distance-dependent zeta operator mac_dz
+// (Mac_dv_dze, include/tools_func.cuh:131-140) of a source row at distance
+// dist from the boundary.
+template <int DIR>
+__device__ __forceinline__ float mac_at(int dist, bool solid_side, int m) {
+    if (dist >= 3) return drp_at<DIR>(m);
     if (dist == 2) {
-        if constexpr (DIR == 0) { add(0, M42_0 * g); add(1, M42_1 * g); add(2, M42_2 * g); }
-        else { add(0, -M42_0 * g); add(-1, -M42_1 * g); add(-2, -M42_2 * g); }
-        return;
+        if constexpr (DIR == 0) return m == 0 ? M42_0 : m == 1 ? M42_1 : m == 2 ? M42_2 : 0.0f;
+        else return m == 0 ? -M42_0 : m == -1 ? -M42_1 : m == -2 ? -M42_2 : 0.0f;
     }
     bool fwd = DIR == 0;
     if (dist == 0) fwd = solid_side;
-    if (fwd) { add(0, -g); add(1, g); }
-    else { add(0, g); add(-1, -g); }
+    if (fwd) return m == 0 ? -1.0f : m == 1 ? 1.0f : 0.0f;
+    return m == 0 ? 1.0f : m == -1 ? -1.0f : 0.0f;
 }
-
-// Forward one-sided derivative reading global memory (stride 1 or P).
-template <int DIR>
-__device__ __forceinline__ float drp_g(const float *f, long long i, long long stride) {
-    constexpr int o = lo_off<DIR>();
-    return drp5<DIR>(f[i + o * stride], f[i + (o + 1) * stride], f[i + (o + 2) * stride], f[i + (o + 3) * stride],
-                     f[i + (o + 4) * stride]);
-}
-template <int DIR>
-__device__ __forceinline__ float mac_g(const float *f, long long i, long long P, int dist, bool solid_side) {
-    if (dist >= 3) return drp_g<DIR>(f, i, P);
-    if (dist == 2) {
-        if constexpr (DIR == 0) return M42_0 * f[i] + M42_1 * f[i + P] + M42_2 * f[i + 2 * P];
-        else return -(M42_0 * f[i] + M42_1 * f[i - P] + M42_2 * f[i - 2 * P]);
-    }
-    bool fwd = DIR == 0;
-    if (dist == 0) fwd = solid_side;
-    return fwd ? (-f[i] + f[i + P]) : -(-f[i] + f[i - P]);
-}
-
-// LDS accumulator of one wave: 5 fields x (its RW rows + 3-row halos) x the
-// tile's 70 columns (64 + 3-column halos).
-template <int TZ>
-struct WaveAcc {
-    float v[5][TZ / 4 + 2 * HALO][LXP];
-};
 
 struct AdjIn {
     const float *S;      // forward stage state
     const float *LP;     // lambda'
     const float *TBp;    // previous stage's Tbar (may be null when cprev == 0)
     const float *qif;    // [8][P] interface qbar = B^T kbar
-    float *TB;           // output Tbar (zeroed)
+    float *TB;           // output Tbar
+    float *prod;         // adjoint products: [ADJ_NQF][nzf][P], then [ADJ_NQS][nzs][P]
     float clam, cprev;
     long long bstride;   // floats between the state buffers of consecutive shots
     long long qstride;   // floats between the qif buffers of consecutive shots
+    long long pstride;   // floats between the product buffers of consecutive shots
 };
+
+// Adjoint products per cell.  Fluid: 0, 1 = the velocity equations' pressure
+// stencils (x, z: dpx_bar, dpz_bar; on the ghost-flux rows near the surface
+// and the interface the flux sums sx_bar, sz_bar instead); 2..5 = the pressure
+// equation's velocity stencils (vx along x, vz along x, vx along z, vz along
+// z).  Solid: 0..3 = the stress equations' velocity stencils (vx x, vz x,
+// vx z, vz z); 4..9 = the velocity equations' stress stencils (sxx x, szz x,
+// sxz x, sxx z, szz z, sxz z; on the ghost-flux rows 0..2 the flux sums
+// sx_bar, sz_bar in 4, 5 and zeros in 6..9).
+constexpr int ADJ_NQF = 6, ADJ_NQS = 10;
+
+__device__ __forceinline__ long long prod_fluid(const Geo &g, int k) { return (long long)k * g.nzf * g.P; }
+__device__ __forceinline__ long long prod_solid(const Geo &g, int k) {
+    return (long long)ADJ_NQF * g.nzf * g.P + (long long)k * g.nzs * g.P;
+}
 
 // Inputs of shot `shot` of a multi-shot launch (blockIdx.y); every shot's
 // buffers sit at a fixed stride from shot 0's.
@@ -114,530 +112,608 @@ __device__ __forceinline__ AdjIn adj_of_shot(AdjIn a, int shot) {
     a.S += o; a.LP += o; a.TB += o;
     if (a.TBp) a.TBp += o;
     a.qif += (long long)shot * a.qstride;
+    a.prod += (long long)shot * a.pstride;
     return a;
 }
 
+// Load from memory that no thread writes during the kernel (the adjoint
+// inputs lambda', the previous stage's Tbar, the interface qbar, the products
+// in the gather kernel, the model coefficients and metrics): through the
+// constant address space, so the compiler may issue it ahead of earlier
+// stores to the outputs instead of waiting for them.
+template <class T>
+__device__ __forceinline__ T ldc(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 __device__ __forceinline__ float kbar(const AdjIn &a, long long i) {
-    float v = a.clam * a.LP[i];
-    if (a.TBp) v += a.cprev * a.TBp[i];
+    float v = a.clam * ldc(&a.LP[i]);
+    if (a.TBp) v += a.cprev * ldc(&a.TBp[i]);
     return v;
 }
 
-// x-direction scatter of one field on one row: tap q of lane t lands on tile
-// column t + o + q (o = lo_off<DIR>).  It runs as a gather over the wave: lane u
-// sums the taps landing on column u and lanes 0..3 the taps landing on the
-// four halo columns (F: -1, 64, 65, 66; B: -3, -2, -1, 64), so every
-// accumulator cell has a single writing lane and the read-modify-writes need
-// no ordering.  v[] is zero on lanes that contribute nothing; all 64 lanes
-// must reach the call (the wave's row is uniform).
-template <int DIR>
-__device__ __forceinline__ void xgather(float (*row)[LXP], int lr, int tx, const float v[5]) {
-    constexpr int o = lo_off<DIR>();
-    const int hc = DIR == 0 ? (tx == 0 ? -1 : TX - 1 + tx) : (tx < 3 ? tx - 3 : TX);
-    float s = 0.0f, sh = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        const int src = tx - o - q, hsrc = hc - o - q;
-        const float x = __shfl(v[q], src & (TX - 1));
-        const float y = __shfl(v[q], hsrc & (TX - 1));
-        s += (src >= 0 && src < TX) ? x : 0.0f;
-        sh += (hsrc >= 0 && hsrc < TX) ? y : 0.0f;
-    }
-    row[lr][tx + HALO] += s;
-    if (tx < 4) row[lr][hc + HALO] += sh;
+// Interior source tiles (the products of a cell come from the plain DRP
+// stencils and no PML term): every column outside the side strips, every row
+// at least 3 rows from the surface, the interface and the bottom strip.
+__device__ __forceinline__ bool adj_fluid_interior(const Geo &g, int x0, int z0, int TZ) {
+    return cols_interior(g, x0) && z0 >= 6 && z0 + TZ - 1 <= g.nzf - 4;
+}
+__device__ __forceinline__ bool adj_solid_interior(const Geo &g, int x0, int z0, int TZ) {
+    return cols_interior(g, x0) && z0 >= 3 && z0 + TZ - 1 < g.nzs - 3 - g.npd;
 }
 
 // ---------------------------------------------------------------------------
-// Fluid tile: rows 3..nzf-1.  acc[0..2] = vx, vz, p adjoint accumulators.
+// Products, fluid source tile: rows 3..nzf-1 (z0 = 3 + bz * TZ).  The
+// arithmetic is the transpose of fluid_body, expression by expression.
 // ---------------------------------------------------------------------------
-template <int DIR, int TZ>
-__device__ __forceinline__ void fluid_adj_body(WaveAcc<TZ> *acc, const Geo &g, const Coef &c,
-                                               const AdjIn &a, const Grad &gr, int bx, int bz) {
-    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
+template <int DIR, int TZ, int MET, int INT>
+__device__ __forceinline__ void fluid_prod_body(const float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+                                                const AdjIn &a, const Grad &gr, int bx, int bz) {
+    // a wave holds consecutive rows of the tile: ty (and every row test) is
+    // uniform per wave, which readfirstlane tells the compiler
+    const int tx = threadIdx.x & (TX - 1), ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
     constexpr int RW = TZ / 4;   // consecutive tile rows per wave
     const int x0 = bx * TX, z0 = 3 + bz * TZ;
     const int P = g.P, nx = g.nx, nzf = g.nzf;
     const float *S = a.S;
-    const float *Sp = S + g.off[MSFDM_FP], *Svx = S + g.off[MSFDM_FVX], *Svz = S + g.off[MSFDM_FVZ];
+    const float (*shvx)[LXP] = sh[0];
+    const float (*shvz)[LXP] = sh[1];
+    const float (*shp)[LXP] = sh[2];
+    const int sc = tx + XOFF;
     const int ix = x0 + tx;
-    const bool active_col = ix >= 3 && ix <= nx - 4;
-    const int scol = active_col ? side_col(ix, nx, g.npd) : -1;
-    const int cc = tx + HALO;
-    // this wave's accumulator; its rows start at tile row ty * RW - HALO
-    float (*wa)[RW + 2 * HALO][LXP] = acc[ty].v;
-    // Every accumulator cell has one writing lane: single-cell updates go to the
-    // lane's own column (z-direction taps, ghost reflections), the x-direction
-    // taps go through xgather.  Lanes outside columns 3..nx-4 run the row with
-    // in-bounds loads but contribute nothing.
-    const bool act = active_col;
-    auto add = [&](int fld, int iz_to, float v) {
-        if (act) wa[fld][iz_to - z0 - ty * RW + HALO][tx + HALO] += v;
-    };
-    auto addx = [&](int fld, int iz_to, const float (&v)[5]) {
-        float w[5];
+    const bool act = INT || (ix >= 3 && ix <= nx - 4);
+    const int scol = (INT || !act) ? -1 : side_col(ix, nx, g.npd);
+    float *Q = a.prod;
+    const long long pP = (long long)nzf * P;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) w[q] = act ? v[q] : 0.0f;
-        xgather<DIR>(wa[fld], iz_to - z0 - ty * RW + HALO, tx, w);
-    };
     for (int j = 0; j < RW; ++j) {
         const int iz = z0 + ty * RW + j;
-        if (iz > nzf - 1) continue;   // uniform over the wave
+        if (!INT && iz > nzf - 1) break;   // uniform over the wave
         const long long gi = (long long)iz * P + ix;
-        const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
-        float xx, xz;
-        xi_metrics<2>(c, gi, xx, xz);
-        const float gxixi = xx * xx + xz * xz;
-        const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
-        const bool at_row = dist_flso == 0;
-        const float qvx = at_row ? a.qif[0 * (long long)P + ix] : kbar(a, g.off[MSFDM_FVX] + gi);
-        const float qvz = at_row ? a.qif[1 * (long long)P + ix] : kbar(a, g.off[MSFDM_FVZ] + gi);
-        const float qp = at_row ? a.qif[2 * (long long)P + ix] : kbar(a, g.off[MSFDM_FP] + gi);
-        const float rinv = c.rhof_inv[gi];
-        // ---- velocity equations
-        float Dxb = 0.f, Dzb = 0.f;
-        const float rvxb = qvx * rinv, rvzb = qvz * rinv;
-        float pxa = 0.f, pxb = 0.f, pxc = 1.f;
-        long long ai = 0;
-        float Vx = 0.f, Vz = 0.f, Ex = 0.f, Ez = 0.f;
-        if (scol >= 0) {
-            pxa = c.pxa[ix]; pxb = c.pxb[ix]; pxc = c.pxc[ix];
-            ai = (long long)iz * g.W2 + scol;
-            const float qVx = kbar(a, g.off[MSFDM_FXVX] + ai), qVz = kbar(a, g.off[MSFDM_FXVZ] + ai);
-            Dxb = (pxc - 1.0f) * rinv * qvx + pxb * qVx;
-            Dzb = (pxc - 1.0f) * rinv * qvz + pxb * qVz;
-            a.TB[g.off[MSFDM_FXVX] + ai] += -pxc * rinv * qvx - pxa * qVx;
-            a.TB[g.off[MSFDM_FXVZ] + ai] += -pxc * rinv * qvz - pxa * qVz;
-            Vx = S[g.off[MSFDM_FXVX] + ai]; Vz = S[g.off[MSFDM_FXVZ] + ai];
-            Ex = S[g.off[MSFDM_FXEX] + ai]; Ez = S[g.off[MSFDM_FXEZ] + ai];
-        }
-        float rvx, rvz, Dx, Dz;   // forward values for the gradient
-        if (dist_flso < 3 || dist_fs < 3) {
-            const bool at_if = dist_flso < 3;
-            auto fze_row = [&](int rr, float &fx, float &fz) {
-                const long long q = (long long)rr * P + ix;
-                const float mJp = -c.J[q] * Sp[q];
-                fx = mJp * c.zex[q];
-                fz = mJp * c.zez[q];
-            };
-            float sx = 0.f, sz = 0.f;
-            constexpr int o = lo_off<DIR>();
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                const int m = o + q;
-                const float w = drp_w<DIR>(q);
-                const long long gq = gi + m;
-                const float mJp = -c.J[gq] * Sp[gq];
-                float xxq, xzq;
-                xi_metrics<2>(c, gq, xxq, xzq);
-                sx += w * mJp * xxq;
-                sz += w * mJp * xzq;
-                const int rr = iz + m;
-                float bx, bz;
-                if (at_if) {
-                    if (rr <= nzf - 1) fze_row(rr, bx, bz);
-                    else { float f0x, f0z, fnx, fnz; fze_row(nzf - 1, f0x, f0z); fze_row(2 * (nzf - 1) - rr, fnx, fnz); bx = 2.f * f0x - fnx; bz = 2.f * f0z - fnz; }
-                } else {
-                    if (rr >= 3) fze_row(rr, bx, bz);
-                    else { float fnx, fnz; fze_row(6 - rr, fnx, fnz); bx = -fnx; bz = -fnz; }
-                }
-                sx += w * bx;
-                sz += w * bz;
-            }
-            rvx = sx / J; rvz = sz / J;
-            const float proj = (xx * sx + xz * sz) / (gxixi * J);
-            Dx = xx * proj; Dz = xz * proj;
-            const float projb = xx * Dxb + xz * Dzb;
-            const float sxb = rvxb / J + xx * projb / (gxixi * J);
-            const float szb = rvzb / J + xz * projb / (gxixi * J);
-            auto fze_add = [&](int rr, float fbx, float fbz) {
-                const long long q = (long long)rr * P + ix;
-                add(2, rr, -c.J[q] * (c.zex[q] * fbx + c.zez[q] * fbz));
-            };
-            float xv[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                const int m = o + q;
-                const float w = drp_w<DIR>(q);
-                const long long gq = gi + m;
-                float xxq, xzq;
-                xi_metrics<2>(c, gq, xxq, xzq);
-                xv[q] = -c.J[gq] * (xxq * w * sxb + xzq * w * szb);
-                const int rr = iz + m;
-                const float fbx = w * sxb, fbz = w * szb;
-                if (at_if) {
-                    if (rr <= nzf - 1) fze_add(rr, fbx, fbz);
-                    else { fze_add(nzf - 1, 2.f * fbx, 2.f * fbz); fze_add(2 * (nzf - 1) - rr, -fbx, -fbz); }
-                } else {
-                    if (rr >= 3) fze_add(rr, fbx, fbz);
-                    else fze_add(6 - rr, -fbx, -fbz);
-                }
-            }
-            addx(2, iz, xv);
-        } else {
-            const float dpx = drp_g<DIR>(Sp, gi, 1), dpz = drp_g<DIR>(Sp, gi, P);
-            const float gxize = xx * zx + xz * zz;
-            rvx = -(xx * dpx + zx * dpz); rvz = -(xz * dpx + zz * dpz);
-            const float t = (gxixi * dpx + gxize * dpz) / gxixi;
-            Dx = -xx * t; Dz = -xz * t;
-            const float tb = -xx * Dxb - xz * Dzb;
-            const float dpxb = -xx * rvxb - xz * rvzb + tb;
-            const float dpzb = -zx * rvxb - zz * rvzb + tb * gxize / gxixi;
-            constexpr int o = lo_off<DIR>();
-            float xv[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                xv[q] = drp_w<DIR>(q) * dpxb;
-                add(2, iz + o + q, drp_w<DIR>(q) * dpzb);
-            }
-            addx(2, iz, xv);
-        }
-        float kvx = rvx, kvz = rvz;
-        if (scol >= 0) { kvx += (pxc - 1.0f) * Dx - pxc * Vx; kvz += (pxc - 1.0f) * Dz - pxc * Vz; }
-        if (act) gr.rho_f[gi] += -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
-        // ---- pressure equation (row 3: p_k = 0, no dependence)
-        if (dist_fs != 0) {
-            const float Km = c.Kf[gi];
-            const int dzn = min(dist_fs, dist_flso);
-            const float dvxx = drp_g<DIR>(Svx, gi, 1), dvzx = drp_g<DIR>(Svz, gi, 1);
-            const float dvxz = mac_g<DIR>(Svx, gi, P, dzn, false), dvzz = mac_g<DIR>(Svz, gi, P, dzn, false);
-            float bracket = -(xx * dvxx + zx * dvxz + xz * dvzx + zz * dvzz);
-            float bxx = -Km * xx * qp, bxz = -Km * zx * qp, bzx = -Km * xz * qp, bzz = -Km * zz * qp;
+        const int r = iz - z0 + HALO;   // tile row of the cell in sh
+        float q[ADJ_NQF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (act) {
+            const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
+            // this cell's FP64 gradient sums, loaded before the row's arithmetic
+            const double g_rho = gr.rho_f[gi], g_K = dist_fs != 0 ? gr.K[gi] : 0.0;
+            const float J = ldc(&c.J[gi]), zx = ldc(&c.zex[gi]), zz = ldc(&c.zez[gi]);
+            float xx, xz;
+            xi_metrics<MET>(c, gi, xx, xz);
+            const float gxixi = xx * xx + xz * xz;
+            const bool at_row = !INT && dist_flso == 0;
+            const float qvx = at_row ? ldc(&a.qif[0 * (long long)P + ix]) : kbar(a, g.off[MSFDM_FVX] + gi);
+            const float qvz = at_row ? ldc(&a.qif[1 * (long long)P + ix]) : kbar(a, g.off[MSFDM_FVZ] + gi);
+            const float qp = at_row ? ldc(&a.qif[2 * (long long)P + ix]) : kbar(a, g.off[MSFDM_FP] + gi);
+            const float rinv = ldc(&c.rhof_inv[gi]);
+            // ---- velocity equations
+            float Dxb = 0.f, Dzb = 0.f;
+            const float rvxb = qvx * rinv, rvzb = qvz * rinv;
+            float pxa = 0.f, pxb = 0.f, pxc = 1.f;
+            long long ai = 0;
+            float Vx = 0.f, Vz = 0.f, Ex = 0.f, Ez = 0.f;
             if (scol >= 0) {
-                const float gg = (xx * zx + xz * zz) / gxixi;
-                const float qEx = kbar(a, g.off[MSFDM_FXEX] + ai), qEz = kbar(a, g.off[MSFDM_FXEZ] + ai);
-                const float Dvxb = -Km * (pxc - 1.0f) * xx * qp + pxb * qEx;
-                const float Dvzb = -Km * (pxc - 1.0f) * xz * qp + pxb * qEz;
-                a.TB[g.off[MSFDM_FXEX] + ai] += Km * pxc * xx * qp - pxa * qEx;
-                a.TB[g.off[MSFDM_FXEZ] + ai] += Km * pxc * xz * qp - pxa * qEz;
-                bxx += Dvxb; bxz += gg * Dvxb; bzx += Dvzb; bzz += gg * Dvzb;
-                const float Dvx = dvxx + gg * dvxz, Dvz = dvzx + gg * dvzz;
-                bracket += -(pxc - 1.0f) * xx * Dvx - (pxc - 1.0f) * xz * Dvz + pxc * xx * Ex + pxc * xz * Ez;
+                pxa = ldc(&c.pxa[ix]); pxb = ldc(&c.pxb[ix]); pxc = ldc(&c.pxc[ix]);
+                ai = (long long)iz * g.W2 + scol;
+                const float qVx = kbar(a, g.off[MSFDM_FXVX] + ai), qVz = kbar(a, g.off[MSFDM_FXVZ] + ai);
+                Dxb = (pxc - 1.0f) * rinv * qvx + pxb * qVx;
+                Dzb = (pxc - 1.0f) * rinv * qvz + pxb * qVz;
+                a.TB[g.off[MSFDM_FXVX] + ai] = -pxc * rinv * qvx - pxa * qVx;
+                a.TB[g.off[MSFDM_FXVZ] + ai] = -pxc * rinv * qvz - pxa * qVz;
+                Vx = S[g.off[MSFDM_FXVX] + ai]; Vz = S[g.off[MSFDM_FXVZ] + ai];
+                Ex = S[g.off[MSFDM_FXEX] + ai]; Ez = S[g.off[MSFDM_FXEZ] + ai];
             }
-            float xa[5], xb[5];
+            float rvx, rvz, Dx = 0.f, Dz = 0.f;   // forward values for the gradient
+            if (!INT && (dist_flso < 3 || dist_fs < 3)) {
+                // ghost-flux rows: rho v_k = (D_xi(-J p xi) + D_ze(-J p ze)) / J
+                const bool at_if = dist_flso < 3;
+                auto fze_row = [&](int rr, float &fx, float &fz) {
+                    const long long qq = (long long)rr * P + ix;
+                    const float mJp = -ldc(&c.J[qq]) * shp[r + (rr - iz)][sc];
+                    fx = mJp * ldc(&c.zex[qq]);
+                    fz = mJp * ldc(&c.zez[qq]);
+                };
+                float sx = 0.f, sz = 0.f;
+                constexpr int o = lo_off<DIR>();
 #pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                xa[q] = drp_w<DIR>(q) * bxx;
-                xb[q] = drp_w<DIR>(q) * bzx;
+                for (int k = 0; k < 5; ++k) {
+                    const int m = o + k;
+                    const float w = drp_w<DIR>(k);
+                    const long long gq = gi + m;
+                    const float mJp = -ldc(&c.J[gq]) * shp[r][sc + m];
+                    float xxq, xzq;
+                    xi_metrics<MET>(c, gq, xxq, xzq);
+                    sx += w * mJp * xxq;
+                    sz += w * mJp * xzq;
+                    const int rr = iz + m;
+                    float fbx, fbz;
+                    if (at_if) {
+                        if (rr <= nzf - 1) fze_row(rr, fbx, fbz);
+                        else { float f0x, f0z, fnx, fnz; fze_row(nzf - 1, f0x, f0z); fze_row(2 * (nzf - 1) - rr, fnx, fnz); fbx = 2.f * f0x - fnx; fbz = 2.f * f0z - fnz; }
+                    } else {
+                        if (rr >= 3) fze_row(rr, fbx, fbz);
+                        else { float fnx, fnz; fze_row(6 - rr, fnx, fnz); fbx = -fnx; fbz = -fnz; }
+                    }
+                    sx += w * fbx;
+                    sz += w * fbz;
+                }
+                rvx = sx / J; rvz = sz / J;
+                const float proj = (xx * sx + xz * sz) / (gxixi * J);
+                Dx = xx * proj; Dz = xz * proj;
+                const float projb = xx * Dxb + xz * Dzb;
+                q[0] = rvxb / J + xx * projb / (gxixi * J);   // sx_bar
+                q[1] = rvzb / J + xz * projb / (gxixi * J);   // sz_bar
+            } else {
+                const float dpx = dx_lds<DIR>(shp[r], sc), dpz = dz_lds<DIR>(shp, r, sc);
+                const float gxize = xx * zx + xz * zz;
+                rvx = -(xx * dpx + zx * dpz); rvz = -(xz * dpx + zz * dpz);
+                float tb = 0.0f, tgx = 0.0f;
+                if (scol >= 0) {
+                    const float t = (gxixi * dpx + gxize * dpz) / gxixi;
+                    Dx = -xx * t; Dz = -xz * t;
+                    tb = -xx * Dxb - xz * Dzb;
+                    tgx = tb * gxize / gxixi;
+                }
+                q[0] = -xx * rvxb - xz * rvzb + tb;   // dpx_bar
+                q[1] = -zx * rvxb - zz * rvzb + tgx;  // dpz_bar
             }
-            addx(0, iz, xa);
-            addx(1, iz, xb);
-            mac_dz_adj<DIR>(dzn, false, bxz, [&](int m, float v) { add(0, iz + m, v); });
-            mac_dz_adj<DIR>(dzn, false, bzz, [&](int m, float v) { add(1, iz + m, v); });
-            if (act) gr.K[gi] += (double)qp * (double)bracket;
+            float kvx = rvx, kvz = rvz;
+            if (scol >= 0) { kvx += (pxc - 1.0f) * Dx - pxc * Vx; kvz += (pxc - 1.0f) * Dz - pxc * Vz; }
+            gr.rho_f[gi] = g_rho + -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
+            // ---- pressure equation (row 3: p_k = 0, no dependence)
+            if (dist_fs != 0) {
+                const float Km = ldc(&c.Kf[gi]);
+                const int dzn = INT ? 3 : min(dist_fs, dist_flso);
+                const float dvxx = dx_lds<DIR>(shvx[r], sc), dvzx = dx_lds<DIR>(shvz[r], sc);
+                const float dvxz = mac_dz<DIR>(shvx, r, sc, dzn, false), dvzz = mac_dz<DIR>(shvz, r, sc, dzn, false);
+                float bracket = -(xx * dvxx + zx * dvxz + xz * dvzx + zz * dvzz);
+                float bxx = -Km * xx * qp, bxz = -Km * zx * qp, bzx = -Km * xz * qp, bzz = -Km * zz * qp;
+                if (scol >= 0) {
+                    const float gg = (xx * zx + xz * zz) / gxixi;
+                    const float qEx = kbar(a, g.off[MSFDM_FXEX] + ai), qEz = kbar(a, g.off[MSFDM_FXEZ] + ai);
+                    const float Dvxb = -Km * (pxc - 1.0f) * xx * qp + pxb * qEx;
+                    const float Dvzb = -Km * (pxc - 1.0f) * xz * qp + pxb * qEz;
+                    a.TB[g.off[MSFDM_FXEX] + ai] = Km * pxc * xx * qp - pxa * qEx;
+                    a.TB[g.off[MSFDM_FXEZ] + ai] = Km * pxc * xz * qp - pxa * qEz;
+                    bxx += Dvxb; bxz += gg * Dvxb; bzx += Dvzb; bzz += gg * Dvzb;
+                    const float Dvx = dvxx + gg * dvxz, Dvz = dvzx + gg * dvzz;
+                    bracket += -(pxc - 1.0f) * xx * Dvx - (pxc - 1.0f) * xz * Dvz + pxc * xx * Ex + pxc * xz * Ez;
+                }
+                q[2] = bxx; q[3] = bzx; q[4] = bxz; q[5] = bzz;
+                gr.K[gi] = g_K + (double)qp * (double)bracket;
+            }
         }
+#pragma unroll
+        for (int k = 0; k < ADJ_NQF; ++k) Q[k * pP + gi] = q[k];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Solid tile: rows 0..nzs-4.  acc[0..4] = vx, vz, sx, sz, sxz accumulators.
+// Products, solid source tile: rows 0..nzs-4 (z0 = bz * TZ), metrics at the
+// global row iz + nzf - 1.  The transpose of solid_body, expression by
+// expression.
 // ---------------------------------------------------------------------------
-template <int DIR, int TZ>
-__device__ __forceinline__ void solid_adj_body(WaveAcc<TZ> *acc, const Geo &g, const Coef &c,
-                                               const AdjIn &a, const Grad &gr, int bx, int bz) {
-    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
-    constexpr int RW = TZ / 4;   // consecutive tile rows per wave
+template <int DIR, int TZ, int MET, int INT>
+__device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+                                                const AdjIn &a, const Grad &gr, int bx, int bz) {
+    const int tx = threadIdx.x & (TX - 1), ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
+    constexpr int RW = TZ / 4;
     const int x0 = bx * TX, z0 = bz * TZ;
     const int P = g.P, nx = g.nx, nzs = g.nzs, npd = g.npd;
     const float *S = a.S;
-    const float *Svx = S + g.off[MSFDM_SVX], *Svz = S + g.off[MSFDM_SVZ];
-    const float *Ssx = S + g.off[MSFDM_SSX], *Ssz = S + g.off[MSFDM_SSZ], *Ssxz = S + g.off[MSFDM_SSXZ];
+    const float (*svx)[LXP] = sh[0];
+    const float (*svz)[LXP] = sh[1];
+    const float (*ssx)[LXP] = sh[2];
+    const float (*ssz)[LXP] = sh[3];
+    const float (*ssxz)[LXP] = sh[4];
+    const int sc = tx + XOFF;
     const int ix = x0 + tx;
-    const bool active_col = ix >= 3 && ix <= nx - 4;
-    const int scol = active_col ? side_col(ix, nx, npd) : -1;
+    const bool act = INT || (ix >= 3 && ix <= nx - 4);
+    const int scol = (INT || !act) ? -1 : side_col(ix, nx, npd);
     const long long moff = (long long)(g.nzf - 1) * P;
     const int zb0 = nzs - 3 - npd;
-    // this wave's accumulator; its rows start at tile row ty * RW - HALO
-    float (*wa)[RW + 2 * HALO][LXP] = acc[ty].v;
-    // Every accumulator cell has one writing lane: single-cell updates go to the
-    // lane's own column (z-direction taps, ghost reflections), the x-direction
-    // taps go through xgather.  Lanes outside columns 3..nx-4 run the row with
-    // in-bounds loads but contribute nothing.
-    const bool act = active_col;
-    auto add = [&](int fld, int iz_to, float v) {
-        if (act) wa[fld][iz_to - z0 - ty * RW + HALO][tx + HALO] += v;
-    };
-    auto addx = [&](int fld, int iz_to, const float (&v)[5]) {
-        float w[5];
+    float *Q = a.prod + prod_solid(g, 0);
+    const long long pP = (long long)nzs * P;
+    constexpr int o = lo_off<DIR>();
 #pragma unroll
-        for (int q = 0; q < 5; ++q) w[q] = act ? v[q] : 0.0f;
-        xgather<DIR>(wa[fld], iz_to - z0 - ty * RW + HALO, tx, w);
-    };
     for (int j = 0; j < RW; ++j) {
         const int iz = z0 + ty * RW + j;
-        if (iz > nzs - 4) continue;   // uniform over the wave
+        if (!INT && iz > nzs - 4) break;   // uniform over the wave
         const long long si = (long long)iz * P + ix, gi = moff + si;
-        const float J = c.J[gi], zx = c.zex[gi], zz = c.zez[gi];
-        float xx, xz;
-        xi_metrics<2>(c, gi, xx, xz);
-        const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
-        const bool side = scol >= 0, bottom = act && iz >= zb0;   // PML memory only in updated columns
-        const bool at_row = iz == 0;
-        const float qvx = at_row ? a.qif[3 * (long long)P + ix] : kbar(a, g.off[MSFDM_SVX] + si);
-        const float qvz = at_row ? a.qif[4 * (long long)P + ix] : kbar(a, g.off[MSFDM_SVZ] + si);
-        const float qsx = at_row ? a.qif[5 * (long long)P + ix] : kbar(a, g.off[MSFDM_SSX] + si);
-        const float qsz = at_row ? a.qif[6 * (long long)P + ix] : kbar(a, g.off[MSFDM_SSZ] + si);
-        const float qsxz = at_row ? a.qif[7 * (long long)P + ix] : kbar(a, g.off[MSFDM_SSXZ] + si);
-        const float c11 = c.C11[si], c13 = c.C13[si], c33 = c.C33[si], c55 = c.C55[si], rinv = c.rhos_inv[si];
+        const int r = iz - z0 + HALO;
+        float q[ADJ_NQS] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (act) {
+            const double g11 = gr.C11[si], g13 = gr.C13[si], g33 = gr.C33[si], g55 = gr.C55[si], grs = gr.rho_s[si];
+            const float J = ldc(&c.J[gi]), zx = ldc(&c.zex[gi]), zz = ldc(&c.zez[gi]);
+            float xx, xz;
+            xi_metrics<MET>(c, gi, xx, xz);
+            const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
+            const bool side = scol >= 0, bottom = !INT && iz >= zb0;
+            const bool at_row = !INT && iz == 0;
+            const float qvx = at_row ? ldc(&a.qif[3 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SVX] + si);
+            const float qvz = at_row ? ldc(&a.qif[4 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SVZ] + si);
+            const float qsx = at_row ? ldc(&a.qif[5 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SSX] + si);
+            const float qsz = at_row ? ldc(&a.qif[6 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SSZ] + si);
+            const float qsxz = at_row ? ldc(&a.qif[7 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SSXZ] + si);
+            const float c11 = ldc(&c.C11[si]), c13 = ldc(&c.C13[si]), c33 = ldc(&c.C33[si]), c55 = ldc(&c.C55[si]);
+            const float rinv = ldc(&c.rhos_inv[si]);
+            const int dist = INT ? 3 : iz;
 
-        // ---- stress equations
-        {
-            const float dvxx = drp_g<DIR>(Svx, si, 1), dvzx = drp_g<DIR>(Svz, si, 1);
-            const float dvxz = mac_g<DIR>(Svx, si, P, iz, true), dvzz = mac_g<DIR>(Svz, si, P, iz, true);
-            const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
-            float A11 = exx, A13 = ezz, A55 = xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz;
-            const float exxb = c11 * qsx + c13 * qsz, ezzb = c13 * qsx + c33 * qsz;
-            float bxx = xx * exxb + c55 * xz * qsxz, bxz = zx * exxb + c55 * zz * qsxz;
-            float bzx = xz * ezzb + c55 * xx * qsxz, bzz = zz * ezzb + c55 * zx * qsxz;
-            if (side) {
-                const float gx = gxize / gxixi;
-                const float pxa = c.pxa[ix], pxb = c.pxb[ix], pxc = c.pxc[ix], q1 = pxc - 1.0f;
-                const long long ai = (long long)iz * g.W2 + scol;
-                const float Ex = S[g.off[MSFDM_SXEX] + ai], Ez = S[g.off[MSFDM_SXEZ] + ai];
-                const float qEx = kbar(a, g.off[MSFDM_SXEX] + ai), qEz = kbar(a, g.off[MSFDM_SXEZ] + ai);
-                const float Dvxb = q1 * xx * exxb + c55 * q1 * xz * qsxz + pxb * qEx;
-                const float Dvzb = q1 * xz * ezzb + c55 * q1 * xx * qsxz + pxb * qEz;
-                a.TB[g.off[MSFDM_SXEX] + ai] += -pxc * xx * exxb - pxc * c55 * xz * qsxz - pxa * qEx;
-                a.TB[g.off[MSFDM_SXEZ] + ai] += -pxc * xz * ezzb - pxc * c55 * xx * qsxz - pxa * qEz;
-                bxx += Dvxb; bxz += gx * Dvxb; bzx += Dvzb; bzz += gx * Dvzb;
-                const float Dvx = dvxx + gx * dvxz, Dvz = dvzx + gx * dvzz;
-                A11 += q1 * xx * Dvx - pxc * xx * Ex;
-                A13 += q1 * xz * Dvz - pxc * xz * Ez;
-                A55 += q1 * (xz * Dvx + xx * Dvz) - pxc * xz * Ex - pxc * xx * Ez;
+            // ---- stress equations
+            {
+                const float dvxx = dx_lds<DIR>(svx[r], sc), dvzx = dx_lds<DIR>(svz[r], sc);
+                const float dvxz = mac_dz<DIR>(svx, r, sc, dist, true), dvzz = mac_dz<DIR>(svz, r, sc, dist, true);
+                const float exx = xx * dvxx + zx * dvxz, ezz = xz * dvzx + zz * dvzz;
+                float A11 = exx, A13 = ezz, A55 = xz * dvxx + zz * dvxz + xx * dvzx + zx * dvzz;
+                const float exxb = c11 * qsx + c13 * qsz, ezzb = c13 * qsx + c33 * qsz;
+                float bxx = xx * exxb + c55 * xz * qsxz, bxz = zx * exxb + c55 * zz * qsxz;
+                float bzx = xz * ezzb + c55 * xx * qsxz, bzz = zz * ezzb + c55 * zx * qsxz;
+                if (side) {
+                    const float gx = gxize / gxixi;
+                    const float pxa = ldc(&c.pxa[ix]), pxb = ldc(&c.pxb[ix]), pxc = ldc(&c.pxc[ix]), q1 = pxc - 1.0f;
+                    const long long ai = (long long)iz * g.W2 + scol;
+                    const float Ex = S[g.off[MSFDM_SXEX] + ai], Ez = S[g.off[MSFDM_SXEZ] + ai];
+                    const float qEx = kbar(a, g.off[MSFDM_SXEX] + ai), qEz = kbar(a, g.off[MSFDM_SXEZ] + ai);
+                    const float Dvxb = q1 * xx * exxb + c55 * q1 * xz * qsxz + pxb * qEx;
+                    const float Dvzb = q1 * xz * ezzb + c55 * q1 * xx * qsxz + pxb * qEz;
+                    a.TB[g.off[MSFDM_SXEX] + ai] = -pxc * xx * exxb - pxc * c55 * xz * qsxz - pxa * qEx;
+                    a.TB[g.off[MSFDM_SXEZ] + ai] = -pxc * xz * ezzb - pxc * c55 * xx * qsxz - pxa * qEz;
+                    bxx += Dvxb; bxz += gx * Dvxb; bzx += Dvzb; bzz += gx * Dvzb;
+                    const float Dvx = dvxx + gx * dvxz, Dvz = dvzx + gx * dvzz;
+                    A11 += q1 * xx * Dvx - pxc * xx * Ex;
+                    A13 += q1 * xz * Dvz - pxc * xz * Ez;
+                    A55 += q1 * (xz * Dvx + xx * Dvz) - pxc * xz * Ex - pxc * xx * Ez;
+                }
+                if (bottom) {
+                    const float gz = gxize / gzeze;
+                    const float pza = ldc(&c.pza[iz]), pzb = ldc(&c.pzb[iz]), pzc = ldc(&c.pzc[iz]), q1 = pzc - 1.0f;
+                    const long long ai = (long long)(iz - zb0) * P + ix;
+                    const float Ex = S[g.off[MSFDM_SZEX] + ai], Ez = S[g.off[MSFDM_SZEZ] + ai];
+                    const float qEx = kbar(a, g.off[MSFDM_SZEX] + ai), qEz = kbar(a, g.off[MSFDM_SZEZ] + ai);
+                    const float Dvxb = q1 * zx * exxb + c55 * q1 * zz * qsxz + pzb * qEx;
+                    const float Dvzb = q1 * zz * ezzb + c55 * q1 * zx * qsxz + pzb * qEz;
+                    a.TB[g.off[MSFDM_SZEX] + ai] = -pzc * zx * exxb - pzc * c55 * zz * qsxz - pza * qEx;
+                    a.TB[g.off[MSFDM_SZEZ] + ai] = -pzc * zz * ezzb - pzc * c55 * zx * qsxz - pza * qEz;
+                    bxz += Dvxb; bxx += gz * Dvxb; bzz += Dvzb; bzx += gz * Dvzb;
+                    const float Dvx = dvxz + gz * dvxx, Dvz = dvzz + gz * dvzx;
+                    A11 += q1 * zx * Dvx - pzc * zx * Ex;
+                    A13 += q1 * zz * Dvz - pzc * zz * Ez;
+                    A55 += q1 * (zz * Dvx + zx * Dvz) - pzc * zz * Ex - pzc * zx * Ez;
+                }
+                q[0] = bxx; q[1] = bzx; q[2] = bxz; q[3] = bzz;
+                gr.C11[si] = g11 + (double)qsx * (double)A11;
+                gr.C13[si] = g13 + ((double)qsx * (double)A13 + (double)qsz * (double)A11);
+                gr.C33[si] = g33 + (double)qsz * (double)A13;
+                gr.C55[si] = g55 + (double)qsxz * (double)A55;
             }
-            if (bottom) {
-                const float gz = gxize / gzeze;
-                const float pza = c.pza[iz], pzb = c.pzb[iz], pzc = c.pzc[iz], q1 = pzc - 1.0f;
-                const long long ai = (long long)(iz - zb0) * P + ix;
-                const float Ex = S[g.off[MSFDM_SZEX] + ai], Ez = S[g.off[MSFDM_SZEZ] + ai];
-                const float qEx = kbar(a, g.off[MSFDM_SZEX] + ai), qEz = kbar(a, g.off[MSFDM_SZEZ] + ai);
-                const float Dvxb = q1 * zx * exxb + c55 * q1 * zz * qsxz + pzb * qEx;
-                const float Dvzb = q1 * zz * ezzb + c55 * q1 * zx * qsxz + pzb * qEz;
-                a.TB[g.off[MSFDM_SZEX] + ai] += -pzc * zx * exxb - pzc * c55 * zz * qsxz - pza * qEx;
-                a.TB[g.off[MSFDM_SZEZ] + ai] += -pzc * zz * ezzb - pzc * c55 * zx * qsxz - pza * qEz;
-                bxz += Dvxb; bxx += gz * Dvxb; bzz += Dvzb; bzx += gz * Dvzb;
-                const float Dvx = dvxz + gz * dvxx, Dvz = dvzz + gz * dvzx;
-                A11 += q1 * zx * Dvx - pzc * zx * Ex;
-                A13 += q1 * zz * Dvz - pzc * zz * Ez;
-                A55 += q1 * (zz * Dvx + zx * Dvz) - pzc * zz * Ex - pzc * zx * Ez;
-            }
-            float xa[5], xb[5];
+            // ---- velocity equations
+            {
+                const float rvxb = qvx * rinv, rvzb = qvz * rinv;
+                float Dxxb = 0.f, Dxzb = 0.f, Dzxb = 0.f, Dzzb = 0.f;
+                float q1x = 0.f, pxc = 1.f, q1z = 0.f, pzc = 1.f;
+                float Vxs = 0.f, Vzs = 0.f, Vxb = 0.f, Vzb = 0.f;
+                if (side) {
+                    const float pxa = ldc(&c.pxa[ix]), pxb = ldc(&c.pxb[ix]);
+                    pxc = ldc(&c.pxc[ix]); q1x = pxc - 1.0f;
+                    const long long ai = (long long)iz * g.W2 + scol;
+                    const float qVx = kbar(a, g.off[MSFDM_SXVX] + ai), qVz = kbar(a, g.off[MSFDM_SXVZ] + ai);
+                    Dxxb = q1x * rinv * qvx + pxb * qVx;
+                    Dxzb = q1x * rinv * qvz + pxb * qVz;
+                    a.TB[g.off[MSFDM_SXVX] + ai] = -pxc * rinv * qvx - pxa * qVx;
+                    a.TB[g.off[MSFDM_SXVZ] + ai] = -pxc * rinv * qvz - pxa * qVz;
+                    Vxs = S[g.off[MSFDM_SXVX] + ai]; Vzs = S[g.off[MSFDM_SXVZ] + ai];
+                }
+                if (bottom) {
+                    const float pza = ldc(&c.pza[iz]), pzb = ldc(&c.pzb[iz]);
+                    pzc = ldc(&c.pzc[iz]); q1z = pzc - 1.0f;
+                    const long long ai = (long long)(iz - zb0) * P + ix;
+                    const float qVx = kbar(a, g.off[MSFDM_SZVX] + ai), qVz = kbar(a, g.off[MSFDM_SZVZ] + ai);
+                    Dzxb = q1z * rinv * qvx + pzb * qVx;
+                    Dzzb = q1z * rinv * qvz + pzb * qVz;
+                    a.TB[g.off[MSFDM_SZVX] + ai] = -pzc * rinv * qvx - pza * qVx;
+                    a.TB[g.off[MSFDM_SZVZ] + ai] = -pzc * rinv * qvz - pza * qVz;
+                    Vxb = S[g.off[MSFDM_SZVX] + ai]; Vzb = S[g.off[MSFDM_SZVZ] + ai];
+                }
+                float rvx, rvz, Dxx = 0.f, Dxz = 0.f, Dzx = 0.f, Dzz = 0.f;
+                if (!INT && iz < 3) {
+                    // ghost-flux rows 0..2: rho v_k = (D_xi(J sigma xi) + D_ze(J sigma ze)) / J
+                    auto fze_row = [&](int rr, float &fx, float &fz) {
+                        const long long qq = moff + (long long)rr * P + ix;
+                        const int tr = r + (rr - iz);
+                        const float Jq = ldc(&c.J[qq]), zxq = ldc(&c.zex[qq]), zzq = ldc(&c.zez[qq]);
+                        fx = Jq * (zxq * ssx[tr][sc] + zzq * ssxz[tr][sc]);
+                        fz = Jq * (zxq * ssxz[tr][sc] + zzq * ssz[tr][sc]);
+                    };
+                    float sx = 0.f, sz = 0.f;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                xa[q] = drp_w<DIR>(q) * bxx;
-                xb[q] = drp_w<DIR>(q) * bzx;
-            }
-            addx(0, iz, xa);
-            addx(1, iz, xb);
-            mac_dz_adj<DIR>(iz, true, bxz, [&](int m, float v) { add(0, iz + m, v); });
-            mac_dz_adj<DIR>(iz, true, bzz, [&](int m, float v) { add(1, iz + m, v); });
-            if (act) {
-                gr.C11[si] += (double)qsx * (double)A11;
-                gr.C13[si] += (double)qsx * (double)A13 + (double)qsz * (double)A11;
-                gr.C33[si] += (double)qsz * (double)A13;
-                gr.C55[si] += (double)qsxz * (double)A55;
+                    for (int k = 0; k < 5; ++k) {
+                        const int m = o + k;
+                        const float w = drp_w<DIR>(k);
+                        const long long gq = gi + m;
+                        const float Jq = ldc(&c.J[gq]);
+                        float xxq, xzq;
+                        xi_metrics<MET>(c, gq, xxq, xzq);
+                        sx += w * Jq * (xxq * ssx[r][sc + m] + xzq * ssxz[r][sc + m]);
+                        sz += w * Jq * (xxq * ssxz[r][sc + m] + xzq * ssz[r][sc + m]);
+                        const int rr = iz + m;
+                        float fbx, fbz;
+                        if (rr >= 0) fze_row(rr, fbx, fbz);
+                        else { float f0x, f0z, fnx, fnz; fze_row(0, f0x, f0z); fze_row(-rr, fnx, fnz); fbx = 2.f * f0x - fnx; fbz = 2.f * f0z - fnz; }
+                        sx += w * fbx;
+                        sz += w * fbz;
+                    }
+                    rvx = sx / J; rvz = sz / J;
+                    const float pxi = (xx * sx + xz * sz) / (gxixi * J), pze = (zx * sx + zz * sz) / (gzeze * J);
+                    Dxx = xx * pxi; Dxz = xz * pxi; Dzx = zx * pze; Dzz = zz * pze;
+                    const float pxib = xx * Dxxb + xz * Dxzb, pzeb = zx * Dzxb + zz * Dzzb;
+                    q[4] = rvxb / J + xx * pxib / (gxixi * J) + zx * pzeb / (gzeze * J);   // sx_bar
+                    q[5] = rvzb / J + xz * pxib / (gxixi * J) + zz * pzeb / (gzeze * J);   // sz_bar
+                } else {
+                    const float A = dx_lds<DIR>(ssx[r], sc), Bv = dz_lds<DIR>(ssx, r, sc);
+                    const float Cv = dx_lds<DIR>(ssz[r], sc), Dv = dz_lds<DIR>(ssz, r, sc);
+                    const float E = dx_lds<DIR>(ssxz[r], sc), Fv = dz_lds<DIR>(ssxz, r, sc);
+                    rvx = xx * A + zx * Bv + xz * E + zz * Fv;
+                    rvz = xx * E + zx * Fv + xz * Cv + zz * Dv;
+                    if (INT) {
+                        q[4] = xx * rvxb;                   // sxx along x
+                        q[5] = xz * rvzb;                   // szz along x
+                        q[6] = xz * rvxb + xx * rvzb;       // sxz along x
+                        q[7] = zx * rvxb;                   // sxx along z
+                        q[8] = zz * rvzb;                   // szz along z
+                        q[9] = zz * rvxb + zx * rvzb;       // sxz along z
+                    } else {
+                        const float gx = gxize / gxixi, gz = gxize / gzeze;
+                        Dxx = xx * (A + gx * Bv) + xz * (E + gx * Fv);
+                        Dxz = xx * (E + gx * Fv) + xz * (Cv + gx * Dv);
+                        Dzx = zx * (Bv + gz * A) + zz * (gz * E + Fv);
+                        Dzz = zx * (gz * E + Fv) + zz * (gz * Cv + Dv);
+                        q[4] = xx * rvxb + xx * Dxxb + zx * gz * Dzxb;
+                        q[5] = xz * rvzb + xz * Dxzb + zz * gz * Dzzb;
+                        q[6] = xz * rvxb + xx * rvzb + xz * Dxxb + xx * Dxzb + zz * gz * Dzxb + zx * gz * Dzzb;
+                        q[7] = zx * rvxb + xx * gx * Dxxb + zx * Dzxb;
+                        q[8] = zz * rvzb + xz * gx * Dxzb + zz * Dzzb;
+                        q[9] = zz * rvxb + zx * rvzb + xz * gx * Dxxb + xx * gx * Dxzb + zz * Dzxb + zx * Dzzb;
+                    }
+                }
+                float kvx = rvx, kvz = rvz;
+                if (side) { kvx += q1x * Dxx - pxc * Vxs; kvz += q1x * Dxz - pxc * Vzs; }
+                if (bottom) { kvx += q1z * Dzx - pzc * Vxb; kvz += q1z * Dzz - pzc * Vzb; }
+                gr.rho_s[si] = grs + -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
             }
         }
-        // ---- velocity equations
-        {
-            const float rvxb = qvx * rinv, rvzb = qvz * rinv;
-            float Dxxb = 0.f, Dxzb = 0.f, Dzxb = 0.f, Dzzb = 0.f;
-            float q1x = 0.f, pxc = 1.f, q1z = 0.f, pzc = 1.f;
-            float Vxs = 0.f, Vzs = 0.f, Vxb = 0.f, Vzb = 0.f;
-            if (side) {
-                const float pxa = c.pxa[ix], pxb = c.pxb[ix];
-                pxc = c.pxc[ix]; q1x = pxc - 1.0f;
-                const long long ai = (long long)iz * g.W2 + scol;
-                const float qVx = kbar(a, g.off[MSFDM_SXVX] + ai), qVz = kbar(a, g.off[MSFDM_SXVZ] + ai);
-                Dxxb = q1x * rinv * qvx + pxb * qVx;
-                Dxzb = q1x * rinv * qvz + pxb * qVz;
-                a.TB[g.off[MSFDM_SXVX] + ai] += -pxc * rinv * qvx - pxa * qVx;
-                a.TB[g.off[MSFDM_SXVZ] + ai] += -pxc * rinv * qvz - pxa * qVz;
-                Vxs = S[g.off[MSFDM_SXVX] + ai]; Vzs = S[g.off[MSFDM_SXVZ] + ai];
-            }
-            if (bottom) {
-                const float pza = c.pza[iz], pzb = c.pzb[iz];
-                pzc = c.pzc[iz]; q1z = pzc - 1.0f;
-                const long long ai = (long long)(iz - zb0) * P + ix;
-                const float qVx = kbar(a, g.off[MSFDM_SZVX] + ai), qVz = kbar(a, g.off[MSFDM_SZVZ] + ai);
-                Dzxb = q1z * rinv * qvx + pzb * qVx;
-                Dzzb = q1z * rinv * qvz + pzb * qVz;
-                a.TB[g.off[MSFDM_SZVX] + ai] += -pzc * rinv * qvx - pza * qVx;
-                a.TB[g.off[MSFDM_SZVZ] + ai] += -pzc * rinv * qvz - pza * qVz;
-                Vxb = S[g.off[MSFDM_SZVX] + ai]; Vzb = S[g.off[MSFDM_SZVZ] + ai];
-            }
-            float rvx, rvz, Dxx, Dxz, Dzx, Dzz;
-            constexpr int o = lo_off<DIR>();
-            if (iz < 3) {
-                auto fze_row = [&](int rr, float &fx, float &fz) {
-                    const long long q = moff + (long long)rr * P + ix, sq = (long long)rr * P + ix;
-                    const float Jq = c.J[q], zxq = c.zex[q], zzq = c.zez[q];
-                    fx = Jq * (zxq * Ssx[sq] + zzq * Ssxz[sq]);
-                    fz = Jq * (zxq * Ssxz[sq] + zzq * Ssz[sq]);
-                };
-                float sx = 0.f, sz = 0.f;
 #pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    const int m = o + q;
-                    const float w = drp_w<DIR>(q);
-                    const long long gq = gi + m, sq = si + m;
-                    const float Jq = c.J[gq];
-                    float xxq, xzq;
-                    xi_metrics<2>(c, gq, xxq, xzq);
-                    sx += w * Jq * (xxq * Ssx[sq] + xzq * Ssxz[sq]);
-                    sz += w * Jq * (xxq * Ssxz[sq] + xzq * Ssz[sq]);
-                    const int rr = iz + m;
-                    float bxv, bzv;
-                    if (rr >= 0) fze_row(rr, bxv, bzv);
-                    else { float f0x, f0z, fnx, fnz; fze_row(0, f0x, f0z); fze_row(-rr, fnx, fnz); bxv = 2.f * f0x - fnx; bzv = 2.f * f0z - fnz; }
-                    sx += w * bxv;
-                    sz += w * bzv;
-                }
-                rvx = sx / J; rvz = sz / J;
-                const float pxi = (xx * sx + xz * sz) / (gxixi * J), pze = (zx * sx + zz * sz) / (gzeze * J);
-                Dxx = xx * pxi; Dxz = xz * pxi; Dzx = zx * pze; Dzz = zz * pze;
-                const float pxib = xx * Dxxb + xz * Dxzb, pzeb = zx * Dzxb + zz * Dzzb;
-                const float sxb = rvxb / J + xx * pxib / (gxixi * J) + zx * pzeb / (gzeze * J);
-                const float szb = rvzb / J + xz * pxib / (gxixi * J) + zz * pzeb / (gzeze * J);
-                auto fze_add = [&](int rr, float Fx, float Fz) {
-                    const long long q = moff + (long long)rr * P + ix;
-                    const float Jq = c.J[q], zxq = c.zex[q], zzq = c.zez[q];
-                    add(2, rr, Jq * zxq * Fx);
-                    add(4, rr, Jq * (zzq * Fx + zxq * Fz));
-                    add(3, rr, Jq * zzq * Fz);
-                };
-                float x2[5], x3[5], x4[5];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    const int m = o + q;
-                    const float w = drp_w<DIR>(q);
-                    const long long gq = gi + m;
-                    const float Jq = c.J[gq];
-                    float xxq, xzq;
-                    xi_metrics<2>(c, gq, xxq, xzq);
-                    const float Fx = w * sxb, Fz = w * szb;
-                    x2[q] = Jq * xxq * Fx;
-                    x4[q] = Jq * (xzq * Fx + xxq * Fz);
-                    x3[q] = Jq * xzq * Fz;
-                    const int rr = iz + m;
-                    if (rr >= 0) fze_add(rr, Fx, Fz);
-                    else { fze_add(0, 2.f * Fx, 2.f * Fz); fze_add(-rr, -Fx, -Fz); }
-                }
-                addx(2, iz, x2);
-                addx(4, iz, x4);
-                addx(3, iz, x3);
-            } else {
-                const float A = drp_g<DIR>(Ssx, si, 1), Bv = drp_g<DIR>(Ssx, si, P);
-                const float Cv = drp_g<DIR>(Ssz, si, 1), Dv = drp_g<DIR>(Ssz, si, P);
-                const float E = drp_g<DIR>(Ssxz, si, 1), Fv = drp_g<DIR>(Ssxz, si, P);
-                const float gx = gxize / gxixi, gz = gxize / gzeze;
-                rvx = xx * A + zx * Bv + xz * E + zz * Fv;
-                rvz = xx * E + zx * Fv + xz * Cv + zz * Dv;
-                Dxx = xx * (A + gx * Bv) + xz * (E + gx * Fv);
-                Dxz = xx * (E + gx * Fv) + xz * (Cv + gx * Dv);
-                Dzx = zx * (Bv + gz * A) + zz * (gz * E + Fv);
-                Dzz = zx * (gz * E + Fv) + zz * (gz * Cv + Dv);
-                const float ab = xx * rvxb + xx * Dxxb + zx * gz * Dzxb;
-                const float bb = zx * rvxb + xx * gx * Dxxb + zx * Dzxb;
-                const float cb = xz * rvzb + xz * Dxzb + zz * gz * Dzzb;
-                const float db = zz * rvzb + xz * gx * Dxzb + zz * Dzzb;
-                const float eb = xz * rvxb + xx * rvzb + xz * Dxxb + xx * Dxzb + zz * gz * Dzxb + zx * gz * Dzzb;
-                const float fb = zz * rvxb + zx * rvzb + xz * gx * Dxxb + xx * gx * Dxzb + zz * Dzxb + zx * Dzzb;
-                float x2[5], x3[5], x4[5];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    const float w = drp_w<DIR>(q);
-                    x2[q] = w * ab;
-                    add(2, iz + o + q, w * bb);
-                    x3[q] = w * cb;
-                    add(3, iz + o + q, w * db);
-                    x4[q] = w * eb;
-                    add(4, iz + o + q, w * fb);
-                }
-                addx(2, iz, x2);
-                addx(3, iz, x3);
-                addx(4, iz, x4);
-            }
-            float kvx = rvx, kvz = rvz;
-            if (side) { kvx += q1x * Dxx - pxc * Vxs; kvz += q1x * Dxz - pxc * Vzs; }
-            if (bottom) { kvx += q1z * Dzx - pzc * Vxb; kvz += q1z * Dzz - pzc * Vzb; }
-            if (act) gr.rho_s[si] += -(double)(qvx * kvx + qvz * kvz) * (double)rinv * (double)rinv;
-        }
+        for (int k = 0; k < ADJ_NQS; ++k) Q[k * pP + si] = q[k];
     }
 }
 
-// Zero the per-wave LDS accumulators, run the medium body, add the tile and
-// its halo to the output.  Wave w owns tile rows w*RW .. w*RW+RW-1 and
-// accumulates into its own copy covering rows w*RW-3 .. w*RW+RW+2; no other
-// wave touches the copy and each cell of it has one writing lane, so every LDS
-// update is a plain read-modify-write.  The flush adds the overlapping copies
-// in wave order; no other block of the launch (one tile colour) covers the
-// same output cells.
-template <int NF, int TZ>
-__device__ __forceinline__ void acc_zero(WaveAcc<TZ> *acc) {
-    constexpr int RW = TZ / 4, N = (RW + 2 * HALO) * LXW;
-    for (int w = 0; w < 4; ++w)
-        for (int f = 0; f < NF; ++f)
-            for (int e = threadIdx.x; e < N; e += NTHR) {
-                const int r = e / LXW, cc = e - r * LXW;
-                acc[w].v[f][r][cc] = 0.0f;
-            }
-}
-template <int NF, int TZ>
-__device__ __forceinline__ void acc_flush(WaveAcc<TZ> *acc, float *TB, const long long *off, int f0, int z0, int x0,
-                                          int nrows, int P) {
-    constexpr int RW = TZ / 4, N = (TZ + 2 * HALO) * LXW;
-    for (int f = 0; f < NF; ++f) {
-        float *dst = TB + off[f0 + f];
-        for (int e = threadIdx.x; e < N; e += NTHR) {
-            const int r = e / LXW, cc = e - r * LXW;   // tile row r - HALO
-            float v = 0.0f;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const int lr = r - w * RW;
-                if (lr >= 0 && lr < RW + 2 * HALO) v += acc[w].v[f][lr][cc];
-            }
-            const int gz = z0 - HALO + r, gx = x0 - HALO + cc;
-            if (v != 0.0f && gz >= 0 && gz < nrows && gx >= 0 && gx < P) dst[(long long)gz * P + gx] += v;
-        }
-    }
-}
-
-// Tiles of colour `color` (bit 0: tile column parity, bit 1: band parity within
-// its medium).  Two tiles of one colour are at least one tile apart in x (64
-// columns) and in z (TZ >= 8 rows), more than the 2 x 3-cell halos of their
-// accumulators.
-__host__ __device__ __forceinline__ int adj_color_tiles(int nbx, int nbz_f, int nbz_s, int color) {
-    const int px = color & 1, pz = color >> 1;
-    return ((nbx - px + 1) / 2) * ((nbz_f - pz + 1) / 2 + (nbz_s - pz + 1) / 2);
-}
-
-template <int DIR, int TZ>
+template <int DIR, int TZ, int MET>
 __global__ void __launch_bounds__(NTHR)
-adj_stage_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int nbz_s, int color) {
-    static_assert(TZ >= 2 * HALO, "tiles of one colour must not share accumulator cells");
-    __shared__ WaveAcc<TZ> acc[4];
+adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int nbz_s) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];   // forward stage state of the tile + halo
     const AdjIn a = adj_of_shot(a0, blockIdx.y);
     const Grad gr = grad_of_shot(gr0, blockIdx.y);
     const int per_xcd = gridDim.x >> 3;
     const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (t >= adj_color_tiles(nbx, nbz_f, nbz_s, color)) return;
-    const int px = color & 1, pz = color >> 1;
-    const int nbxc = (nbx - px + 1) / 2, nfc = (nbz_f - pz + 1) / 2;
-    const int bc = t / nbxc, bx = 2 * (t - bc * nbxc) + px;
-    if (bc < nfc) {
-        const int band = 2 * bc + pz;
-        acc_zero<3, TZ>(acc);
+    if (t >= nbx * (nbz_f + nbz_s)) return;
+    const int band = t / nbx, bx = t - band * nbx;
+    if (band < nbz_f) {
+        const int z0 = 3 + band * TZ;
+        load_tile<3, TZ>(sh, a.S, g.off, MSFDM_FVX, z0, bx * TX, g.nzf, g.P);
         __syncthreads();
-        fluid_adj_body<DIR, TZ>(acc, g, c, a, gr, bx, band);
-        __syncthreads();
-        acc_flush<3, TZ>(acc, a.TB, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);
+        if (adj_fluid_interior(g, bx * TX, z0, TZ)) fluid_prod_body<DIR, TZ, MET, 1>(sh, g, c, a, gr, bx, band);
+        else fluid_prod_body<DIR, TZ, MET, 0>(sh, g, c, a, gr, bx, band);
     } else {
-        const int band = 2 * (bc - nfc) + pz;
-        acc_zero<5, TZ>(acc);
+        const int sb = band - nbz_f;
+        load_tile<5, TZ>(sh, a.S, g.off, MSFDM_SVX, sb * TZ, bx * TX, g.nzs, g.P);
         __syncthreads();
-        solid_adj_body<DIR, TZ>(acc, g, c, a, gr, bx, band);
+        if (adj_solid_interior(g, bx * TX, sb * TZ, TZ)) solid_prod_body<DIR, TZ, MET, 1>(sh, g, c, a, gr, bx, sb);
+        else solid_prod_body<DIR, TZ, MET, 0>(sh, g, c, a, gr, bx, sb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather: Sbar at the output cells of a 64 x TZ tile from the products of the
+// tile and its 3-cell halo (LDS).  A source row's stencil along x is always
+// the DRP (the ghost-flux rows weigh it with the output cell's J xi); along z
+// it depends on the source row: DRP, a closure, or a ghost-flux row whose
+// taps beyond the boundary reflect into rows inside.
+// ---------------------------------------------------------------------------
+// Weight of ghost-flux source row s at output row t: the taps rr = s + m of
+// its DRP stencil, reflected as the forward ghosts are (interface: 2 F(nzf-1)
+// - F(2 (nzf-1) - rr); fluid surface: -F(6 - rr); solid below the interface:
+// 2 F(0) - F(-rr)).
+constexpr int GH_IF = 0, GH_SURF = 1, GH_SOLID = 2;
+template <int DIR, int GH>
+__device__ __forceinline__ float ghost_at(int s, int t, int nzf) {
+    float w = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int rr = s + lo_off<DIR>() + k;
+        const float wk = drp_w<DIR>(k);
+        if constexpr (GH == GH_IF) {
+            if (rr <= nzf - 1) w += rr == t ? wk : 0.0f;
+            else w += (t == nzf - 1 ? 2.0f * wk : 0.0f) - (t == 2 * (nzf - 1) - rr ? wk : 0.0f);
+        } else if constexpr (GH == GH_SURF) {
+            if (rr >= 3) w += rr == t ? wk : 0.0f;
+            else w -= t == 6 - rr ? wk : 0.0f;
+        } else {
+            if (rr >= 0) w += rr == t ? wk : 0.0f;
+            else w += (t == 0 ? 2.0f * wk : 0.0f) - (t == -rr ? wk : 0.0f);
+        }
+    }
+    return w;
+}
+
+// sum over k of w_k * v[c - o - k]: the x-direction transpose of a DRP row.
+template <int DIR>
+__device__ __forceinline__ float xsum(const float *row, int c) {
+    constexpr int o = lo_off<DIR>();
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s += drp_w<DIR>(k) * row[c - o - k];
+    return s;
+}
+// sum over k of w_k * v[r - o - k][c]: the z-direction transpose of DRP rows.
+template <int DIR>
+__device__ __forceinline__ float zsum(const float (*t)[LXP], int r, int c) {
+    constexpr int o = lo_off<DIR>();
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s += drp_w<DIR>(k) * t[r - o - k][c];
+    return s;
+}
+
+// Fluid output rows 0..nzf-1 of one tile (z0 = bz * TZ).
+template <int DIR, int TZ, int MET>
+__device__ __forceinline__ void fluid_gather_body(const float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+                                                  const AdjIn &a, int bx, int bz) {
+    const int tx = threadIdx.x & (TX - 1), ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
+    constexpr int RW = TZ / 4;
+    const int x0 = bx * TX, z0 = bz * TZ;
+    const int P = g.P, nzf = g.nzf;
+    const int sc = tx + XOFF, ix = x0 + tx;
+    auto flux_row = [&](int s) { return s >= 3 && s <= nzf - 1 && (nzf - 1 - s < 3 || s - 3 < 3); };
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int t = z0 + ty * RW + j;
+        if (t > nzf - 1) break;   // uniform over the wave
+        const int r = t - z0 + HALO;
+        const long long gi = (long long)t * P + ix;
+        float bp, bvx, bvz;
+        if (t >= 9 && t <= nzf - 7) {
+            // every source row within 3 rows is a plain DRP row
+            bp = xsum<DIR>(sh[0][r], sc) + zsum<DIR>(sh[1], r, sc);
+            bvx = xsum<DIR>(sh[2][r], sc) + zsum<DIR>(sh[4], r, sc);
+            bvz = xsum<DIR>(sh[3][r], sc) + zsum<DIR>(sh[5], r, sc);
+        } else {
+            // x: the source row is this row
+            if (flux_row(t)) {
+                float xx, xz;
+                xi_metrics<MET>(c, gi, xx, xz);
+                bp = -ldc(&c.J[gi]) * (xx * xsum<DIR>(sh[0][r], sc) + xz * xsum<DIR>(sh[1][r], sc));
+            } else {
+                bp = xsum<DIR>(sh[0][r], sc);
+            }
+            bvx = xsum<DIR>(sh[2][r], sc);
+            bvz = xsum<DIR>(sh[3][r], sc);
+            // z: every source row s within 3 rows, with its own stencil
+            float gx = 0.0f, gz = 0.0f;
+            for (int s = max(t - 3, 3); s <= min(t + 3, nzf - 1); ++s) {
+                const int rs = s - z0 + HALO;
+                if (flux_row(s)) {
+                    const float w = nzf - 1 - s < 3 ? ghost_at<DIR, GH_IF>(s, t, nzf) : ghost_at<DIR, GH_SURF>(s, t, nzf);
+                    gx += w * sh[0][rs][sc];
+                    gz += w * sh[1][rs][sc];
+                } else {
+                    bp += drp_at<DIR>(t - s) * sh[1][rs][sc];
+                }
+                if (s > 3) {   // pressure equation (none on the surface row 3)
+                    const float w = mac_at<DIR>(min(s - 3, nzf - 1 - s), false, t - s);
+                    bvx += w * sh[4][rs][sc];
+                    bvz += w * sh[5][rs][sc];
+                }
+            }
+            bp += -ldc(&c.J[gi]) * (ldc(&c.zex[gi]) * gx + ldc(&c.zez[gi]) * gz);
+        }
+        a.TB[g.off[MSFDM_FVX] + gi] = bvx;
+        a.TB[g.off[MSFDM_FVZ] + gi] = bvz;
+        a.TB[g.off[MSFDM_FP] + gi] = bp;
+    }
+}
+
+// Solid output rows 0..nzs-1 of one tile (z0 = bz * TZ).
+template <int DIR, int TZ, int MET>
+__device__ __forceinline__ void solid_gather_body(const float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+                                                  const AdjIn &a, int bx, int bz) {
+    const int tx = threadIdx.x & (TX - 1), ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
+    constexpr int RW = TZ / 4;
+    const int x0 = bx * TX, z0 = bz * TZ;
+    const int P = g.P, nzs = g.nzs;
+    const int sc = tx + XOFF, ix = x0 + tx;
+    const long long moff = (long long)(g.nzf - 1) * P;
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int t = z0 + ty * RW + j;
+        if (t > nzs - 1) break;   // uniform over the wave
+        const int r = t - z0 + HALO;
+        const long long si = (long long)t * P + ix;
+        float bvx, bvz, bsx, bsz, bsxz;
+        if (t >= 6) {
+            // every source row within 3 rows is a plain DRP row
+            bvx = xsum<DIR>(sh[0][r], sc) + zsum<DIR>(sh[2], r, sc);
+            bvz = xsum<DIR>(sh[1][r], sc) + zsum<DIR>(sh[3], r, sc);
+            bsx = xsum<DIR>(sh[4][r], sc) + zsum<DIR>(sh[7], r, sc);
+            bsz = xsum<DIR>(sh[5][r], sc) + zsum<DIR>(sh[8], r, sc);
+            bsxz = xsum<DIR>(sh[6][r], sc) + zsum<DIR>(sh[9], r, sc);
+        } else {
+            bvx = xsum<DIR>(sh[0][r], sc);
+            bvz = xsum<DIR>(sh[1][r], sc);
+            const long long gi = moff + si;
+            const float Jt = ldc(&c.J[gi]), zxt = ldc(&c.zex[gi]), zzt = ldc(&c.zez[gi]);
+            if (t <= 2) {   // ghost-flux row: x taps weighed with this cell's J xi
+                float xx, xz;
+                xi_metrics<MET>(c, gi, xx, xz);
+                const float fx = xsum<DIR>(sh[4][r], sc), fz = xsum<DIR>(sh[5][r], sc);
+                bsx = Jt * xx * fx;
+                bsz = Jt * xz * fz;
+                bsxz = Jt * (xz * fx + xx * fz);
+            } else {
+                bsx = xsum<DIR>(sh[4][r], sc);
+                bsz = xsum<DIR>(sh[5][r], sc);
+                bsxz = xsum<DIR>(sh[6][r], sc);
+            }
+            float fx = 0.0f, fz = 0.0f;
+            for (int s = max(t - 3, 0); s <= min(t + 3, nzs - 4); ++s) {
+                const int rs = s - z0 + HALO;
+                const float wm = mac_at<DIR>(s, true, t - s);   // stress equations: mac_dz at distance s
+                bvx += wm * sh[2][rs][sc];
+                bvz += wm * sh[3][rs][sc];
+                if (s <= 2) {   // ghost-flux velocity rows
+                    const float w = ghost_at<DIR, GH_SOLID>(s, t, 0);
+                    fx += w * sh[4][rs][sc];
+                    fz += w * sh[5][rs][sc];
+                } else {
+                    const float w = drp_at<DIR>(t - s);
+                    bsx += w * sh[7][rs][sc];
+                    bsz += w * sh[8][rs][sc];
+                    bsxz += w * sh[9][rs][sc];
+                }
+            }
+            bsx += Jt * zxt * fx;
+            bsz += Jt * zzt * fz;
+            bsxz += Jt * (zzt * fx + zxt * fz);
+        }
+        a.TB[g.off[MSFDM_SVX] + si] = bvx;
+        a.TB[g.off[MSFDM_SVZ] + si] = bvz;
+        a.TB[g.off[MSFDM_SSX] + si] = bsx;
+        a.TB[g.off[MSFDM_SSZ] + si] = bsz;
+        a.TB[g.off[MSFDM_SSXZ] + si] = bsxz;
+    }
+}
+
+// Output tiles: P / 64 columns x (fluid bands of rows 0..nzf-1, then solid
+// bands of rows 0..nzs-1).
+template <int DIR, int TZ, int MET>
+__global__ void __launch_bounds__(NTHR)
+adj_gather_kernel(Geo g, Coef c, AdjIn a0, int nbx, int nbz_f, int nbz_s) {
+    __shared__ float sh[ADJ_NQS][TZ + 2 * HALO][LXP];
+    const AdjIn a = adj_of_shot(a0, blockIdx.y);
+    const int per_xcd = gridDim.x >> 3;
+    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= nbx * (nbz_f + nbz_s)) return;
+    const int band = t / nbx, bx = t - band * nbx;
+    long long poff[ADJ_NQS];
+    if (band < nbz_f) {
+#pragma unroll
+        for (int k = 0; k < ADJ_NQF; ++k) poff[k] = prod_fluid(g, k);
+        load_tile<ADJ_NQF, TZ>(sh, a.prod, poff, 0, band * TZ, bx * TX, g.nzf, g.P);
         __syncthreads();
-        acc_flush<5, TZ>(acc, a.TB, g.off, MSFDM_SVX, band * TZ, bx * TX, g.nzs, g.P);
+        fluid_gather_body<DIR, TZ, MET>(sh, g, c, a, bx, band);
+    } else {
+#pragma unroll
+        for (int k = 0; k < ADJ_NQS; ++k) poff[k] = prod_solid(g, k);
+        load_tile<ADJ_NQS, TZ>(sh, a.prod, poff, 0, (band - nbz_f) * TZ, bx * TX, g.nzs, g.P);
+        __syncthreads();
+        solid_gather_body<DIR, TZ, MET>(sh, g, c, a, bx, band - nbz_f);
     }
 }
 
