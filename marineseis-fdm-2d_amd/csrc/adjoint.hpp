@@ -80,9 +80,10 @@ __device__ __forceinline__ float mac_at(int dist, bool solid_side, int m) {
 struct AdjIn {
     const float *S;      // forward stage state
     const float *LP;     // lambda'
-    const float *TBp;    // previous stage's Tbar (may be null when cprev == 0)
+    const float *TBp;    // previous stage's Tbar (LP with cprev = 0 when there is none)
     const float *qif;    // [8][P] interface qbar = B^T kbar
     float *TB;           // output Tbar
+    float *ACC;          // null, or the step's accumulator: ACC += TB wherever the stage writes TB
     float *prod;         // adjoint products: [ADJ_NQF][nzf][P], then [ADJ_NQS][nzs][P]
     float clam, cprev;
     long long bstride;   // floats between the state buffers of consecutive shots
@@ -109,8 +110,8 @@ __device__ __forceinline__ long long prod_solid(const Geo &g, int k) {
 // buffers sit at a fixed stride from shot 0's.
 __device__ __forceinline__ AdjIn adj_of_shot(AdjIn a, int shot) {
     const long long o = (long long)shot * a.bstride;
-    a.S += o; a.LP += o; a.TB += o;
-    if (a.TBp) a.TBp += o;
+    a.S += o; a.LP += o; a.TBp += o; a.TB += o;
+    if (a.ACC) a.ACC += o;
     a.qif += (long long)shot * a.qstride;
     a.prod += (long long)shot * a.pstride;
     return a;
@@ -126,20 +127,33 @@ __device__ __forceinline__ T ldc(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// Tbar[i] = v, and ACC[i] += v when the stage accumulates into ACC.
+__device__ __forceinline__ void tb_out(const AdjIn &a, long long i, float v) {
+    a.TB[i] = v;
+    if (a.ACC) a.ACC[i] += v;
+}
+
+// kbar = clam LP + cprev TBp (the host passes TBp = LP and cprev = 0 for the
+// first stage of a step, which has no previous Tbar).
 __device__ __forceinline__ float kbar(const AdjIn &a, long long i) {
     float v = a.clam * ldc(&a.LP[i]);
-    if (a.TBp) v += a.cprev * ldc(&a.TBp[i]);
+    v += a.cprev * ldc(&a.TBp[i]);
     return v;
 }
 
 // Interior source tiles (the products of a cell come from the plain DRP
 // stencils and no PML term): every column outside the side strips, every row
-// at least 3 rows from the surface, the interface and the bottom strip.
-__device__ __forceinline__ bool adj_fluid_interior(const Geo &g, int x0, int z0, int TZ) {
-    return cols_interior(g, x0) && z0 >= 6 && z0 + TZ - 1 <= g.nzf - 4;
+// at least 3 rows from the surface, the interface and the bottom strip.  Fluid
+// tiles start at row 3 + band * TZ, solid tiles at row band * TZ.  The host
+// sorts the tiles of a grid into an interior and a boundary list with these.
+__host__ __device__ inline bool adj_cols_interior(int nx, int npd, int x0) {
+    return x0 >= 3 + npd && x0 + TX - 1 <= nx - 4 - npd;
 }
-__device__ __forceinline__ bool adj_solid_interior(const Geo &g, int x0, int z0, int TZ) {
-    return cols_interior(g, x0) && z0 >= 3 && z0 + TZ - 1 < g.nzs - 3 - g.npd;
+__host__ __device__ inline bool adj_fluid_interior(int nx, int npd, int nzf, int x0, int z0, int TZ) {
+    return adj_cols_interior(nx, npd, x0) && z0 >= 6 && z0 + TZ - 1 <= nzf - 4;
+}
+__host__ __device__ inline bool adj_solid_interior(int nx, int npd, int nzs, int x0, int z0, int TZ) {
+    return adj_cols_interior(nx, npd, x0) && z0 >= 3 && z0 + TZ - 1 < nzs - 3 - npd;
 }
 
 // ---------------------------------------------------------------------------
@@ -165,6 +179,17 @@ __device__ __forceinline__ void fluid_prod_body(const float (*sh)[TZ + 2 * HALO]
     const int scol = (INT || !act) ? -1 : side_col(ix, nx, g.npd);
     float *Q = a.prod;
     const long long pP = (long long)nzf * P;
+    // the FP64 gradient sums of the wave's cells, loaded before any store of
+    // the tile (no load waits behind an earlier row's stores)
+    double grho[RW], gK[RW];
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int iz = z0 + ty * RW + j;
+        const long long gi = (long long)iz * P + ix;
+        const bool in = act && (INT || iz <= nzf - 1);
+        grho[j] = in ? gr.rho_f[gi] : 0.0;
+        gK[j] = in && iz != 3 ? gr.K[gi] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int iz = z0 + ty * RW + j;
@@ -174,8 +199,7 @@ __device__ __forceinline__ void fluid_prod_body(const float (*sh)[TZ + 2 * HALO]
         float q[ADJ_NQF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (act) {
             const int dist_flso = nzf - 1 - iz, dist_fs = iz - 3;
-            // this cell's FP64 gradient sums, loaded before the row's arithmetic
-            const double g_rho = gr.rho_f[gi], g_K = dist_fs != 0 ? gr.K[gi] : 0.0;
+            const double g_rho = grho[j], g_K = gK[j];
             const float J = ldc(&c.J[gi]), zx = ldc(&c.zex[gi]), zz = ldc(&c.zez[gi]);
             float xx, xz;
             xi_metrics<MET>(c, gi, xx, xz);
@@ -197,8 +221,8 @@ __device__ __forceinline__ void fluid_prod_body(const float (*sh)[TZ + 2 * HALO]
                 const float qVx = kbar(a, g.off[MSFDM_FXVX] + ai), qVz = kbar(a, g.off[MSFDM_FXVZ] + ai);
                 Dxb = (pxc - 1.0f) * rinv * qvx + pxb * qVx;
                 Dzb = (pxc - 1.0f) * rinv * qvz + pxb * qVz;
-                a.TB[g.off[MSFDM_FXVX] + ai] = -pxc * rinv * qvx - pxa * qVx;
-                a.TB[g.off[MSFDM_FXVZ] + ai] = -pxc * rinv * qvz - pxa * qVz;
+                tb_out(a, g.off[MSFDM_FXVX] + ai, -pxc * rinv * qvx - pxa * qVx);
+                tb_out(a, g.off[MSFDM_FXVZ] + ai, -pxc * rinv * qvz - pxa * qVz);
                 Vx = S[g.off[MSFDM_FXVX] + ai]; Vz = S[g.off[MSFDM_FXVZ] + ai];
                 Ex = S[g.off[MSFDM_FXEX] + ai]; Ez = S[g.off[MSFDM_FXEZ] + ai];
             }
@@ -272,8 +296,8 @@ __device__ __forceinline__ void fluid_prod_body(const float (*sh)[TZ + 2 * HALO]
                     const float qEx = kbar(a, g.off[MSFDM_FXEX] + ai), qEz = kbar(a, g.off[MSFDM_FXEZ] + ai);
                     const float Dvxb = -Km * (pxc - 1.0f) * xx * qp + pxb * qEx;
                     const float Dvzb = -Km * (pxc - 1.0f) * xz * qp + pxb * qEz;
-                    a.TB[g.off[MSFDM_FXEX] + ai] = Km * pxc * xx * qp - pxa * qEx;
-                    a.TB[g.off[MSFDM_FXEZ] + ai] = Km * pxc * xz * qp - pxa * qEz;
+                    tb_out(a, g.off[MSFDM_FXEX] + ai, Km * pxc * xx * qp - pxa * qEx);
+                    tb_out(a, g.off[MSFDM_FXEZ] + ai, Km * pxc * xz * qp - pxa * qEz);
                     bxx += Dvxb; bxz += gg * Dvxb; bzx += Dvzb; bzz += gg * Dvzb;
                     const float Dvx = dvxx + gg * dvxz, Dvz = dvzx + gg * dvzz;
                     bracket += -(pxc - 1.0f) * xx * Dvx - (pxc - 1.0f) * xz * Dvz + pxc * xx * Ex + pxc * xz * Ez;
@@ -314,6 +338,20 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
     float *Q = a.prod + prod_solid(g, 0);
     const long long pP = (long long)nzs * P;
     constexpr int o = lo_off<DIR>();
+    // the FP64 gradient sums of the wave's cells, loaded before any store of
+    // the tile (no load waits behind an earlier row's stores)
+    double gc[RW][5];
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int iz = z0 + ty * RW + j;
+        const long long si = (long long)iz * P + ix;
+        const bool in = act && (INT || iz <= nzs - 4);
+        gc[j][0] = in ? gr.C11[si] : 0.0;
+        gc[j][1] = in ? gr.C13[si] : 0.0;
+        gc[j][2] = in ? gr.C33[si] : 0.0;
+        gc[j][3] = in ? gr.C55[si] : 0.0;
+        gc[j][4] = in ? gr.rho_s[si] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int iz = z0 + ty * RW + j;
@@ -322,7 +360,7 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
         const int r = iz - z0 + HALO;
         float q[ADJ_NQS] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (act) {
-            const double g11 = gr.C11[si], g13 = gr.C13[si], g33 = gr.C33[si], g55 = gr.C55[si], grs = gr.rho_s[si];
+            const double g11 = gc[j][0], g13 = gc[j][1], g33 = gc[j][2], g55 = gc[j][3], grs = gc[j][4];
             const float J = ldc(&c.J[gi]), zx = ldc(&c.zex[gi]), zz = ldc(&c.zez[gi]);
             float xx, xz;
             xi_metrics<MET>(c, gi, xx, xz);
@@ -355,8 +393,8 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
                     const float qEx = kbar(a, g.off[MSFDM_SXEX] + ai), qEz = kbar(a, g.off[MSFDM_SXEZ] + ai);
                     const float Dvxb = q1 * xx * exxb + c55 * q1 * xz * qsxz + pxb * qEx;
                     const float Dvzb = q1 * xz * ezzb + c55 * q1 * xx * qsxz + pxb * qEz;
-                    a.TB[g.off[MSFDM_SXEX] + ai] = -pxc * xx * exxb - pxc * c55 * xz * qsxz - pxa * qEx;
-                    a.TB[g.off[MSFDM_SXEZ] + ai] = -pxc * xz * ezzb - pxc * c55 * xx * qsxz - pxa * qEz;
+                    tb_out(a, g.off[MSFDM_SXEX] + ai, -pxc * xx * exxb - pxc * c55 * xz * qsxz - pxa * qEx);
+                    tb_out(a, g.off[MSFDM_SXEZ] + ai, -pxc * xz * ezzb - pxc * c55 * xx * qsxz - pxa * qEz);
                     bxx += Dvxb; bxz += gx * Dvxb; bzx += Dvzb; bzz += gx * Dvzb;
                     const float Dvx = dvxx + gx * dvxz, Dvz = dvzx + gx * dvzz;
                     A11 += q1 * xx * Dvx - pxc * xx * Ex;
@@ -371,8 +409,8 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
                     const float qEx = kbar(a, g.off[MSFDM_SZEX] + ai), qEz = kbar(a, g.off[MSFDM_SZEZ] + ai);
                     const float Dvxb = q1 * zx * exxb + c55 * q1 * zz * qsxz + pzb * qEx;
                     const float Dvzb = q1 * zz * ezzb + c55 * q1 * zx * qsxz + pzb * qEz;
-                    a.TB[g.off[MSFDM_SZEX] + ai] = -pzc * zx * exxb - pzc * c55 * zz * qsxz - pza * qEx;
-                    a.TB[g.off[MSFDM_SZEZ] + ai] = -pzc * zz * ezzb - pzc * c55 * zx * qsxz - pza * qEz;
+                    tb_out(a, g.off[MSFDM_SZEX] + ai, -pzc * zx * exxb - pzc * c55 * zz * qsxz - pza * qEx);
+                    tb_out(a, g.off[MSFDM_SZEZ] + ai, -pzc * zz * ezzb - pzc * c55 * zx * qsxz - pza * qEz);
                     bxz += Dvxb; bxx += gz * Dvxb; bzz += Dvzb; bzx += gz * Dvzb;
                     const float Dvx = dvxz + gz * dvxx, Dvz = dvzz + gz * dvzx;
                     A11 += q1 * zx * Dvx - pzc * zx * Ex;
@@ -398,8 +436,8 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
                     const float qVx = kbar(a, g.off[MSFDM_SXVX] + ai), qVz = kbar(a, g.off[MSFDM_SXVZ] + ai);
                     Dxxb = q1x * rinv * qvx + pxb * qVx;
                     Dxzb = q1x * rinv * qvz + pxb * qVz;
-                    a.TB[g.off[MSFDM_SXVX] + ai] = -pxc * rinv * qvx - pxa * qVx;
-                    a.TB[g.off[MSFDM_SXVZ] + ai] = -pxc * rinv * qvz - pxa * qVz;
+                    tb_out(a, g.off[MSFDM_SXVX] + ai, -pxc * rinv * qvx - pxa * qVx);
+                    tb_out(a, g.off[MSFDM_SXVZ] + ai, -pxc * rinv * qvz - pxa * qVz);
                     Vxs = S[g.off[MSFDM_SXVX] + ai]; Vzs = S[g.off[MSFDM_SXVZ] + ai];
                 }
                 if (bottom) {
@@ -409,8 +447,8 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
                     const float qVx = kbar(a, g.off[MSFDM_SZVX] + ai), qVz = kbar(a, g.off[MSFDM_SZVZ] + ai);
                     Dzxb = q1z * rinv * qvx + pzb * qVx;
                     Dzzb = q1z * rinv * qvz + pzb * qVz;
-                    a.TB[g.off[MSFDM_SZVX] + ai] = -pzc * rinv * qvx - pza * qVx;
-                    a.TB[g.off[MSFDM_SZVZ] + ai] = -pzc * rinv * qvz - pza * qVz;
+                    tb_out(a, g.off[MSFDM_SZVX] + ai, -pzc * rinv * qvx - pza * qVx);
+                    tb_out(a, g.off[MSFDM_SZVZ] + ai, -pzc * rinv * qvz - pza * qVz);
                     Vxb = S[g.off[MSFDM_SZVX] + ai]; Vzb = S[g.off[MSFDM_SZVZ] + ai];
                 }
                 float rvx, rvz, Dxx = 0.f, Dxz = 0.f, Dzx = 0.f, Dzz = 0.f;
@@ -485,28 +523,34 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
     }
 }
 
-template <int DIR, int TZ, int MET>
+// Products over the source tiles of a list: tiles[t] = (column tile, band),
+// bands 0..nbz_f-1 fluid, then solid.  Three lists and three kernels, so the
+// registers and code of one kind of tile do not weigh on another:
+// ADJ_INTERIOR, the interior tiles of both media (adj_fluid_interior /
+// adj_solid_interior; bodies without a boundary branch); ADJ_FLUID and
+// ADJ_SOLID, the other tiles of each medium.
+constexpr int ADJ_INTERIOR = 0, ADJ_FLUID = 1, ADJ_SOLID = 2;
+template <int DIR, int TZ, int MET, int KIND>
 __global__ void __launch_bounds__(NTHR)
-adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, int nbx, int nbz_f, int nbz_s) {
+adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, const int2 *__restrict__ tiles, int ntiles, int nbz_f) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];   // forward stage state of the tile + halo
+    constexpr int INT = KIND == ADJ_INTERIOR;
     const AdjIn a = adj_of_shot(a0, blockIdx.y);
     const Grad gr = grad_of_shot(gr0, blockIdx.y);
     const int per_xcd = gridDim.x >> 3;
     const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (t >= nbx * (nbz_f + nbz_s)) return;
-    const int band = t / nbx, bx = t - band * nbx;
-    if (band < nbz_f) {
-        const int z0 = 3 + band * TZ;
-        load_tile<3, TZ>(sh, a.S, g.off, MSFDM_FVX, z0, bx * TX, g.nzf, g.P);
+    if (t >= ntiles) return;
+    const int2 tl = tiles[t];
+    const int bx = tl.x, band = tl.y;
+    if (KIND == ADJ_FLUID || (KIND == ADJ_INTERIOR && band < nbz_f)) {
+        load_tile<3, TZ, (bool)INT>(sh, a.S, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);
         __syncthreads();
-        if (adj_fluid_interior(g, bx * TX, z0, TZ)) fluid_prod_body<DIR, TZ, MET, 1>(sh, g, c, a, gr, bx, band);
-        else fluid_prod_body<DIR, TZ, MET, 0>(sh, g, c, a, gr, bx, band);
+        fluid_prod_body<DIR, TZ, MET, INT>(sh, g, c, a, gr, bx, band);
     } else {
         const int sb = band - nbz_f;
-        load_tile<5, TZ>(sh, a.S, g.off, MSFDM_SVX, sb * TZ, bx * TX, g.nzs, g.P);
+        load_tile<5, TZ, (bool)INT>(sh, a.S, g.off, MSFDM_SVX, sb * TZ, bx * TX, g.nzs, g.P);
         __syncthreads();
-        if (adj_solid_interior(g, bx * TX, sb * TZ, TZ)) solid_prod_body<DIR, TZ, MET, 1>(sh, g, c, a, gr, bx, sb);
-        else solid_prod_body<DIR, TZ, MET, 0>(sh, g, c, a, gr, bx, sb);
+        solid_prod_body<DIR, TZ, MET, INT>(sh, g, c, a, gr, bx, sb);
     }
 }
 
@@ -572,6 +616,15 @@ __device__ __forceinline__ void fluid_gather_body(const float (*sh)[TZ + 2 * HAL
     const int P = g.P, nzf = g.nzf;
     const int sc = tx + XOFF, ix = x0 + tx;
     auto flux_row = [&](int s) { return s >= 3 && s <= nzf - 1 && (nzf - 1 - s < 3 || s - 3 < 3); };
+    // the accumulator entries of the wave's cells, loaded before any store
+    float acc[RW][3];
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int t = z0 + ty * RW + j;
+        const long long gi = (long long)t * P + ix;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) acc[j][f] = (a.ACC && t <= nzf - 1) ? a.ACC[g.off[MSFDM_FVX + f] + gi] : 0.0f;
+    }
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int t = z0 + ty * RW + j;
@@ -617,6 +670,11 @@ __device__ __forceinline__ void fluid_gather_body(const float (*sh)[TZ + 2 * HAL
         a.TB[g.off[MSFDM_FVX] + gi] = bvx;
         a.TB[g.off[MSFDM_FVZ] + gi] = bvz;
         a.TB[g.off[MSFDM_FP] + gi] = bp;
+        if (a.ACC) {
+            a.ACC[g.off[MSFDM_FVX] + gi] = acc[j][0] + bvx;
+            a.ACC[g.off[MSFDM_FVZ] + gi] = acc[j][1] + bvz;
+            a.ACC[g.off[MSFDM_FP] + gi] = acc[j][2] + bp;
+        }
     }
 }
 
@@ -630,6 +688,15 @@ __device__ __forceinline__ void solid_gather_body(const float (*sh)[TZ + 2 * HAL
     const int P = g.P, nzs = g.nzs;
     const int sc = tx + XOFF, ix = x0 + tx;
     const long long moff = (long long)(g.nzf - 1) * P;
+    // the accumulator entries of the wave's cells, loaded before any store
+    float acc[RW][5];
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int t = z0 + ty * RW + j;
+        const long long si = (long long)t * P + ix;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) acc[j][f] = (a.ACC && t <= nzs - 1) ? a.ACC[g.off[MSFDM_SVX + f] + si] : 0.0f;
+    }
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int t = z0 + ty * RW + j;
@@ -682,11 +749,13 @@ __device__ __forceinline__ void solid_gather_body(const float (*sh)[TZ + 2 * HAL
             bsz += Jt * zzt * fz;
             bsxz += Jt * (zzt * fx + zxt * fz);
         }
-        a.TB[g.off[MSFDM_SVX] + si] = bvx;
-        a.TB[g.off[MSFDM_SVZ] + si] = bvz;
-        a.TB[g.off[MSFDM_SSX] + si] = bsx;
-        a.TB[g.off[MSFDM_SSZ] + si] = bsz;
-        a.TB[g.off[MSFDM_SSXZ] + si] = bsxz;
+        const float out[5] = {bvx, bvz, bsx, bsz, bsxz};
+#pragma unroll
+        for (int f = 0; f < 5; ++f) a.TB[g.off[MSFDM_SVX + f] + si] = out[f];
+        if (a.ACC) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) a.ACC[g.off[MSFDM_SVX + f] + si] = acc[j][f] + out[f];
+        }
     }
 }
 
@@ -842,12 +911,6 @@ __global__ void grad_sum_kernel(double *__restrict__ total, const double *__rest
         for (int k = 0; k < nshot; ++k) t += g[(long long)k * gstride + i];
         total[i] = t;
     }
-}
-
-// ACC += TB over n floats.
-__global__ void axpy_kernel(float *__restrict__ acc, const float *__restrict__ tb, long long n) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        acc[i] += tb[i];
 }
 
 // Record residual injection into lambda at record slot r (shot blockIdx.y:

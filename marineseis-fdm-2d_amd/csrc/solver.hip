@@ -70,6 +70,8 @@ struct msfdm_solver {
     float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float *qif = nullptr;
     float *aprod = nullptr;          // adjoint products of one stage (adj_prod_kernel -> adj_gather_kernel)
+    int2 *atiles = nullptr;          // adjoint source tiles: interior, fluid boundary, solid boundary
+    int atiles_n[3] = {0, 0, 0};
     double *gmem = nullptr;          // 7 gradient arrays, [rows][P] doubles, three times:
                                      // segment partial sums, Kahan totals, compensations
     long long gfl = 0;               // doubles of one set of 7 arrays
@@ -379,6 +381,7 @@ void free_adjoint(msfdm_solver *s) {
     for (auto &p : s->adj) if (p) { (void)hipFree(p); p = nullptr; }
     if (s->qif) { (void)hipFree(s->qif); s->qif = nullptr; }
     if (s->aprod) { (void)hipFree(s->aprod); s->aprod = nullptr; }
+    if (s->atiles) { (void)hipFree(s->atiles); s->atiles = nullptr; }
     if (s->gmem) { (void)hipFree(s->gmem); s->gmem = nullptr; }
     for (float *p : s->ckpt) (void)hipFree(p);
     for (float *p : s->seg) (void)hipFree(p);
@@ -1286,10 +1289,41 @@ namespace {
 
 enum AdjRole { A_LAM = 0, A_LP = 1, A_ACC = 2, A_TBA = 3, A_TBB = 4 };
 
+// Tile height of the adjoint stage kernels (4-row product tiles measured
+// slower: 186 against 168 us on 32 shots of 401 x 201).
+constexpr int ADJ_TZ = 8;
+
 // Floats of one shot's adjoint product buffer (fluid products, then solid).
 long long adj_prod_floats(const msfdm_solver *s) {
     const long long P = s->geo.P;
     return align64((long long)D::ADJ_NQF * s->d.nz_fluid * P + (long long)D::ADJ_NQS * s->d.nz_solid * P);
+}
+
+// The adjoint source tiles of the grid (column tile, band; fluid bands first)
+// in three lists, one per products kernel: the interior tiles of both media
+// (D::ADJ_INTERIOR), then the other fluid tiles (D::ADJ_FLUID) and the other
+// solid tiles (D::ADJ_SOLID).
+int ensure_adj_tiles(msfdm_solver *s) {
+    if (s->atiles) return MSFDM_OK;
+    const int nx = s->d.nx, npd = s->geo.npd, nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
+    const int nbx = (nx - 4) / D::TX + 1;
+    const int nbf = (nzf - 3 + ADJ_TZ - 1) / ADJ_TZ, nbs = (nzs - 3 + ADJ_TZ - 1) / ADJ_TZ;
+    std::vector<int2> list[3];
+    for (int band = 0; band < nbf + nbs; ++band)
+        for (int bx = 0; bx < nbx; ++bx) {
+            const bool fluid = band < nbf;
+            const bool inner = fluid ? D::adj_fluid_interior(nx, npd, nzf, bx * D::TX, 3 + band * ADJ_TZ, ADJ_TZ)
+                                     : D::adj_solid_interior(nx, npd, nzs, bx * D::TX, (band - nbf) * ADJ_TZ, ADJ_TZ);
+            list[inner ? D::ADJ_INTERIOR : fluid ? D::ADJ_FLUID : D::ADJ_SOLID].push_back(make_int2(bx, band));
+        }
+    std::vector<int2> all;
+    for (int k = 0; k < 3; ++k) {
+        s->atiles_n[k] = (int)list[k].size();
+        all.insert(all.end(), list[k].begin(), list[k].end());
+    }
+    HIPCHK(hipMalloc(&s->atiles, all.size() * sizeof(int2)));
+    HIPCHK(hipMemcpy(s->atiles, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+    return MSFDM_OK;
 }
 
 int ensure_adjoint(msfdm_solver *s) {
@@ -1304,6 +1338,7 @@ int ensure_adjoint(msfdm_solver *s) {
         HIPCHK(hipMalloc(&s->aprod, n * sizeof(float)));
         HIPCHK(hipMemset(s->aprod, 0, n * sizeof(float)));
     }
+    if (int rc = ensure_adj_tiles(s)) return rc;
     if (!s->gmem) {
         const long long nF = (long long)s->d.nz_fluid * P, nS = (long long)s->d.nz_solid * P;
         const long long total = 2 * nF + 5 * nS;
@@ -1323,30 +1358,31 @@ int ensure_adjoint(msfdm_solver *s) {
     return MSFDM_OK;
 }
 
-// The adjoint stage kernels run 8-row tiles.
-constexpr int ADJ_TZ = 8;
-// One adjoint stage's two launches: the products of every source tile, then
-// the gather into every output tile (Tbar of the state fields; the products
-// kernel writes the PML memory entries).
-template <int DIR>
-void launch_adj(hipStream_t st, const D::Geo &g, const D::Coef &c, const D::AdjIn &a, const D::Grad &gr, int nx,
-                int nzf, int nzs, int nshot) {
+// One adjoint stage's launches: the products of the interior source tiles,
+// of the other fluid and the other solid source tiles, then the gather into
+// every output tile (Tbar of the state fields; the products kernels write the
+// PML memory entries).
+template <int DIR, int MET>
+void launch_adj_met(msfdm_solver *s, hipStream_t st, const D::AdjIn &a, const D::Grad &gr, int nshot) {
     constexpr int TZ = ADJ_TZ;
-    const int nbx = (nx - 4) / D::TX + 1;   // source tiles cover the updated columns 3..nx-4
-    const int nbf = (nzf - 3 + TZ - 1) / TZ, nbs = (nzs - 3 + TZ - 1) / TZ;
-    const int np = nbx * (nbf + nbs);
-    const dim3 gp(8 * ((np + 7) / 8), nshot);
+    const D::Geo &g = s->geo;
+    const D::Coef &c = s->coef;
+    const int nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
+    const int nbf = (nzf - 3 + TZ - 1) / TZ;
+    const int2 *tl = s->atiles;
+    const int *n = s->atiles_n;
+    auto grid = [&](int nt) { return dim3(8 * ((nt + 7) / 8), nshot); };
+    if (n[0]) D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_INTERIOR><<<grid(n[0]), D::NTHR, 0, st>>>(g, c, a, gr, tl, n[0], nbf);
+    if (n[1]) D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_FLUID><<<grid(n[1]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0], n[1], nbf);
+    if (n[2]) D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_SOLID><<<grid(n[2]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0] + n[1], n[2], nbf);
     const int gbx = g.P / D::TX;            // output tiles cover every column of the pitch
     const int gbf = (nzf + TZ - 1) / TZ, gbs = (nzs + TZ - 1) / TZ;
-    const int ng = gbx * (gbf + gbs);
-    const dim3 gg(8 * ((ng + 7) / 8), nshot);
-    if (c.vert) {   // vertical grid columns: xi_x, xi_z are constants of the kernel
-        D::adj_prod_kernel<DIR, TZ, 1><<<gp, D::NTHR, 0, st>>>(g, c, a, gr, nbx, nbf, nbs);
-        D::adj_gather_kernel<DIR, TZ, 1><<<gg, D::NTHR, 0, st>>>(g, c, a, gbx, gbf, gbs);
-    } else {
-        D::adj_prod_kernel<DIR, TZ, 0><<<gp, D::NTHR, 0, st>>>(g, c, a, gr, nbx, nbf, nbs);
-        D::adj_gather_kernel<DIR, TZ, 0><<<gg, D::NTHR, 0, st>>>(g, c, a, gbx, gbf, gbs);
-    }
+    D::adj_gather_kernel<DIR, TZ, MET><<<grid(gbx * (gbf + gbs)), D::NTHR, 0, st>>>(g, c, a, gbx, gbf, gbs);
+}
+template <int DIR>
+void launch_adj(msfdm_solver *s, hipStream_t st, const D::AdjIn &a, const D::Grad &gr, int nshot) {
+    if (s->coef.vert) launch_adj_met<DIR, 1>(s, st, a, gr, nshot);   // vertical grid columns: xi constants
+    else launch_adj_met<DIR, 0>(s, st, a, gr, nshot);
 }
 // total += segment partial sums (Kahan), partial sums = 0; n doubles from p.
 void kahan_fold(hipStream_t st, double *part, long long n) {
@@ -1355,14 +1391,17 @@ void kahan_fold(hipStream_t st, double *part, long long n) {
 
 // One adjoint stage: TB = L_dir^T B_dir^T (clam LP + cprev TBp) at forward state S.
 // Every entry of TB a stage can reach is written by it (the others stay zero).
+// ACC (may be null): the step's accumulator, ACC += TB.
 template <int DIR>
-int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, float clam, float cprev) {
+int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, float clam, float cprev,
+              float *ACC = nullptr) {
     D::AdjIn a;
-    a.S = S; a.LP = s->adj[A_LP]; a.TBp = TBp; a.qif = s->qif; a.TB = TB; a.prod = s->aprod;
-    a.clam = clam; a.cprev = cprev;
+    a.S = S; a.LP = s->adj[A_LP]; a.qif = s->qif; a.TB = TB; a.ACC = ACC; a.prod = s->aprod;
+    a.TBp = TBp ? TBp : a.LP;   // no previous Tbar: LP with weight 0
+    a.clam = clam; a.cprev = TBp ? cprev : 0.0f;
     a.bstride = 0; a.qstride = 0; a.pstride = 0;
     D::adj_iface_q<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, a, s->qif);
-    launch_adj<DIR>(s->stream, s->geo, s->coef, a, s->grad, s->d.nx, s->d.nz_fluid, s->d.nz_solid, 1);
+    launch_adj<DIR>(s, s->stream, a, s->grad, 1);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
 }
@@ -1377,19 +1416,13 @@ int adj_step(msfdm_solver *s, int it, const float *Un, const float *T2, const fl
     HIPCHK(hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream));
     const float dt = s->d.dt;
     const bool odd = it & 1;                 // forward stage directions: d1 d2 d1 d2
-    const long long n = s->buf_floats;
-    const int ag = (int)std::min<long long>((n + 255) / 256, 8192);
-    auto acc = [&](const float *tb) { D::axpy_kernel<<<ag, 256, 0, s->stream>>>(ACC, tb, n); };
     int rc;
-    // stage 4 (d2, state T4), stage 3 (d1, T3), stage 2 (d2, T2), stage 1 (d1, U^n)
-    if ((rc = odd ? adj_stage<0>(s, T4, nullptr, TA, dt / 6, 0.f) : adj_stage<1>(s, T4, nullptr, TA, dt / 6, 0.f))) return rc;
-    acc(TA);
-    if ((rc = odd ? adj_stage<1>(s, T3, TA, TBb, dt / 3, dt) : adj_stage<0>(s, T3, TA, TBb, dt / 3, dt))) return rc;
-    acc(TBb);
-    if ((rc = odd ? adj_stage<0>(s, T2, TBb, TA, dt / 3, dt / 2) : adj_stage<1>(s, T2, TBb, TA, dt / 3, dt / 2))) return rc;
-    acc(TA);
-    if ((rc = odd ? adj_stage<1>(s, Un, TA, TBb, dt / 6, dt / 2) : adj_stage<0>(s, Un, TA, TBb, dt / 6, dt / 2))) return rc;
-    acc(TBb);
+    // stage 4 (d2, state T4), stage 3 (d1, T3), stage 2 (d2, T2), stage 1 (d1, U^n);
+    // each adds its Tbar to ACC
+    if ((rc = odd ? adj_stage<0>(s, T4, nullptr, TA, dt / 6, 0.f, ACC) : adj_stage<1>(s, T4, nullptr, TA, dt / 6, 0.f, ACC))) return rc;
+    if ((rc = odd ? adj_stage<1>(s, T3, TA, TBb, dt / 3, dt, ACC) : adj_stage<0>(s, T3, TA, TBb, dt / 3, dt, ACC))) return rc;
+    if ((rc = odd ? adj_stage<0>(s, T2, TBb, TA, dt / 3, dt / 2, ACC) : adj_stage<1>(s, T2, TBb, TA, dt / 3, dt / 2, ACC))) return rc;
+    if ((rc = odd ? adj_stage<1>(s, Un, TA, TBb, dt / 6, dt / 2, ACC) : adj_stage<0>(s, Un, TA, TBb, dt / 6, dt / 2, ACC))) return rc;
     std::swap(s->adj[A_LAM], s->adj[A_ACC]);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
@@ -1717,13 +1750,14 @@ template <int DIR>
 void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, float *TB, float clam, float cprev) {
     msfdm_solver *s = w.s;
     D::AdjIn a;
-    a.S = S; a.LP = w.adj[W_LP]; a.TBp = TBp; a.qif = w.qif; a.TB = TB; a.prod = w.prod;
-    a.clam = clam; a.cprev = cprev;
+    a.S = S; a.LP = w.adj[W_LP]; a.qif = w.qif; a.TB = TB; a.ACC = w.adj[W_ACC]; a.prod = w.prod;
+    a.TBp = TBp ? TBp : a.LP;   // no previous Tbar: LP with weight 0
+    a.clam = clam; a.cprev = TBp ? cprev : 0.0f;
     a.bstride = w.p.bf;
     a.qstride = 8 * w.p.P;
     a.pstride = w.p.pfl;
     D::adj_iface_q<DIR><<<dim3((s->d.nx + 127) / 128, kk), 128, 0, s->stream>>>(s->geo, s->coef, a, w.qif);
-    launch_adj<DIR>(s->stream, s->geo, s->coef, a, wave_grad(w), s->d.nx, s->d.nz_fluid, s->d.nz_solid, kk);
+    launch_adj<DIR>(s, s->stream, a, wave_grad(w), kk);
 }
 
 // Reverse of step `it` for the kk shots of the wave (segment slot j holds the
@@ -1738,18 +1772,13 @@ void wave_adj_step(ShotsWave &w, int kk, int it, int j) {
     (void)hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream);
     const float dt = s->d.dt;
     const bool odd = it & 1;
-    const int ag = (int)std::min<long long>((n + 255) / 256, 8192);
-    auto acc = [&](const float *tb) { D::axpy_kernel<<<ag, 256, 0, s->stream>>>(ACC, tb, n); };
     const float *Un = w.slab(w.seg, 4LL * j), *T2 = w.slab(w.seg, 4LL * j + 1), *T3 = w.slab(w.seg, 4LL * j + 2),
                 *T4 = w.slab(w.seg, 4LL * j + 3);
+    // each stage adds its Tbar to ACC (wave_adj_stage)
     if (odd) wave_adj_stage<0>(w, kk, T4, nullptr, TA, dt / 6, 0.f); else wave_adj_stage<1>(w, kk, T4, nullptr, TA, dt / 6, 0.f);
-    acc(TA);
     if (odd) wave_adj_stage<1>(w, kk, T3, TA, TBb, dt / 3, dt); else wave_adj_stage<0>(w, kk, T3, TA, TBb, dt / 3, dt);
-    acc(TBb);
     if (odd) wave_adj_stage<0>(w, kk, T2, TBb, TA, dt / 3, dt / 2); else wave_adj_stage<1>(w, kk, T2, TBb, TA, dt / 3, dt / 2);
-    acc(TA);
     if (odd) wave_adj_stage<1>(w, kk, Un, TA, TBb, dt / 6, dt / 2); else wave_adj_stage<0>(w, kk, Un, TA, TBb, dt / 6, dt / 2);
-    acc(TBb);
     std::swap(w.adj[W_LAM], w.adj[W_ACC]);
 }
 
@@ -1831,6 +1860,7 @@ extern "C" int msfdm_gradient_shots(msfdm_solver *s, int nshot, const int *isrc,
         if (isrc[i] < 0 || isrc[i] >= nsrc) return set_error(MSFDM_EINVAL, "shot index out of range");
     HIPCHK(hipSetDevice(s->device));
     if (int rc = grow_tables(s, nsteps)) return rc;
+    if (int rc = ensure_adj_tiles(s)) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     ShotsWave w;
     w.s = s;
