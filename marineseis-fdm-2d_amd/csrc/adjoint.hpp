@@ -82,7 +82,7 @@ struct AdjIn {
     const float *LP;     // lambda'
     const float *TBp;    // previous stage's Tbar (LP with cprev = 0 when there is none)
     const float *qif;    // [8][P] interface qbar = B^T kbar
-    float *TB;           // output Tbar
+    float *TB;           // output Tbar (null: the last stage of a step, whose Tbar only goes to ACC)
     float *ACC;          // null, or the step's accumulator: ACC += TB wherever the stage writes TB
     float *prod;         // adjoint products: [ADJ_NQF][nzf][P], then [ADJ_NQS][nzs][P]
     float clam, cprev;
@@ -110,7 +110,8 @@ __device__ __forceinline__ long long prod_solid(const Geo &g, int k) {
 // buffers sit at a fixed stride from shot 0's.
 __device__ __forceinline__ AdjIn adj_of_shot(AdjIn a, int shot) {
     const long long o = (long long)shot * a.bstride;
-    a.S += o; a.LP += o; a.TBp += o; a.TB += o;
+    a.S += o; a.LP += o; a.TBp += o;
+    if (a.TB) a.TB += o;
     if (a.ACC) a.ACC += o;
     a.qif += (long long)shot * a.qstride;
     a.prod += (long long)shot * a.pstride;
@@ -127,9 +128,10 @@ __device__ __forceinline__ T ldc(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
 }
 
-// Tbar[i] = v, and ACC[i] += v when the stage accumulates into ACC.
+// Tbar[i] = v (unless the stage keeps no Tbar), and ACC[i] += v when the
+// stage accumulates into ACC.
 __device__ __forceinline__ void tb_out(const AdjIn &a, long long i, float v) {
-    a.TB[i] = v;
+    if (a.TB) a.TB[i] = v;
     if (a.ACC) a.ACC[i] += v;
 }
 
@@ -667,9 +669,11 @@ __device__ __forceinline__ void fluid_gather_body(const float (*sh)[TZ + 2 * HAL
             }
             bp += -ldc(&c.J[gi]) * (ldc(&c.zex[gi]) * gx + ldc(&c.zez[gi]) * gz);
         }
-        a.TB[g.off[MSFDM_FVX] + gi] = bvx;
-        a.TB[g.off[MSFDM_FVZ] + gi] = bvz;
-        a.TB[g.off[MSFDM_FP] + gi] = bp;
+        if (a.TB) {
+            a.TB[g.off[MSFDM_FVX] + gi] = bvx;
+            a.TB[g.off[MSFDM_FVZ] + gi] = bvz;
+            a.TB[g.off[MSFDM_FP] + gi] = bp;
+        }
         if (a.ACC) {
             a.ACC[g.off[MSFDM_FVX] + gi] = acc[j][0] + bvx;
             a.ACC[g.off[MSFDM_FVZ] + gi] = acc[j][1] + bvz;
@@ -751,7 +755,8 @@ __device__ __forceinline__ void solid_gather_body(const float (*sh)[TZ + 2 * HAL
         }
         const float out[5] = {bvx, bvz, bsx, bsz, bsxz};
 #pragma unroll
-        for (int f = 0; f < 5; ++f) a.TB[g.off[MSFDM_SVX + f] + si] = out[f];
+        for (int f = 0; f < 5; ++f)
+            if (a.TB) a.TB[g.off[MSFDM_SVX + f] + si] = out[f];
         if (a.ACC) {
 #pragma unroll
             for (int f = 0; f < 5; ++f) a.ACC[g.off[MSFDM_SVX + f] + si] = acc[j][f] + out[f];

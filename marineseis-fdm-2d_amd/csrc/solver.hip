@@ -1422,7 +1422,8 @@ int adj_step(msfdm_solver *s, int it, const float *Un, const float *T2, const fl
     if ((rc = odd ? adj_stage<0>(s, T4, nullptr, TA, dt / 6, 0.f, ACC) : adj_stage<1>(s, T4, nullptr, TA, dt / 6, 0.f, ACC))) return rc;
     if ((rc = odd ? adj_stage<1>(s, T3, TA, TBb, dt / 3, dt, ACC) : adj_stage<0>(s, T3, TA, TBb, dt / 3, dt, ACC))) return rc;
     if ((rc = odd ? adj_stage<0>(s, T2, TBb, TA, dt / 3, dt / 2, ACC) : adj_stage<1>(s, T2, TBb, TA, dt / 3, dt / 2, ACC))) return rc;
-    if ((rc = odd ? adj_stage<1>(s, Un, TA, TBb, dt / 6, dt / 2, ACC) : adj_stage<0>(s, Un, TA, TBb, dt / 6, dt / 2, ACC))) return rc;
+    // (the last stage's Tbar only goes to ACC)
+    if ((rc = odd ? adj_stage<1>(s, Un, TA, nullptr, dt / 6, dt / 2, ACC) : adj_stage<0>(s, Un, TA, nullptr, dt / 6, dt / 2, ACC))) return rc;
     std::swap(s->adj[A_LAM], s->adj[A_ACC]);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
@@ -1778,7 +1779,8 @@ void wave_adj_step(ShotsWave &w, int kk, int it, int j) {
     if (odd) wave_adj_stage<0>(w, kk, T4, nullptr, TA, dt / 6, 0.f); else wave_adj_stage<1>(w, kk, T4, nullptr, TA, dt / 6, 0.f);
     if (odd) wave_adj_stage<1>(w, kk, T3, TA, TBb, dt / 3, dt); else wave_adj_stage<0>(w, kk, T3, TA, TBb, dt / 3, dt);
     if (odd) wave_adj_stage<0>(w, kk, T2, TBb, TA, dt / 3, dt / 2); else wave_adj_stage<1>(w, kk, T2, TBb, TA, dt / 3, dt / 2);
-    if (odd) wave_adj_stage<1>(w, kk, Un, TA, TBb, dt / 6, dt / 2); else wave_adj_stage<0>(w, kk, Un, TA, TBb, dt / 6, dt / 2);
+    // (the last stage's Tbar only goes to ACC)
+    if (odd) wave_adj_stage<1>(w, kk, Un, TA, nullptr, dt / 6, dt / 2); else wave_adj_stage<0>(w, kk, Un, TA, nullptr, dt / 6, dt / 2);
     std::swap(w.adj[W_LAM], w.adj[W_ACC]);
 }
 
