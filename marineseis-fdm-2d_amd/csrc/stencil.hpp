@@ -1169,13 +1169,17 @@ __device__ __forceinline__ bool solid_interior(const Geo &g, int x0, int z0, int
 
 // Resident waves per SIMD of the fluid-solid stage kernels: at most 4 for stage
 // 4, the stage that streams the most bytes (its registers would allow 5, which
-// measured slower), the compiler's choice otherwise.
+// measured slower); at least 5 for stage 1, which moves the fewest bytes per
+// cell and so gains from more waves in flight (22.7 against 23.3 us on the
+// 2001 x 1001 seafloor grid, with a 40-byte spill in its boundary path); the
+// compiler's choice (4) for stages 2 and 3, which lose 2.8 us at 5.
 template <int STAGE> __host__ __device__ constexpr int stage_max_waves() { return STAGE == 4 ? 4 : 8; }
+template <int STAGE> __host__ __device__ constexpr int stage_min_waves() { return STAGE == 1 ? 5 : 1; }
 
 // FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
 // and record (finish_interface_row); cprev is then the time step.
 template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(1, stage_max_waves<STAGE>())))
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(stage_min_waves<STAGE>(), stage_max_waves<STAGE>())))
 stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
              const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O,
              float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
@@ -1310,7 +1314,7 @@ struct Shot {
 // MET as in stage_kernel: 1 when every shot of the launch has vertical grid
 // columns (xi_x a constant, xi_z = 0), 2 to decide per shot at run time.
 template <int DIR, int STAGE, int TZ, int MET, int FIN = 0>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(1, stage_max_waves<STAGE>())))
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(stage_min_waves<STAGE>(), stage_max_waves<STAGE>())))
 batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
                    int jstep, float ct, float cprev) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");

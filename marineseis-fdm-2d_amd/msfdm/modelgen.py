@@ -406,7 +406,30 @@ def write_model(m: Model, model_dir: str, case: str, relative_to: str | None = N
     with open(jpath, "w") as f:
         json.dump(doc, f, indent=2)
         f.write("\n")
+    write_pad(m, model_dir)
     return jpath
+
+
+def write_pad(m: Model, model_dir: str) -> str:
+    """Write ``<model_dir>/pad.txt``, the padding description the reference's
+    model generators write beside the JSON and its comparison tools read
+    (tools/run_models/model_FANDS_PANDF_double_cos_padmove.py:302-314;
+    tools/compare_sem_scholte.py:59-64 reads pad_flat, tools/compare_double_cos.py:51
+    reads shift): one "key value" line each.  The outer pad is the 3 frozen
+    columns plus the npd-column PML on each side, so grid column c holds the
+    physical x = (c - pad_flat) h; the fluid array starts with 3 frozen rows and
+    the solid array ends with the bottom PML and 3 frozen rows."""
+    pad = m.npd + 3
+    fluid_full = m.nz_fluid if m.calc_type != SOLID else 0
+    lines = [("nx_phys", m.nx - 2 * pad), ("shift", pad), ("pad_flat", pad), ("outer", pad), ("npd", m.npd),
+             ("nz_fluid_phys", max(fluid_full - 3, 0)), ("nz_solid_phys", m.nz_solid - pad),
+             ("nz_fluid_full", fluid_full), ("nz_solid_full", m.nz_solid), ("src_z_offset", 3),
+             ("phys0", pad), ("phys1", m.nx - pad - 1)]
+    path = os.path.join(model_dir, "pad.txt")
+    with open(path, "w") as f:
+        for k, v in lines:
+            f.write(f"{k} {v}\n")
+    return path
 
 
 def read_model(json_path: str, cwd: str | None = None) -> Model:

@@ -36,6 +36,23 @@ def model_dir(tmp_path, monkeypatch):
     return m, jpath
 
 
+def test_pad_file_beside_the_model(model_dir):
+    """pad.txt as the reference's generators write it: "key value" lines, the
+    outer pad being the 3 frozen columns plus the PML, read the way
+    tools/compare_sem_scholte.py (pad_flat) and tools/compare_double_cos.py
+    (the token after "shift") read it."""
+    m, jpath = model_dir
+    text = open(os.path.join(os.path.dirname(jpath), "pad.txt")).read()
+    kv = dict(line.split() for line in text.splitlines())
+    pad = m.npd + 3
+    assert int(kv["pad_flat"]) == pad and int(kv["shift"]) == pad and int(kv["outer"]) == pad
+    assert int(text.split("shift")[1].split()[0]) == pad
+    assert int(kv["nx_phys"]) == m.nx - 2 * pad
+    assert int(kv["phys0"]) == pad and int(kv["phys1"]) == m.nx - pad - 1
+    assert int(kv["nz_fluid_full"]) == m.nz_fluid and int(kv["nz_solid_full"]) == m.nz_solid
+    assert int(kv["npd"]) == m.npd
+
+
 def _edit_json(jpath, fn):
     doc = json.load(open(jpath))
     fn(doc)
