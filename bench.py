@@ -228,13 +228,17 @@ def bench_shots(args, workload=None, steps=None, nseq=None):
     t0 = time.perf_counter()
     g, wave = s.gradient_shots(list(range(n)), steps, res)
     t_batch = time.perf_counter() - t0
+    print(f"[shots] {workload}: {n} shots x {steps} steps batched in {t_batch:.1f} s "
+          f"({wave} shots per wave)", file=sys.stderr, flush=True)
     k = min(n, nseq or args.shots_seq)
     idx = [int(i) for i in np.linspace(0, n - 1, k)]
+    s.gradient(idx[0], steps, res[idx[0]])                     # warm-up (checkpoint and segment buffers)
     t_seq, ref = 0.0, None
     for i in idx:
         t0 = time.perf_counter()
         gi = s.gradient(i, steps, res[i])
         t_seq += time.perf_counter() - t0
+        print(f"[shots] {workload}: shot {i} alone in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         ref = gi if ref is None else {a: ref[a] + gi[a] for a in ref}
     t_seq *= n / k
     gsub, _ = s.gradient_shots(idx, steps, [res[i] for i in idx])
@@ -460,7 +464,7 @@ def main():
     if not args.no_shots:
         shots = {
             "gradient_flat401_32": bench_shots(args, workload="flat401", steps=2000, nseq=4),
-            "gradient_twoscale4001_32": bench_shots(args, workload="twoscale4001", steps=40, nseq=2),
+            "gradient_twoscale4001_32": bench_shots(args, workload="twoscale4001", steps=200, nseq=2),
             "forward_batch64": bench_batch(args, nsteps=200),
         }
 

@@ -49,6 +49,8 @@ struct msfdm_solver {
     long long buf_floats = 0;
     float *buf[4] = {nullptr, nullptr, nullptr, nullptr};
     float *kbuf = nullptr;       // raw-k output of msfdm_eval_rhs
+    void *stage = nullptr;       // packed rows of a host transfer (host_rows)
+    size_t stage_bytes = 0;
     float *kif = nullptr;        // [KIF_ROWS][P]: raw interface-row k of the 4 stages + U^n interface rows
     float *coefmem = nullptr;
     float *amp = nullptr;
@@ -76,9 +78,10 @@ struct msfdm_solver {
                                      // segment partial sums, Kahan totals, compensations
     long long gfl = 0;               // doubles of one set of 7 arrays
     D::Grad grad{};
-    std::vector<float *> ckpt;       // checkpoints of the full state every adj_every steps
+    std::vector<float *> ckpt;       // checkpoints of the full state every adj_every steps (the first
+                                     // adj_nseg in use; kept allocated across gradients for reuse)
     std::vector<float *> seg;        // per-step U^n, T2, T3, T4 of one segment
-    int adj_every = 0, adj_nsteps = 0, adj_isrc = 0;
+    int adj_every = 0, adj_nsteps = 0, adj_isrc = 0, adj_nseg = 0;
     bool solid_only = false;         // grid.calc_type = 0: a solid grid
     int top = D::TOP_IFACE;          // top kind of the solid stage: interface, free surface or plain
     // In-sequence kernel timing (msfdm_profile_run): while `on`, every launch of
@@ -316,6 +319,34 @@ int field_rows(const msfdm_solver *s, int f) {
     return s->d.nz_solid;
 }
 
+// `rows` rows of row_bytes between packed host memory and a device array of
+// pitch dpitch.  A pitched copy straight into or out of pageable host memory
+// is split by the runtime into one transfer per row (some 6500 for the
+// gradients of a 4001 x 1501 grid, 0.4-0.9 s); packing the rows on the device
+// first leaves a single transfer.
+int host_rows(msfdm_solver *s, void *host, void *dev, size_t dpitch, size_t row_bytes, int rows, bool to_host) {
+    if (rows <= 0) return MSFDM_OK;
+    const size_t n = row_bytes * (size_t)rows;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->stage_bytes < n) {
+        if (s->stage) HIPCHK(hipFree(s->stage));
+        s->stage = nullptr;
+        s->stage_bytes = 0;
+        HIPCHK(hipMalloc(&s->stage, n));
+        s->stage_bytes = n;
+    }
+    if (to_host) {
+        HIPCHK(hipMemcpy2DAsync(s->stage, row_bytes, dev, dpitch, row_bytes, rows, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpy(host, s->stage, n, hipMemcpyDeviceToHost));
+    } else {
+        HIPCHK(hipMemcpy(s->stage, host, n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy2DAsync(dev, dpitch, s->stage, row_bytes, row_bytes, rows, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return MSFDM_OK;
+}
+
 // Copy between the reference layout [rows][nx] and the device layout of field f.
 int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float *host_in) {
     const int nx = s->d.nx, P = s->geo.P, npd = s->geo.npd, W2 = s->geo.W2;
@@ -327,16 +358,14 @@ int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float
         HIPCHK(hipStreamSynchronize(s->stream));
         if (host_out) std::fill(host_out, host_out + (size_t)rows * nx, 0.0f);
         if (!s->geo.fcpml || npd == 0) return MSFDM_OK;
-        if (host_out) HIPCHK(hipMemcpy2D(host_out + 3 * (size_t)nx, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
-        else HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in + 3 * (size_t)nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
-        return MSFDM_OK;
+        return host_rows(s, host_out ? host_out + 3 * (size_t)nx : (void *)(host_in + 3 * (size_t)nx), dp,
+                         P * sizeof(float), nx * sizeof(float), npd, host_out != nullptr);
     }
     const bool bottom = f >= MSFDM_SZVX;
     HIPCHK(hipStreamSynchronize(s->stream));
     if (!side && !bottom) {
-        if (host_out) HIPCHK(hipMemcpy2D(host_out, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), rows, hipMemcpyDeviceToHost));
-        else HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in, nx * sizeof(float), nx * sizeof(float), rows, hipMemcpyHostToDevice));
-        return MSFDM_OK;
+        return host_rows(s, host_out ? host_out : (void *)host_in, dp, P * sizeof(float), nx * sizeof(float), rows,
+                         host_out != nullptr);
     }
     if (side) {
         std::vector<float> c((size_t)rows * W2);
@@ -365,16 +394,13 @@ int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float
     if (npd == 0) return MSFDM_OK;
     float *bp = dp;
     if (s->geo.ztop_pml) {
-        if (host_out) HIPCHK(hipMemcpy2D(host_out + 3 * (size_t)nx, nx * sizeof(float), dp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
-        else HIPCHK(hipMemcpy2D(dp, P * sizeof(float), host_in + 3 * (size_t)nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
+        if (int rc = host_rows(s, host_out ? host_out + 3 * (size_t)nx : (void *)(host_in + 3 * (size_t)nx), dp,
+                               P * sizeof(float), nx * sizeof(float), npd, host_out != nullptr))
+            return rc;
         bp = dp + (size_t)npd * P;
     }
-    if (host_out) {
-        HIPCHK(hipMemcpy2D(host_out + (size_t)zb0 * nx, nx * sizeof(float), bp, P * sizeof(float), nx * sizeof(float), npd, hipMemcpyDeviceToHost));
-    } else {
-        HIPCHK(hipMemcpy2D(bp, P * sizeof(float), host_in + (size_t)zb0 * nx, nx * sizeof(float), nx * sizeof(float), npd, hipMemcpyHostToDevice));
-    }
-    return MSFDM_OK;
+    return host_rows(s, host_out ? host_out + (size_t)zb0 * nx : (void *)(host_in + (size_t)zb0 * nx), bp,
+                     P * sizeof(float), nx * sizeof(float), npd, host_out != nullptr);
 }
 
 void free_adjoint(msfdm_solver *s) {
@@ -395,7 +421,8 @@ void free_all(msfdm_solver *s) {
     s->prof.ev.clear();
     free_adjoint(s);
     for (auto &b : s->buf) if (b) { (void)hipFree(b); b = nullptr; }
-    for (void *p : {(void *)s->kbuf, (void *)s->kif, (void *)s->coefmem, (void *)s->amp, (void *)s->base, (void *)s->rec})
+    for (void *p : {(void *)s->kbuf, s->stage, (void *)s->kif, (void *)s->coefmem, (void *)s->amp, (void *)s->base,
+                    (void *)s->rec})
         if (p) (void)hipFree(p);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -1431,9 +1458,7 @@ int adj_step(msfdm_solver *s, int it, const float *Un, const float *T2, const fl
 
 int copy_grad_out(msfdm_solver *s, const double *dsrc, int rows, double *out) {
     if (!out) return MSFDM_OK;
-    HIPCHK(hipMemcpy2D(out, s->d.nx * sizeof(double), dsrc, s->geo.P * sizeof(double), s->d.nx * sizeof(double), rows,
-                       hipMemcpyDeviceToHost));
-    return MSFDM_OK;
+    return host_rows(s, out, (void *)dsrc, s->geo.P * sizeof(double), s->d.nx * sizeof(double), rows, true);
 }
 
 }  // namespace
@@ -1445,18 +1470,24 @@ extern "C" int msfdm_adjoint_forward(msfdm_solver *s, int isrc, int nsteps, int 
     if (every <= 0) every = std::max(2, (int)std::ceil(std::sqrt((double)nsteps)));
     if (every & 1) ++every;                  // keep segment starts even (F/B parity of the graph)
     if (int rc = ensure_adjoint(s)) return rc;
-    for (float *p : s->ckpt) (void)hipFree(p);
-    s->ckpt.clear();
+    s->adj_nseg = 0;   // no valid forward until the checkpoints are written
     if (int rc = msfdm_reset_shot(s, isrc)) return rc;
     const size_t bytes = s->buf_floats * sizeof(float);
-    for (int it0 = 0; it0 < nsteps; it0 += every) {
+    // checkpoint buffers stay allocated between gradients (an FWI loop calls
+    // this with the same nsteps every iteration); more are added when needed
+    const int nseg = (nsteps + every - 1) / every;
+    while ((int)s->ckpt.size() < nseg) {
         float *c = nullptr;
         if (hipMalloc(&c, bytes) != hipSuccess)
             return set_error(MSFDM_EDEVICE, "out of device memory for adjoint checkpoints");
         s->ckpt.push_back(c);
-        HIPCHK(hipMemcpyAsync(c, s->buf[R_U], bytes, hipMemcpyDeviceToDevice, s->stream));
+    }
+    for (int seg = 0; seg < nseg; ++seg) {
+        const int it0 = seg * every;
+        HIPCHK(hipMemcpyAsync(s->ckpt[seg], s->buf[R_U], bytes, hipMemcpyDeviceToDevice, s->stream));
         if (int rc = msfdm_run(s, std::min(every, nsteps - it0))) return rc;
     }
+    s->adj_nseg = nseg;
     s->adj_every = every;
     s->adj_nsteps = nsteps;
     s->adj_isrc = isrc;
@@ -1469,7 +1500,7 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
                                       double *gC55, double *grho_s) {
     clear_error();
     if (!s || !res_vx || !res_vz || !res_p) return set_error(MSFDM_EINVAL, "null argument");
-    if (s->ckpt.empty()) return set_error(MSFDM_ESTATE, "msfdm_adjoint_forward has not been called");
+    if (s->adj_nseg == 0) return set_error(MSFDM_ESTATE, "msfdm_adjoint_forward has not been called");
     HIPCHK(hipSetDevice(s->device));
     const int nsteps = s->adj_nsteps, every = s->adj_every, gap = s->gap;
     const int nrec = (nsteps - 1) / gap + 1;
@@ -1493,7 +1524,7 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
     HIPCHK(hipMemsetAsync(s->adj[A_LAM], 0, bytes, s->stream));
     HIPCHK(hipMemsetAsync(s->gmem, 0, 3 * s->gfl * sizeof(double), s->stream));
     int rc = MSFDM_OK;
-    for (int seg = (int)s->ckpt.size() - 1; seg >= 0 && rc == MSFDM_OK; --seg) {
+    for (int seg = s->adj_nseg - 1; seg >= 0 && rc == MSFDM_OK; --seg) {
         const int it0 = seg * every, it1 = std::min(nsteps, it0 + every);
         // recompute the segment, keeping every step's stage states
         if (hipMemcpyAsync(s->buf[R_U], s->ckpt[seg], bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {

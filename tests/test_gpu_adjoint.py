@@ -125,6 +125,16 @@ def test_gradient_matches_oracle(curvi):
     gs3 = s.gradient(0, nsteps, res, every=8)
     for n in msfdm._lib.GRADS:
         assert np.array_equal(gs3[n], gs[n]), n
+    # checkpoint buffers kept from the longer gradient serve a shorter one: same
+    # bits as a solver that never ran the longer one
+    short = 30
+    rs = [r[:(short - 1) // o.gap + 1] for r in res]
+    gshort = s.gradient(0, short, rs, every=8)
+    fresh = msfdm.Solver(m)
+    gfresh = fresh.gradient(0, short, rs, every=8)
+    fresh.close()
+    for n in msfdm._lib.GRADS:
+        assert np.array_equal(gshort[n], gfresh[n]), n
 
 
 def test_gradient_matches_gpu_central_differences():
