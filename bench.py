@@ -455,6 +455,12 @@ def main():
     t_rep4 = s.time_kernel(1, reps)
 
     kname, traffic = measured_traffic(args.workload, stage2_kernel_names(False, tile_rows))
+    # what a plain streaming kernel reaches on this part: 1 GB streams (HBM),
+    # and streams the size of one stage-2 launch's bytes (partly cache resident)
+    stream = {"copy_GBps": round(1e3 * msfdm.stream_probe(1000, 1), 1),
+              "read3_write1_GBps": round(1e3 * msfdm.stream_probe(1000, 3), 1),
+              "read3_write1_stage2_sized_GBps": round(1e3 * msfdm.stream_probe(b_stage / 4e6, 3, reps=200), 1),
+              "stream_MB": 1000, "stage2_sized_stream_MB": round(b_stage / 4e6, 1)}
 
     cpu = None
     if args.cpu_steps > 0:
@@ -507,7 +513,9 @@ def main():
             "alg_bytes_per_launch": b_stage,
             "timing": f"mean of the {p2['launches'][1]} stage-2 launches of a {args.steps}-step run "
                       f"(only those launches timed, by events their own dispatch stamps)",
+            "frac_of_streaming_kernel": round(achieved / stream["read3_write1_GBps"], 4),
         },
+        "hbm_stream_probe": stream,
         "cpu_baseline": cpu,
         "shots": shots,
     }

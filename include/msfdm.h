@@ -173,6 +173,13 @@ int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms);
  * stream events.  The state advances as msfdm_run's would.  No reference
  * counterpart (measurement). */
 int msfdm_profile_run(msfdm_solver *s, int nsteps, int class_mask, float *kernel_ms, int *launches, float *wall_ms);
+/* HBM streaming probe on `device`: `reads` input streams (1..4) and one
+ * output stream of `mbytes` MB each, out = sum of the inputs over float4
+ * elements, timed over `reps` launches after 3 warm-up launches; *tbps =
+ * (reads + 1) * bytes per stream / time.  It gives the rate a plain streaming
+ * kernel reaches on this part, beside the nominal HBM peak.  Allocates and
+ * frees its own buffers.  No reference counterpart (measurement). */
+int msfdm_stream_probe(int device, double mbytes, int reads, int reps, double *tbps);
 /* Algorithmic HBM bytes of one time step (state + coefficients + PML memory). */
 int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
 /* Algorithmic HBM bytes of one launch timed by msfdm_time_kernel(which): an

@@ -880,6 +880,91 @@ extern "C" int msfdm_time_run(msfdm_solver *s, int nsteps, float *ms) {
     return MSFDM_OK;
 }
 
+// Streaming probe: o[i] = in_0[i] + ... + in_{R-1}[i] over float4 elements,
+// four per thread, every load issued before the first add; the inputs are R
+// consecutive streams of n4 float4 in one allocation.
+template <int R>
+__global__ __launch_bounds__(256) void stream_probe_kernel(const float4 *__restrict__ a, long long n4,
+                                                           float4 *__restrict__ o) {
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    float4 v[R][4];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long i = base + k * 256;
+            v[r][k] = i < n4 ? a[r * n4 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + k * 256;
+        if (i >= n4) continue;
+        float4 t = v[0][k];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            t.x += v[r][k].x; t.y += v[r][k].y; t.z += v[r][k].z; t.w += v[r][k].w;
+        }
+        o[i] = t;
+    }
+}
+
+namespace {
+struct ProbeRes {
+    float4 *in = nullptr, *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~ProbeRes() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (in) (void)hipFree(in);
+        if (out) (void)hipFree(out);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+template <int R>
+void stream_probe_launch(const ProbeRes &p, long long n4) {
+    const unsigned blocks = (unsigned)((n4 + 1023) / 1024);
+    stream_probe_kernel<R><<<blocks, 256, 0, p.st>>>(p.in, n4, p.out);
+}
+}  // namespace
+
+extern "C" int msfdm_stream_probe(int device, double mbytes, int reads, int reps, double *tbps) {
+    clear_error();
+    if (!tbps || reads < 1 || reads > 4 || reps < 1 || !(mbytes > 0.0) || mbytes > 65536.0)
+        return set_error(MSFDM_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(device));
+    ProbeRes p;
+    const long long n4 = std::max(1LL, (long long)(mbytes * 1e6 / 16.0));
+    HIPCHK(hipMalloc(&p.in, (size_t)reads * n4 * sizeof(float4)));
+    HIPCHK(hipMalloc(&p.out, (size_t)n4 * sizeof(float4)));
+    HIPCHK(hipStreamCreateWithFlags(&p.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&p.e0));
+    HIPCHK(hipEventCreate(&p.e1));
+    // non-zero data (1.0f): kernels on zero-filled buffers run measurably faster
+    HIPCHK(hipMemsetD32Async((hipDeviceptr_t)p.in, 0x3f800000, (size_t)reads * n4 * 4, p.st));
+    auto launch = [&]() {
+        switch (reads) {
+            case 1: stream_probe_launch<1>(p, n4); break;
+            case 2: stream_probe_launch<2>(p, n4); break;
+            case 3: stream_probe_launch<3>(p, n4); break;
+            default: stream_probe_launch<4>(p, n4); break;
+        }
+    };
+    for (int i = 0; i < 3; ++i) launch();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(p.e0, p.st));
+    for (int i = 0; i < reps; ++i) launch();
+    HIPCHK(hipEventRecord(p.e1, p.st));
+    HIPCHK(hipEventSynchronize(p.e1));
+    HIPCHK(hipGetLastError());
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p.e0, p.e1));
+    *tbps = (double)(reads + 1) * n4 * sizeof(float4) * reps / (ms * 1e-3) / 1e12;
+    return MSFDM_OK;
+}
+
 // The launch sequence of msfdm_run (GRAPH_STEPS-step sequences where msfdm_run
 // replays its graph, single steps at odd step indices) with direct launches;
 // the launches of the classes in class_mask are stamped by their own dispatch.

@@ -4,7 +4,7 @@
 its C ABI; ``msfdm.modelgen`` builds and writes solver input models.
 """
 from ._lib import (ARRAYS, BIN_PATH, FID, FIELDS, LIB_PATH, NFIELD, Batch, HostModel, MsfdmError,
-                   Solver, grid_metrics, last_error, lib, run_cli)
+                   Solver, grid_metrics, last_error, lib, run_cli, stream_probe)
 
 __all__ = ["ARRAYS", "BIN_PATH", "FID", "FIELDS", "LIB_PATH", "NFIELD", "Batch", "HostModel", "MsfdmError",
-           "Solver", "grid_metrics", "last_error", "lib", "run_cli"]
+           "Solver", "grid_metrics", "last_error", "lib", "run_cli", "stream_probe"]

@@ -47,7 +47,7 @@ EXPORTS = [
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
     "msfdm_update_media", "msfdm_grid_metrics", "msfdm_gradient_shots", "msfdm_gradient_shots_plan",
-    "msfdm_dispersion_image", "msfdm_dispersion_image_noalloc",
+    "msfdm_dispersion_image", "msfdm_dispersion_image_noalloc", "msfdm_stream_probe",
 ]
 GRADS = ["K", "rho_f", "C11", "C13", "C33", "C55", "rho_s"]
 
@@ -137,6 +137,7 @@ def lib():
         "msfdm_dispersion_image": ([fp, i, i, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double), i, i,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)], i),
         "msfdm_dispersion_image_noalloc": ([vp, i, i, vp, C.c_double, vp, i, vp, vp], i),
+        "msfdm_stream_probe": ([i, C.c_double, i, i, C.POINTER(C.c_double)], i),
     }
     for name, (args, res) in protos.items():
         fn = getattr(L, name)
@@ -186,6 +187,15 @@ def check_residuals(res, nsteps: int, dt: float, nx: int):
             raise ValueError(f"residual {name}: shape {r.shape}, expected {want} "
                              f"(records of a {nsteps}-step run over every grid column)")
     return rs
+
+
+def stream_probe(mbytes: float, reads: int, reps: int = 20, device: int = 0) -> float:
+    """TB/s of a plain streaming kernel: `reads` input streams and one output
+    stream of `mbytes` MB each (out = sum of the inputs), timed over `reps`
+    launches on the GPU."""
+    v = C.c_double()
+    check(lib().msfdm_stream_probe(int(device), float(mbytes), int(reads), int(reps), C.byref(v)))
+    return v.value
 
 
 def grid_metrics(x, z, device: int = 0) -> dict:

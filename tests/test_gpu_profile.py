@@ -35,3 +35,18 @@ def test_profile_run_equals_run(preset):
     assert q["launches"] == [0, 70, 0, 0, 0, 0] and q["kernel_ms"][1] > 0
     for x, y in zip(a.records(), c.records()):
         assert np.array_equal(x, y)
+
+
+def test_stream_probe_rates():
+    """The streaming probe's rates are positive and below what the part can move
+    (1 GB streams do not fit in any cache); bad arguments raise."""
+    print("[stream-probe] copy and read3-write1 at 1000 MB", flush=True)
+    copy = msfdm.stream_probe(1000, 1, reps=5)
+    triad = msfdm.stream_probe(1000, 3, reps=5)
+    print(f"  copy {copy:.2f} TB/s, read3+write1 {triad:.2f} TB/s", flush=True)
+    for v in (copy, triad):
+        assert 0.5 < v < 8.0, v
+    with pytest.raises(msfdm.MsfdmError):
+        msfdm.stream_probe(1000, 5)
+    with pytest.raises(msfdm.MsfdmError):
+        msfdm.stream_probe(0, 1)
