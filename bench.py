@@ -274,8 +274,9 @@ def bench_solid(args):
     s = msfdm.Solver(model)
     if args.tile_rows:
         s.tile_rows = args.tile_rows
+        s.occupancy = args.occupancy
     else:
-        s.autotune(64)
+        s.autotune(64)   # tile height and occupancy policy from the library's autotuner
     s.reset(0)
     s.run(args.warmup)
     s.sync()
@@ -302,6 +303,7 @@ def bench_solid(args):
             "workload": f"{args.solid_workload}: {model.nx}x{model.nz} solid grid, free surface on row 3, "
                         f"npd={model.npd}, dt={model.dt}, fpeak={model.fpeak}",
             "nx": model.nx, "nz": model.nz, "time_steps": args.steps, "tile_rows": s.tile_rows,
+            "occupancy_policy": s.occupancy,
             "hbm_GBps_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9, 1),
             "hbm_frac_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
             "bytes_per_step_alg": bytes_step,
@@ -364,6 +366,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=100)
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16],
                     help="tile height of the stage kernels (0 = the library's autotuner)")
+    ap.add_argument("--occupancy", type=int, default=0, choices=[0, 1],
+                    help="occupancy policy of the stage kernels when --tile-rows is given")
     ap.add_argument("--mode", default="solver", choices=["solver", "batch", "gradient", "shots", "solid", "mas"])
     ap.add_argument("--solid-workload", default="solid_topo2001")
     ap.add_argument("--shots", type=int, default=32)
@@ -414,8 +418,9 @@ def main():
     s = msfdm.Solver(model)
     if args.tile_rows:
         s.tile_rows = args.tile_rows
+        s.occupancy = args.occupancy
     else:
-        s.autotune(64)   # tile height from the library's autotuner (8 or 16 rows)
+        s.autotune(64)   # tile height (8 or 16 rows) and occupancy policy from the library's autotuner
     s.reset(0)
     s.run(args.warmup)
     s.sync()
@@ -447,7 +452,7 @@ def main():
     kms, nl = prof["kernel_ms"], prof["launches"]
     stage_ms = [kms[i] / max(nl[i], 1) for i in range(4)]
     b_stage, b_stage4 = s.kernel_bytes(0), s.kernel_bytes(1)
-    tile_rows = s.tile_rows
+    tile_rows, occupancy = s.tile_rows, s.occupancy
     achieved = b_stage / (t_stage * 1e-3) / 1e9
     # the same launch repeated back to back (inputs partly cache resident)
     reps = args.kernel_reps
@@ -491,6 +496,7 @@ def main():
             "workload": f"{args.workload}: {model.nx}x{model.nz} grid (fluid {model.nz_fluid} + solid "
                         f"{model.nz_solid} rows), npd={model.npd}, dt={model.dt}, fpeak={model.fpeak}",
             "nx": model.nx, "nz": model.nz, "time_steps": args.steps, "tile_rows": tile_rows,
+            "occupancy_policy": occupancy,
             "hbm_GBps_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9, 1),
             "hbm_frac_per_step_alg": round(bytes_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
             "bytes_per_step_alg": bytes_step,

@@ -153,6 +153,14 @@ int msfdm_get_rhs(msfdm_solver *s, int field, float *out);
  * The default is picked from the grid size; the autotuner overrides it. */
 int msfdm_set_tile_rows(msfdm_solver *s, int rows);
 int msfdm_get_tile_rows(const msfdm_solver *s, int *rows);
+/* Occupancy policy of the stage kernels: 0 (default) = the resident waves per
+ * SIMD tuned per RK stage on the 2001 x 1001 seafloor grid (stage 1 at least
+ * 5, stages 2-3 the compiler's choice, stage 4 at most 4); 1 = at least 5 in
+ * every stage (kernels compiled for it: grids with vertical columns, not the
+ * fluid-solid CPML grid; EINVAL otherwise).  Results are bit-identical under
+ * both; msfdm_autotune picks one.  No reference counterpart. */
+int msfdm_set_occupancy(msfdm_solver *s, int policy);
+int msfdm_get_occupancy(const msfdm_solver *s, int *policy);
 /* Enable (1) / disable (0) hipGraph replay of the time loop (default 1). */
 int msfdm_set_graph(msfdm_solver *s, int enable);
 /* Time `reps` launches of one kernel on the solver's stream with HIP events
@@ -189,10 +197,12 @@ int msfdm_bytes_per_step(const msfdm_solver *s, double *bytes);
  * updated cell. */
 int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
 
-/* Tile-shape autotuner: times nsteps steps for each tile height (8, 16 rows)
- * three times, the heights interleaved, keeps the height with the fastest
- * single timing and resets the shot (no reference counterpart).
- * msfdm_batch_autotune does the same for a batch. */
+/* Tile-shape autotuner: times nsteps steps for each candidate (tile height 8
+ * or 16 rows x occupancy policy 0 or 1 where msfdm_set_occupancy offers both)
+ * three times, the candidates interleaved, keeps the one with the fastest
+ * single timing (msfdm_get_tile_rows / msfdm_get_occupancy read it back) and
+ * resets the shot (no reference counterpart).  msfdm_batch_autotune does the
+ * same over the tile heights for a batch. */
 int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
 
 /* Replace the medium arrays of an existing solver (each [rows][nx] float32 in

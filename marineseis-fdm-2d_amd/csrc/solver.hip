@@ -63,6 +63,7 @@ struct msfdm_solver {
     int isrc = 0;
     int nrec_done = 0;
     int tz = 16;                 // tile height (rows per block)
+    int occ = 0;                 // occupancy policy of the stage kernels (msfdm_set_occupancy)
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -222,15 +223,18 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     if (s->solid_only) {   // rows 3..nzs-4 of the solid-only grid
         const int nb = (s->d.nz_solid - 6 + TZ - 1) / TZ;
         const int grid = 8 * ((nbx + 7) / 8 + (nbx * (nb - 1) + 7) / 8);   // D::xcd_tile with no fluid bands
-#define MSFDM_FREE_LAUNCH(MET, TOP)                                                                              \
-    klaunch(s, STAGE - 1, D::free_stage_kernel<DIR, STAGE, TZ, MET, TOP>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, \
-            s->ctl, S, U, T2, T3, O, j, ct, amp, nbx, nb)
+#define MSFDM_FREE_LAUNCH(KERNEL, MET, TOP)                                                                    \
+    klaunch(s, STAGE - 1, D::KERNEL<DIR, STAGE, TZ, MET, TOP>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, s->ctl, S, \
+            U, T2, T3, O, j, ct, amp, nbx, nb)
+        const bool w5 = s->occ == 1 && s->coef.vert;   // the occupancy variants exist for vertical columns
         if (s->top == D::TOP_FREE) {
-            if (s->coef.vert) MSFDM_FREE_LAUNCH(1, D::TOP_FREE);
-            else MSFDM_FREE_LAUNCH(0, D::TOP_FREE);
+            if (w5) MSFDM_FREE_LAUNCH(free_stage_kernel_w5, 1, D::TOP_FREE);
+            else if (s->coef.vert) MSFDM_FREE_LAUNCH(free_stage_kernel, 1, D::TOP_FREE);
+            else MSFDM_FREE_LAUNCH(free_stage_kernel, 0, D::TOP_FREE);
         } else {
-            if (s->coef.vert) MSFDM_FREE_LAUNCH(1, D::TOP_PLAIN);
-            else MSFDM_FREE_LAUNCH(0, D::TOP_PLAIN);
+            if (w5) MSFDM_FREE_LAUNCH(free_stage_kernel_w5, 1, D::TOP_PLAIN);
+            else if (s->coef.vert) MSFDM_FREE_LAUNCH(free_stage_kernel, 1, D::TOP_PLAIN);
+            else MSFDM_FREE_LAUNCH(free_stage_kernel, 0, D::TOP_PLAIN);
         }
 #undef MSFDM_FREE_LAUNCH
         return;
@@ -245,6 +249,9 @@ void launch_media(msfdm_solver *s, const float *S, const float *U, const float *
     if (s->geo.fcpml)   // top PML in the water; metric form decided at run time
         klaunch(s, cls, D::stage_kernel<DIR, STAGE, TZ, 2, 1, FIN>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, s->ctl,
                 S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
+    else if (s->coef.vert && s->occ == 1 && STAGE > 0)   // the other occupancy policy
+        klaunch(s, cls, D::stage_kernel_w5<DIR, STAGE, TZ, 1, 0, FIN>, dim3(grid), dim3(D::NTHR), s->geo, s->coef,
+                s->ctl, S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
     else if (s->coef.vert)   // vertical grid columns: xi_x, xi_z are constants of the kernel
         klaunch(s, cls, D::stage_kernel<DIR, STAGE, TZ, 1, 0, FIN>, dim3(grid), dim3(D::NTHR), s->geo, s->coef, s->ctl,
                 S, U, T2, T3, O, s->kif, j, ct, amp, cprev, nbx, nbf, nbs, nfi);
@@ -796,6 +803,30 @@ extern "C" int msfdm_get_tile_rows(const msfdm_solver *s, int *rows) {
     return MSFDM_OK;
 }
 
+// Occupancy policies with a kernel of their own; a grid without vertical
+// columns, or the fluid-solid CPML grid, has the default kernels only.
+static bool occupancy_variants(const msfdm_solver *s) { return s->coef.vert && !s->geo.fcpml; }
+
+extern "C" int msfdm_set_occupancy(msfdm_solver *s, int policy) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (policy != 0 && policy != 1) return set_error(MSFDM_EINVAL, "occupancy policy must be 0 or 1");
+    if (policy == 1 && !occupancy_variants(s))
+        return set_error(MSFDM_EINVAL, "occupancy policy 1 needs vertical grid columns and no fluid CPML");
+    if (policy != s->occ) {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        drop_graph(s);
+        s->occ = policy;
+    }
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_get_occupancy(const msfdm_solver *s, int *policy) {
+    if (!s || !policy) return set_error(MSFDM_EINVAL, "null argument");
+    *policy = s->occ;
+    return MSFDM_OK;
+}
+
 extern "C" int msfdm_set_graph(msfdm_solver *s, int enable) {
     if (!s) return set_error(MSFDM_EINVAL, "null solver");
     s->use_graph = enable != 0;
@@ -1342,12 +1373,13 @@ extern "C" int msfdm_batch_time_run(msfdm_batch *b, int nsteps, float *ms) {
     return MSFDM_OK;
 }
 
-// Tile-shape autotuner: every candidate tile height is timed AUTOTUNE_ROUNDS
-// times on `nsteps` steps, the candidates interleaved round by round (each
-// timing after a graph-building warm-up from a fresh shot, so every candidate
-// sees the same early, partly quiet wavefield); the fastest single timing of a
-// candidate is its score, so a clock or power transient during one timing
-// cannot decide the choice.  The state is reset afterwards.
+// Tile-shape autotuner: every candidate (tile height, and for the single-shot
+// solver the occupancy policy) is timed AUTOTUNE_ROUNDS times on `nsteps`
+// steps, the candidates interleaved round by round (each timing after a
+// graph-building warm-up from a fresh shot, so every candidate sees the same
+// early, partly quiet wavefield); the fastest single timing of a candidate is
+// its score, so a clock or power transient during one timing cannot decide the
+// choice.  The state is reset afterwards.
 constexpr int AUTOTUNE_ROUNDS = 3;
 constexpr int AUTOTUNE_ROWS[2] = {8, 16};
 
@@ -1374,20 +1406,27 @@ extern "C" int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows) 
 extern "C" int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows) {
     clear_error();
     if (!s || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
-    float best[2] = {1e30f, 1e30f};
+    const int nocc = occupancy_variants(s) ? 2 : 1;
+    float best[2][2] = {{1e30f, 1e30f}, {1e30f, 1e30f}};   // [tile height][policy]
     for (int round = 0; round < AUTOTUNE_ROUNDS; ++round)
-        for (int k = 0; k < 2; ++k) {
-            if (int rc = msfdm_set_tile_rows(s, AUTOTUNE_ROWS[k])) return rc;
-            if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
-            if (int rc = msfdm_run(s, 2 * GRAPH_STEPS)) return rc;
-            float ms = 0.f;
-            if (int rc = msfdm_time_run(s, nsteps, &ms)) return rc;
-            best[k] = std::min(best[k], ms);
-        }
-    const int arg = best[1] < best[0] ? AUTOTUNE_ROWS[1] : AUTOTUNE_ROWS[0];
-    if (int rc = msfdm_set_tile_rows(s, arg)) return rc;
+        for (int k = 0; k < 2; ++k)
+            for (int o = 0; o < nocc; ++o) {
+                if (int rc = msfdm_set_tile_rows(s, AUTOTUNE_ROWS[k])) return rc;
+                if (int rc = msfdm_set_occupancy(s, o)) return rc;
+                if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
+                if (int rc = msfdm_run(s, 2 * GRAPH_STEPS)) return rc;
+                float ms = 0.f;
+                if (int rc = msfdm_time_run(s, nsteps, &ms)) return rc;
+                best[k][o] = std::min(best[k][o], ms);
+            }
+    int bk = 0, bo = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int o = 0; o < nocc; ++o)
+            if (best[k][o] < best[bk][bo]) { bk = k; bo = o; }
+    if (int rc = msfdm_set_tile_rows(s, AUTOTUNE_ROWS[bk])) return rc;
+    if (int rc = msfdm_set_occupancy(s, bo)) return rc;
     if (int rc = msfdm_reset_shot(s, s->isrc)) return rc;
-    if (best_rows) *best_rows = arg;
+    if (best_rows) *best_rows = AUTOTUNE_ROWS[bk];
     return MSFDM_OK;
 }
 

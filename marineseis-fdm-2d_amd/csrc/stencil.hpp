@@ -1176,17 +1176,22 @@ __device__ __forceinline__ bool solid_interior(const Geo &g, int x0, int z0, int
 template <int STAGE> __host__ __device__ constexpr int stage_max_waves() { return STAGE == 4 ? 4 : 8; }
 template <int STAGE> __host__ __device__ constexpr int stage_min_waves() { return STAGE == 1 ? 5 : 1; }
 
-// FIN = 1 (stage 1): the launch also finishes the previous step's interface rows
-// and record (finish_interface_row); cprev is then the time step.
-template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(stage_min_waves<STAGE>(), stage_max_waves<STAGE>())))
-stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
-             const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O,
-             float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
-             int nbz_s, int nfi) {
+// The other occupancy policy the autotuner may pick for a grid
+// (msfdm_set_occupancy 1): at least 5 resident waves per SIMD in every stage,
+// compiled as stage_kernel_w5 / free_stage_kernel_w5 from the same tile code.
+constexpr int MORE_WAVES_MIN = 5;
+
+// One block's tile of a fluid-solid RK stage.  FIN = 1 (stage 1): the launch
+// also finishes the previous step's interface rows and record
+// (finish_interface_row); cprev is then the time step.
+template <int DIR, int STAGE, int TZ, int MET, int CPML, int FIN>
+__device__ __forceinline__ void stage_block(float (*sh)[TZ + 2 * HALO][LXP], float (*fin)[FW], const Geo &g,
+                                            const Coef &c, const Ctl &ctl, const float *__restrict__ S,
+                                            UPtr<STAGE> U, const float *__restrict__ T2,
+                                            const float *__restrict__ T3, OPtr<STAGE> O, float *__restrict__ kif,
+                                            int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
+                                            int nbz_s, int nfi) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
-    __shared__ float sh[5][TZ + 2 * HALO][LXP];
-    __shared__ float fin[FIN ? 8 : 1][FW];
     const int2 tb = xcd_tile(blockIdx.x, nbx, nbz_f, nbz_s, nfi);
     if (tb.y < 0) return;
     const int bx = tb.x, band = tb.y;
@@ -1210,16 +1215,40 @@ stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
     }
 }
 
+template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(stage_min_waves<STAGE>(), stage_max_waves<STAGE>())))
+stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
+             const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O,
+             float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
+             int nbz_s, int nfi) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    __shared__ float fin[FIN ? 8 : 1][FW];
+    stage_block<DIR, STAGE, TZ, MET, CPML, FIN>(sh, fin, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
+                                                cprev, nbx, nbz_f, nbz_s, nfi);
+}
+
+template <int DIR, int STAGE, int TZ, int MET, int CPML = 0, int FIN = 0>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MORE_WAVES_MIN, 8)))
+stage_kernel_w5(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
+                const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O,
+                float *__restrict__ kif, int jstep, float ct, float amp_direct, float cprev, int nbx, int nbz_f,
+                int nbz_s, int nfi) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    __shared__ float fin[FIN ? 8 : 1][FW];
+    stage_block<DIR, STAGE, TZ, MET, CPML, FIN>(sh, fin, g, c, ctl, S, U, T2, T3, O, kif, jstep, ct, amp_direct,
+                                                cprev, nbx, nbz_f, nbz_s, nfi);
+}
+
 // One RK stage of a solid-only grid (PmlFs::SolidOnly, src/pml_fs_kernels.cu:29-381;
 // Cpml::SolidOnly; Dirichlet, src/dirichlet_kernels.cu:12-163): 64 x TZ tiles
 // of rows 3..nzs-4, the same XCD-aware block order.  Stage 4 also records vz
 // of the receiver row.
 template <int DIR, int STAGE, int TZ, int MET, int TOP>
-__global__ void __launch_bounds__(NTHR)
-free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
-                  const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O, int jstep,
-                  float ct, float amp_direct, int nbx, int nb) {
-    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+__device__ __forceinline__ void free_stage_block(float (*sh)[TZ + 2 * HALO][LXP], const Geo &g, const Coef &c,
+                                                 const Ctl &ctl, const float *__restrict__ S, UPtr<STAGE> U,
+                                                 const float *__restrict__ T2, const float *__restrict__ T3,
+                                                 OPtr<STAGE> O, int jstep, float ct, float amp_direct, int nbx,
+                                                 int nb) {
     // band 0 (the surface rows, the dearest tiles) spread over the XCDs and
     // dispatched first, then contiguous runs of the other bands (xcd_tile with
     // no fluid bands)
@@ -1232,6 +1261,24 @@ free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAG
     else
         solid_body<DIR, STAGE, TZ, MET, TOP, 0, 0>(sh, g, c, ctl, S, U, T2, T3, O, nullptr, jstep, ct, amp_direct, 0.0f,
                                                    bx, band);
+}
+
+template <int DIR, int STAGE, int TZ, int MET, int TOP>
+__global__ void __launch_bounds__(NTHR)
+free_stage_kernel(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
+                  const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O, int jstep,
+                  float ct, float amp_direct, int nbx, int nb) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    free_stage_block<DIR, STAGE, TZ, MET, TOP>(sh, g, c, ctl, S, U, T2, T3, O, jstep, ct, amp_direct, nbx, nb);
+}
+
+template <int DIR, int STAGE, int TZ, int MET, int TOP>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MORE_WAVES_MIN, 8)))
+free_stage_kernel_w5(Geo g, Coef c, Ctl ctl, const float *__restrict__ S, UPtr<STAGE> U,
+                     const float *__restrict__ T2, const float *__restrict__ T3, OPtr<STAGE> O, int jstep,
+                     float ct, float amp_direct, int nbx, int nb) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    free_stage_block<DIR, STAGE, TZ, MET, TOP>(sh, g, c, ctl, S, U, T2, T3, O, jstep, ct, amp_direct, nbx, nb);
 }
 
 // Raw-k mode (STAGE 0, msfdm_eval_rhs): the interface condition applied to the

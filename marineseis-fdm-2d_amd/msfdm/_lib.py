@@ -38,7 +38,7 @@ EXPORTS = [
     "msfdm_solver_create", "msfdm_solver_destroy", "msfdm_reset_shot", "msfdm_run",
     "msfdm_synchronize", "msfdm_step_index", "msfdm_num_records", "msfdm_get_records",
     "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
-    "msfdm_set_tile_rows", "msfdm_get_tile_rows",
+    "msfdm_set_tile_rows", "msfdm_get_tile_rows", "msfdm_set_occupancy", "msfdm_get_occupancy",
     "msfdm_time_kernel", "msfdm_time_run", "msfdm_profile_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
     "msfdm_run_cli", "msfdm_autotune", "msfdm_batch_create", "msfdm_batch_destroy",
     "msfdm_batch_reset", "msfdm_batch_run", "msfdm_batch_synchronize", "msfdm_batch_size",
@@ -105,6 +105,8 @@ def lib():
         "msfdm_set_graph": ([vp, i], i),
         "msfdm_set_tile_rows": ([vp, i], i),
         "msfdm_get_tile_rows": ([vp, ip], i),
+        "msfdm_set_occupancy": ([vp, i], i),
+        "msfdm_get_occupancy": ([vp, ip], i),
         "msfdm_time_kernel": ([vp, i, i, fp], i),
         "msfdm_time_run": ([vp, i, fp], i),
         "msfdm_profile_run": ([vp, i, i, fp, ip, fp], i),
@@ -364,6 +366,18 @@ class Solver:
     @tile_rows.setter
     def tile_rows(self, rows: int):
         check(lib().msfdm_set_tile_rows(self.h, int(rows)))
+
+    @property
+    def occupancy(self) -> int:
+        """Occupancy policy of the stage kernels (0 = tuned per stage, 1 = at
+        least 5 waves per SIMD in every stage); msfdm_autotune picks one."""
+        v = C.c_int()
+        check(lib().msfdm_get_occupancy(self.h, C.byref(v)))
+        return v.value
+
+    @occupancy.setter
+    def occupancy(self, policy: int):
+        check(lib().msfdm_set_occupancy(self.h, int(policy)))
 
     def set_graph(self, enable: bool):
         check(lib().msfdm_set_graph(self.h, int(bool(enable))))
