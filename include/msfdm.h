@@ -161,6 +161,13 @@ int msfdm_get_tile_rows(const msfdm_solver *s, int *rows);
  * both; msfdm_autotune picks one.  No reference counterpart. */
 int msfdm_set_occupancy(msfdm_solver *s, int policy);
 int msfdm_get_occupancy(const msfdm_solver *s, int *policy);
+/* How the adjoint products of a stage are launched: 1 = one launch over every
+ * source tile (boundary tiles first), 0 = three launches (interior tiles of
+ * both media, the other fluid tiles, the other solid tiles), -1 (default) =
+ * one launch up to 65536 tiles over the shots of a launch, three above.  The
+ * gradients are bit-identical either way.  No reference counterpart. */
+int msfdm_set_adjoint_launch(msfdm_solver *s, int mode);
+int msfdm_get_adjoint_launch(const msfdm_solver *s, int *mode);
 /* Enable (1) / disable (0) hipGraph replay of the time loop (default 1). */
 int msfdm_set_graph(msfdm_solver *s, int enable);
 /* Time `reps` launches of one kernel on the solver's stream with HIP events

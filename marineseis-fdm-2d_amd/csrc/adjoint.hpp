@@ -530,29 +530,52 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
 // registers and code of one kind of tile do not weigh on another:
 // ADJ_INTERIOR, the interior tiles of both media (adj_fluid_interior /
 // adj_solid_interior; bodies without a boundary branch); ADJ_FLUID and
-// ADJ_SOLID, the other tiles of each medium.
-constexpr int ADJ_INTERIOR = 0, ADJ_FLUID = 1, ADJ_SOLID = 2;
+// ADJ_SOLID, the other tiles of each medium.  ADJ_ALL runs the three lists
+// (stored one after another: nint interior, nfl fluid, then solid tiles) in
+// one launch, the boundary tiles dispatched first and dealt round-robin over
+// the XCDs: on a grid of a few hundred tiles each split launch is shorter than
+// one wave of blocks and lasts as long as its slowest tile, so one launch that
+// overlaps them beats three that each carry the interior tiles' registers.
+constexpr int ADJ_INTERIOR = 0, ADJ_FLUID = 1, ADJ_SOLID = 2, ADJ_ALL = 3;
 template <int DIR, int TZ, int MET, int KIND>
 __global__ void __launch_bounds__(NTHR)
-adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, const int2 *__restrict__ tiles, int ntiles, int nbz_f) {
+adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, const int2 *__restrict__ tiles, int ntiles, int nbz_f, int nint,
+                int nfl) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];   // forward stage state of the tile + halo
-    constexpr int INT = KIND == ADJ_INTERIOR;
     const AdjIn a = adj_of_shot(a0, blockIdx.y);
     const Grad gr = grad_of_shot(gr0, blockIdx.y);
-    const int per_xcd = gridDim.x >> 3;
-    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (t >= ntiles) return;
+    int t;
+    if constexpr (KIND == ADJ_ALL) {
+        t = ntiles - 1 - (int)blockIdx.x;   // solid, then fluid boundary tiles, then the interior ones
+        if (t < 0) return;
+    } else {
+        const int per_xcd = gridDim.x >> 3;
+        t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+        if (t >= ntiles) return;
+    }
     const int2 tl = tiles[t];
     const int bx = tl.x, band = tl.y;
-    if (KIND == ADJ_FLUID || (KIND == ADJ_INTERIOR && band < nbz_f)) {
-        load_tile<3, TZ, (bool)INT>(sh, a.S, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);
+    const int kind = KIND != ADJ_ALL ? KIND : t < nint ? ADJ_INTERIOR : t < nint + nfl ? ADJ_FLUID : ADJ_SOLID;
+    if (kind == ADJ_INTERIOR) {
+        if (band < nbz_f) {
+            load_tile<3, TZ, true>(sh, a.S, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);
+            __syncthreads();
+            fluid_prod_body<DIR, TZ, MET, 1>(sh, g, c, a, gr, bx, band);
+        } else {
+            const int sb = band - nbz_f;
+            load_tile<5, TZ, true>(sh, a.S, g.off, MSFDM_SVX, sb * TZ, bx * TX, g.nzs, g.P);
+            __syncthreads();
+            solid_prod_body<DIR, TZ, MET, 1>(sh, g, c, a, gr, bx, sb);
+        }
+    } else if (KIND == ADJ_FLUID || (KIND == ADJ_ALL && kind == ADJ_FLUID)) {
+        load_tile<3, TZ, false>(sh, a.S, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);
         __syncthreads();
-        fluid_prod_body<DIR, TZ, MET, INT>(sh, g, c, a, gr, bx, band);
-    } else {
+        fluid_prod_body<DIR, TZ, MET, 0>(sh, g, c, a, gr, bx, band);
+    } else if (KIND == ADJ_SOLID || KIND == ADJ_ALL) {
         const int sb = band - nbz_f;
-        load_tile<5, TZ, (bool)INT>(sh, a.S, g.off, MSFDM_SVX, sb * TZ, bx * TX, g.nzs, g.P);
+        load_tile<5, TZ, false>(sh, a.S, g.off, MSFDM_SVX, sb * TZ, bx * TX, g.nzs, g.P);
         __syncthreads();
-        solid_prod_body<DIR, TZ, MET, INT>(sh, g, c, a, gr, bx, sb);
+        solid_prod_body<DIR, TZ, MET, 0>(sh, g, c, a, gr, bx, sb);
     }
 }
 

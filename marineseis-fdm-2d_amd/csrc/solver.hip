@@ -64,6 +64,7 @@ struct msfdm_solver {
     int nrec_done = 0;
     int tz = 16;                 // tile height (rows per block)
     int occ = 0;                 // occupancy policy of the stage kernels (msfdm_set_occupancy)
+    int adj_launch = -1;         // adjoint products: -1 by grid size, 0 three launches, 1 one launch
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -827,6 +828,20 @@ extern "C" int msfdm_get_occupancy(const msfdm_solver *s, int *policy) {
     return MSFDM_OK;
 }
 
+extern "C" int msfdm_set_adjoint_launch(msfdm_solver *s, int mode) {
+    clear_error();
+    if (!s) return set_error(MSFDM_EINVAL, "null solver");
+    if (mode < -1 || mode > 1) return set_error(MSFDM_EINVAL, "adjoint launch mode must be -1, 0 or 1");
+    s->adj_launch = mode;
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_get_adjoint_launch(const msfdm_solver *s, int *mode) {
+    if (!s || !mode) return set_error(MSFDM_EINVAL, "null argument");
+    *mode = s->adj_launch;
+    return MSFDM_OK;
+}
+
 extern "C" int msfdm_set_graph(msfdm_solver *s, int enable) {
     if (!s) return set_error(MSFDM_EINVAL, "null solver");
     s->use_graph = enable != 0;
@@ -1443,6 +1458,12 @@ enum AdjRole { A_LAM = 0, A_LP = 1, A_ACC = 2, A_TBA = 3, A_TBB = 4 };
 // Tile height of the adjoint stage kernels (4-row product tiles measured
 // slower: 186 against 168 us on 32 shots of 401 x 201).
 constexpr int ADJ_TZ = 8;
+// Up to this many source tiles (over all shots of a launch) the adjoint
+// products run as one launch (ADJ_ALL), above it as three.  Measured: one
+// launch is 1.32x faster for a gradient of one 401 x 201 shot (182 tiles),
+// 1.10x for 32 of them, 1.02-1.06x for 1-4 shots of 4001 x 1501 (11844 tiles
+// each), 0.96x for 6 and 0.94x for 32 of those.
+constexpr long long ADJ_ONE_LAUNCH_TILES = 65536;
 
 // Floats of one shot's adjoint product buffer (fluid products, then solid).
 long long adj_prod_floats(const msfdm_solver *s) {
@@ -1523,9 +1544,22 @@ void launch_adj_met(msfdm_solver *s, hipStream_t st, const D::AdjIn &a, const D:
     const int2 *tl = s->atiles;
     const int *n = s->atiles_n;
     auto grid = [&](int nt) { return dim3(8 * ((nt + 7) / 8), nshot); };
-    if (n[0]) D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_INTERIOR><<<grid(n[0]), D::NTHR, 0, st>>>(g, c, a, gr, tl, n[0], nbf);
-    if (n[1]) D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_FLUID><<<grid(n[1]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0], n[1], nbf);
-    if (n[2]) D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_SOLID><<<grid(n[2]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0] + n[1], n[2], nbf);
+    const int nall = n[0] + n[1] + n[2];
+    const bool one_launch = s->adj_launch >= 0 ? s->adj_launch == 1 : (long long)nall * nshot <= ADJ_ONE_LAUNCH_TILES;
+    if (one_launch) {
+        D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_ALL><<<dim3(nall, nshot), D::NTHR, 0, st>>>(g, c, a, gr, tl, nall, nbf,
+                                                                                          n[0], n[1]);
+    } else {
+        if (n[0])
+            D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_INTERIOR><<<grid(n[0]), D::NTHR, 0, st>>>(g, c, a, gr, tl, n[0], nbf,
+                                                                                             n[0], n[1]);
+        if (n[1])
+            D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_FLUID><<<grid(n[1]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0], n[1],
+                                                                                          nbf, n[0], n[1]);
+        if (n[2])
+            D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_SOLID><<<grid(n[2]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0] + n[1],
+                                                                                          n[2], nbf, n[0], n[1]);
+    }
     const int gbx = g.P / D::TX;            // output tiles cover every column of the pitch
     const int gbf = (nzf + TZ - 1) / TZ, gbs = (nzs + TZ - 1) / TZ;
     D::adj_gather_kernel<DIR, TZ, MET><<<grid(gbx * (gbf + gbs)), D::NTHR, 0, st>>>(g, c, a, gbx, gbf, gbs);

@@ -39,6 +39,7 @@ EXPORTS = [
     "msfdm_synchronize", "msfdm_step_index", "msfdm_num_records", "msfdm_get_records",
     "msfdm_get_field", "msfdm_set_field", "msfdm_eval_rhs", "msfdm_get_rhs", "msfdm_set_graph",
     "msfdm_set_tile_rows", "msfdm_get_tile_rows", "msfdm_set_occupancy", "msfdm_get_occupancy",
+    "msfdm_set_adjoint_launch", "msfdm_get_adjoint_launch",
     "msfdm_time_kernel", "msfdm_time_run", "msfdm_profile_run", "msfdm_bytes_per_step", "msfdm_kernel_bytes",
     "msfdm_run_cli", "msfdm_autotune", "msfdm_batch_create", "msfdm_batch_destroy",
     "msfdm_batch_reset", "msfdm_batch_run", "msfdm_batch_synchronize", "msfdm_batch_size",
@@ -107,6 +108,8 @@ def lib():
         "msfdm_get_tile_rows": ([vp, ip], i),
         "msfdm_set_occupancy": ([vp, i], i),
         "msfdm_get_occupancy": ([vp, ip], i),
+        "msfdm_set_adjoint_launch": ([vp, i], i),
+        "msfdm_get_adjoint_launch": ([vp, ip], i),
         "msfdm_time_kernel": ([vp, i, i, fp], i),
         "msfdm_time_run": ([vp, i, fp], i),
         "msfdm_profile_run": ([vp, i, i, fp, ip, fp], i),
@@ -378,6 +381,17 @@ class Solver:
     @occupancy.setter
     def occupancy(self, policy: int):
         check(lib().msfdm_set_occupancy(self.h, int(policy)))
+
+    @property
+    def adjoint_launch(self) -> int:
+        """Adjoint products per stage: 1 = one launch, 0 = three, -1 = by grid size."""
+        v = C.c_int()
+        check(lib().msfdm_get_adjoint_launch(self.h, C.byref(v)))
+        return v.value
+
+    @adjoint_launch.setter
+    def adjoint_launch(self, mode: int):
+        check(lib().msfdm_set_adjoint_launch(self.h, int(mode)))
 
     def set_graph(self, enable: bool):
         check(lib().msfdm_set_graph(self.h, int(bool(enable))))

@@ -1,9 +1,12 @@
 """Multi-shot gradient (msfdm_gradient_shots): all shots of a wave stepped together.
 
 * equals the in-order sum of the single-shot gradients (msfdm_gradient) bit for
-  bit at the same checkpoint interval (the adjoint adds its tile colours in a
-  fixed order and both paths fold the same segment sums), with the memory
-  budget forcing waves of 2 + 1 shots and with all shots in one wave;
+  bit at the same checkpoint interval (every adjoint output cell has one writer
+  that sums its terms in a fixed order, and both paths fold the same segment
+  sums), with the memory budget forcing waves of 2 + 1 shots and with all
+  shots in one wave;
+* the adjoint products as one launch or as three give the same bits, single
+  shot and multi-shot;
 * matches the sum of the FP64 oracle's per-shot gradients to 1e-4 relative L2;
 * a budget below one shot is refused with MSFDM_EDEVICE, a bad shot index
   with MSFDM_EINVAL.
@@ -87,3 +90,25 @@ def test_shots_errors():
         s.gradient_shots([0], NSTEPS, _residuals(m, 30)[:1])
     with pytest.raises(ValueError):
         s.gradient(0, NSTEPS, _residuals(m, 30)[0])
+
+
+def test_adjoint_launch_modes_agree_bitwise():
+    print("[shots] adjoint products: one launch against three", flush=True)
+    m = _model()
+    s = msfdm.Solver(m)
+    s.reset(0); s.run(NSTEPS); s.sync()
+    res = _residuals(m, s.num_records())
+    assert s.adjoint_launch == -1
+    out = {}
+    for mode in (0, 1):
+        s.adjoint_launch = mode
+        assert s.adjoint_launch == mode
+        g1 = s.gradient(1, NSTEPS, res[1], every=8)
+        gs, _ = s.gradient_shots([0, 1, 2], NSTEPS, res, every=8)
+        out[mode] = (g1, gs)
+    for k in range(2):
+        for n in msfdm._lib.GRADS:
+            assert np.array_equal(out[0][k][n], out[1][k][n]), (k, n)
+            assert np.any(out[0][k][n] != 0), (k, n)
+    with pytest.raises(msfdm.MsfdmError):
+        s.adjoint_launch = 2
