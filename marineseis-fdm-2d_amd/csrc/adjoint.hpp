@@ -530,32 +530,29 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
 // registers and code of one kind of tile do not weigh on another:
 // ADJ_INTERIOR, the interior tiles of both media (adj_fluid_interior /
 // adj_solid_interior; bodies without a boundary branch); ADJ_FLUID and
-// ADJ_SOLID, the other tiles of each medium.  ADJ_ALL runs the three lists
-// (stored one after another: nint interior, nfl fluid, then solid tiles) in
-// one launch, the boundary tiles dispatched first and dealt round-robin over
-// the XCDs: on a grid of a few hundred tiles each split launch is shorter than
-// one wave of blocks and lasts as long as its slowest tile, so one launch that
-// overlaps them beats three that each carry the interior tiles' registers.
+// ADJ_SOLID, the other tiles of each medium.  ADJ_ALL runs every tile in one
+// launch from a list in eight equal parts, one per XCD, each with its share of
+// the solid and fluid boundary tiles first and then of the interior tiles; the
+// band of an entry carries its kind (band + kind * ADJ_KIND_SHIFT) and entries
+// with a negative column pad the parts.  On a grid of a few hundred tiles each
+// split launch is shorter than one wave of blocks and lasts as long as its
+// slowest tile, so one launch that overlaps them beats three.
 constexpr int ADJ_INTERIOR = 0, ADJ_FLUID = 1, ADJ_SOLID = 2, ADJ_ALL = 3;
+constexpr int ADJ_KIND_SHIFT = 1 << 24;
 template <int DIR, int TZ, int MET, int KIND>
 __global__ void __launch_bounds__(NTHR)
-adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, const int2 *__restrict__ tiles, int ntiles, int nbz_f, int nint,
-                int nfl) {
+adj_prod_kernel(Geo g, Coef c, AdjIn a0, Grad gr0, const int2 *__restrict__ tiles, int ntiles, int nbz_f) {
     __shared__ float sh[5][TZ + 2 * HALO][LXP];   // forward stage state of the tile + halo
     const AdjIn a = adj_of_shot(a0, blockIdx.y);
     const Grad gr = grad_of_shot(gr0, blockIdx.y);
-    int t;
-    if constexpr (KIND == ADJ_ALL) {
-        t = ntiles - 1 - (int)blockIdx.x;   // solid, then fluid boundary tiles, then the interior ones
-        if (t < 0) return;
-    } else {
-        const int per_xcd = gridDim.x >> 3;
-        t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-        if (t >= ntiles) return;
-    }
+    const int per_xcd = gridDim.x >> 3;
+    const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= ntiles) return;
     const int2 tl = tiles[t];
-    const int bx = tl.x, band = tl.y;
-    const int kind = KIND != ADJ_ALL ? KIND : t < nint ? ADJ_INTERIOR : t < nint + nfl ? ADJ_FLUID : ADJ_SOLID;
+    if (KIND == ADJ_ALL && tl.x < 0) return;   // padding of an XCD's part
+    const int bx = tl.x;
+    const int band = KIND == ADJ_ALL ? tl.y % ADJ_KIND_SHIFT : tl.y;
+    const int kind = KIND == ADJ_ALL ? tl.y / ADJ_KIND_SHIFT : KIND;
     if (kind == ADJ_INTERIOR) {
         if (band < nbz_f) {
             load_tile<3, TZ, true>(sh, a.S, g.off, MSFDM_FVX, 3 + band * TZ, bx * TX, g.nzf, g.P);

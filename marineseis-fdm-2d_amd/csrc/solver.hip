@@ -76,6 +76,7 @@ struct msfdm_solver {
     float *aprod = nullptr;          // adjoint products of one stage (adj_prod_kernel -> adj_gather_kernel)
     int2 *atiles = nullptr;          // adjoint source tiles: interior, fluid boundary, solid boundary
     int atiles_n[3] = {0, 0, 0};
+    int atiles_xcd = 0;              // entries per XCD of the one-launch list (after the three lists)
     double *gmem = nullptr;          // 7 gradient arrays, [rows][P] doubles, three times:
                                      // segment partial sums, Kahan totals, compensations
     long long gfl = 0;               // doubles of one set of 7 arrays
@@ -1474,7 +1475,12 @@ long long adj_prod_floats(const msfdm_solver *s) {
 // The adjoint source tiles of the grid (column tile, band; fluid bands first)
 // in three lists, one per products kernel: the interior tiles of both media
 // (D::ADJ_INTERIOR), then the other fluid tiles (D::ADJ_FLUID) and the other
-// solid tiles (D::ADJ_SOLID).
+// solid tiles (D::ADJ_SOLID).  After them, the list of the one-launch form
+// (D::ADJ_ALL): eight equal parts, one per XCD, each holding one eighth of
+// the solid boundary tiles, then of the fluid boundary tiles, then of the
+// interior tiles, every eighth a contiguous run of its list (neighbouring
+// tiles share halo rows in the XCD's L2); the band carries the tile's kind
+// (band + kind * D::ADJ_KIND_SHIFT), entries (-1, -1) pad the parts.
 int ensure_adj_tiles(msfdm_solver *s) {
     if (s->atiles) return MSFDM_OK;
     const int nx = s->d.nx, npd = s->geo.npd, nzf = s->d.nz_fluid, nzs = s->d.nz_solid;
@@ -1492,6 +1498,20 @@ int ensure_adj_tiles(msfdm_solver *s) {
     for (int k = 0; k < 3; ++k) {
         s->atiles_n[k] = (int)list[k].size();
         all.insert(all.end(), list[k].begin(), list[k].end());
+    }
+    std::vector<int2> part[8];
+    for (int k : {D::ADJ_SOLID, D::ADJ_FLUID, D::ADJ_INTERIOR}) {
+        const size_t n = list[k].size();
+        for (int x = 0; x < 8; ++x)
+            for (size_t i = n * x / 8; i < n * (x + 1) / 8; ++i)
+                part[x].push_back(make_int2(list[k][i].x, list[k][i].y + k * D::ADJ_KIND_SHIFT));
+    }
+    size_t per = 0;
+    for (auto &q : part) per = std::max(per, q.size());
+    s->atiles_xcd = (int)per;
+    for (auto &q : part) {
+        q.resize(per, make_int2(-1, -1));
+        all.insert(all.end(), q.begin(), q.end());
     }
     HIPCHK(hipMalloc(&s->atiles, all.size() * sizeof(int2)));
     HIPCHK(hipMemcpy(s->atiles, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -1547,18 +1567,17 @@ void launch_adj_met(msfdm_solver *s, hipStream_t st, const D::AdjIn &a, const D:
     const int nall = n[0] + n[1] + n[2];
     const bool one_launch = s->adj_launch >= 0 ? s->adj_launch == 1 : (long long)nall * nshot <= ADJ_ONE_LAUNCH_TILES;
     if (one_launch) {
-        D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_ALL><<<dim3(nall, nshot), D::NTHR, 0, st>>>(g, c, a, gr, tl, nall, nbf,
-                                                                                          n[0], n[1]);
+        D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_ALL><<<dim3(8 * s->atiles_xcd, nshot), D::NTHR, 0, st>>>(
+            g, c, a, gr, tl + nall, 8 * s->atiles_xcd, nbf);
     } else {
         if (n[0])
-            D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_INTERIOR><<<grid(n[0]), D::NTHR, 0, st>>>(g, c, a, gr, tl, n[0], nbf,
-                                                                                             n[0], n[1]);
+            D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_INTERIOR><<<grid(n[0]), D::NTHR, 0, st>>>(g, c, a, gr, tl, n[0], nbf);
         if (n[1])
             D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_FLUID><<<grid(n[1]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0], n[1],
-                                                                                          nbf, n[0], n[1]);
+                                                                                          nbf);
         if (n[2])
             D::adj_prod_kernel<DIR, TZ, MET, D::ADJ_SOLID><<<grid(n[2]), D::NTHR, 0, st>>>(g, c, a, gr, tl + n[0] + n[1],
-                                                                                          n[2], nbf, n[0], n[1]);
+                                                                                          n[2], nbf);
     }
     const int gbx = g.P / D::TX;            // output tiles cover every column of the pitch
     const int gbf = (nzf + TZ - 1) / TZ, gbs = (nzs + TZ - 1) / TZ;
