@@ -83,7 +83,9 @@ struct AdjIn {
     const float *TBp;    // previous stage's Tbar (LP with cprev = 0 when there is none)
     const float *qif;    // [8][P] interface qbar = B^T kbar
     float *TB;           // output Tbar (null: the last stage of a step, whose Tbar only goes to ACC)
-    float *ACC;          // null, or the step's accumulator: ACC += TB wherever the stage writes TB
+    float *ACC;          // null, or the step's accumulator: ACC = ACCin + TB wherever the stage writes TB
+    const float *ACCin;  // the accumulator's input: ACC itself, or LP for the first stage of a step
+                         // (which so sets ACC = LP + TB on every entry a stage writes)
     float *prod;         // adjoint products: [ADJ_NQF][nzf][P], then [ADJ_NQS][nzs][P]
     float clam, cprev;
     long long bstride;   // floats between the state buffers of consecutive shots
@@ -112,7 +114,7 @@ __device__ __forceinline__ AdjIn adj_of_shot(AdjIn a, int shot) {
     const long long o = (long long)shot * a.bstride;
     a.S += o; a.LP += o; a.TBp += o;
     if (a.TB) a.TB += o;
-    if (a.ACC) a.ACC += o;
+    if (a.ACC) { a.ACC += o; a.ACCin += o; }
     a.qif += (long long)shot * a.qstride;
     a.prod += (long long)shot * a.pstride;
     return a;
@@ -128,11 +130,11 @@ __device__ __forceinline__ T ldc(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
 }
 
-// Tbar[i] = v (unless the stage keeps no Tbar), and ACC[i] += v when the
-// stage accumulates into ACC.
+// Tbar[i] = v (unless the stage keeps no Tbar), and ACC[i] = ACCin[i] + v
+// when the stage accumulates into ACC (each entry has one writer per stage).
 __device__ __forceinline__ void tb_out(const AdjIn &a, long long i, float v) {
     if (a.TB) a.TB[i] = v;
-    if (a.ACC) a.ACC[i] += v;
+    if (a.ACC) a.ACC[i] = a.ACCin[i] + v;
 }
 
 // kbar = clam LP + cprev TBp (the host passes TBp = LP and cprev = 0 for the
@@ -645,7 +647,7 @@ __device__ __forceinline__ void fluid_gather_body(const float (*sh)[TZ + 2 * HAL
         const int t = z0 + ty * RW + j;
         const long long gi = (long long)t * P + ix;
 #pragma unroll
-        for (int f = 0; f < 3; ++f) acc[j][f] = (a.ACC && t <= nzf - 1) ? a.ACC[g.off[MSFDM_FVX + f] + gi] : 0.0f;
+        for (int f = 0; f < 3; ++f) acc[j][f] = (a.ACC && t <= nzf - 1) ? a.ACCin[g.off[MSFDM_FVX + f] + gi] : 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
@@ -719,7 +721,7 @@ __device__ __forceinline__ void solid_gather_body(const float (*sh)[TZ + 2 * HAL
         const int t = z0 + ty * RW + j;
         const long long si = (long long)t * P + ix;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) acc[j][f] = (a.ACC && t <= nzs - 1) ? a.ACC[g.off[MSFDM_SVX + f] + si] : 0.0f;
+        for (int f = 0; f < 5; ++f) acc[j][f] = (a.ACC && t <= nzs - 1) ? a.ACCin[g.off[MSFDM_SVX + f] + si] : 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
@@ -876,6 +878,18 @@ __device__ __forceinline__ void filter_map(float2 n, const float v[8], float o[8
 }
 
 // LP interface rows (columns 3..nx-4) = Filter^T(LAM interface rows).
+// The eight interface-row fields of src copied to the same entries of dst
+// (so that adj_filter can read them from dst while it writes src's).
+__global__ void __launch_bounds__(256) iface_rows_copy(Geo g, const float *__restrict__ src, float *__restrict__ dst,
+                                                       long long bstride) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= g.nx) return;
+    src += (long long)blockIdx.y * bstride;
+    dst += (long long)blockIdx.y * bstride;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) dst[iface_index(g, f, ix)] = src[iface_index(g, f, ix)];
+}
+
 __global__ void __launch_bounds__(FB) adj_filter(Geo g, Coef c, const float *__restrict__ LAM, float *__restrict__ LP,
                                                   long long bstride) {
     __shared__ float vb[8][FB + 2 * HALO];

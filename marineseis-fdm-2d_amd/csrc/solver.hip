@@ -1595,12 +1595,15 @@ void kahan_fold(hipStream_t st, double *part, long long n) {
 
 // One adjoint stage: TB = L_dir^T B_dir^T (clam LP + cprev TBp) at forward state S.
 // Every entry of TB a stage can reach is written by it (the others stay zero).
-// ACC (may be null): the step's accumulator, ACC += TB.
+// ACC (may be null): the step's accumulator, ACC += TB; the first stage of a
+// step (first) sets ACC = LP + TB instead.  LP: lambda' (null: the probe
+// input buffer).
 template <int DIR>
 int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, float clam, float cprev,
-              float *ACC = nullptr) {
+              float *ACC = nullptr, const float *LP = nullptr, bool first = false) {
     D::AdjIn a;
-    a.S = S; a.LP = s->adj[A_LP]; a.qif = s->qif; a.TB = TB; a.ACC = ACC; a.prod = s->aprod;
+    a.S = S; a.LP = LP ? LP : s->adj[A_LP]; a.qif = s->qif; a.TB = TB; a.ACC = ACC; a.prod = s->aprod;
+    a.ACCin = first ? a.LP : ACC;
     a.TBp = TBp ? TBp : a.LP;   // no previous Tbar: LP with weight 0
     a.clam = clam; a.cprev = TBp ? cprev : 0.0f;
     a.bstride = 0; a.qstride = 0; a.pstride = 0;
@@ -1612,22 +1615,28 @@ int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, floa
 
 // Reverse of one RK4 step `it` (forward stage states Un, T2, T3, T4):
 // lambda <- lambda' + sum of the four stage transposes, lambda' = Filter^T lambda.
+// Filter^T changes only the interface rows, so it runs in place on lambda
+// (those rows staged in TA, which the first stage overwrites), and lambda'
+// is lambda; the first stage writes ACC = lambda' + Tbar over every entry a
+// stage writes, so ACC needs no copy of lambda' (the entries no stage writes
+// are zero in every adjoint buffer).
 int adj_step(msfdm_solver *s, int it, const float *Un, const float *T2, const float *T3, const float *T4) {
-    const size_t bytes = s->buf_floats * sizeof(float);
-    float *LAM = s->adj[A_LAM], *LP = s->adj[A_LP], *ACC = s->adj[A_ACC], *TA = s->adj[A_TBA], *TBb = s->adj[A_TBB];
-    HIPCHK(hipMemcpyAsync(LP, LAM, bytes, hipMemcpyDeviceToDevice, s->stream));
-    D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, LAM, LP, 0);
-    HIPCHK(hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream));
+    float *LAM = s->adj[A_LAM], *ACC = s->adj[A_ACC], *TA = s->adj[A_TBA], *TBb = s->adj[A_TBB];
+    D::iface_rows_copy<<<(s->d.nx + 255) / 256, 256, 0, s->stream>>>(s->geo, LAM, TA, 0);
+    D::adj_filter<<<(s->d.nx + D::FB - 1) / D::FB, D::FB, 0, s->stream>>>(s->geo, s->coef, TA, LAM, 0);
+    const float *LP = LAM;
     const float dt = s->d.dt;
     const bool odd = it & 1;                 // forward stage directions: d1 d2 d1 d2
     int rc;
     // stage 4 (d2, state T4), stage 3 (d1, T3), stage 2 (d2, T2), stage 1 (d1, U^n);
     // each adds its Tbar to ACC
-    if ((rc = odd ? adj_stage<0>(s, T4, nullptr, TA, dt / 6, 0.f, ACC) : adj_stage<1>(s, T4, nullptr, TA, dt / 6, 0.f, ACC))) return rc;
-    if ((rc = odd ? adj_stage<1>(s, T3, TA, TBb, dt / 3, dt, ACC) : adj_stage<0>(s, T3, TA, TBb, dt / 3, dt, ACC))) return rc;
-    if ((rc = odd ? adj_stage<0>(s, T2, TBb, TA, dt / 3, dt / 2, ACC) : adj_stage<1>(s, T2, TBb, TA, dt / 3, dt / 2, ACC))) return rc;
+    if ((rc = odd ? adj_stage<0>(s, T4, nullptr, TA, dt / 6, 0.f, ACC, LP, true)
+                  : adj_stage<1>(s, T4, nullptr, TA, dt / 6, 0.f, ACC, LP, true))) return rc;
+    if ((rc = odd ? adj_stage<1>(s, T3, TA, TBb, dt / 3, dt, ACC, LP) : adj_stage<0>(s, T3, TA, TBb, dt / 3, dt, ACC, LP))) return rc;
+    if ((rc = odd ? adj_stage<0>(s, T2, TBb, TA, dt / 3, dt / 2, ACC, LP) : adj_stage<1>(s, T2, TBb, TA, dt / 3, dt / 2, ACC, LP))) return rc;
     // (the last stage's Tbar only goes to ACC)
-    if ((rc = odd ? adj_stage<1>(s, Un, TA, nullptr, dt / 6, dt / 2, ACC) : adj_stage<0>(s, Un, TA, nullptr, dt / 6, dt / 2, ACC))) return rc;
+    if ((rc = odd ? adj_stage<1>(s, Un, TA, nullptr, dt / 6, dt / 2, ACC, LP)
+                  : adj_stage<0>(s, Un, TA, nullptr, dt / 6, dt / 2, ACC, LP))) return rc;
     std::swap(s->adj[A_LAM], s->adj[A_ACC]);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
@@ -1955,11 +1964,15 @@ D::Grad wave_grad(const ShotsWave &w) {
     return g;
 }
 
+// One adjoint stage of the wave (adj_stage's arithmetic; LP: lambda', first:
+// the step's first stage, which sets ACC = LP + Tbar).
 template <int DIR>
-void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, float *TB, float clam, float cprev) {
+void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, float *TB, float clam, float cprev,
+                    const float *LP, bool first = false) {
     msfdm_solver *s = w.s;
     D::AdjIn a;
-    a.S = S; a.LP = w.adj[W_LP]; a.qif = w.qif; a.TB = TB; a.ACC = w.adj[W_ACC]; a.prod = w.prod;
+    a.S = S; a.LP = LP; a.qif = w.qif; a.TB = TB; a.ACC = w.adj[W_ACC]; a.prod = w.prod;
+    a.ACCin = first ? LP : a.ACC;
     a.TBp = TBp ? TBp : a.LP;   // no previous Tbar: LP with weight 0
     a.clam = clam; a.cprev = TBp ? cprev : 0.0f;
     a.bstride = w.p.bf;
@@ -1971,24 +1984,26 @@ void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, floa
 
 // Reverse of step `it` for the kk shots of the wave (segment slot j holds the
 // forward stage states of that step).
+// As adj_step: Filter^T in place on lambda (interface rows staged in TA),
+// lambda' = lambda, and the first stage writes ACC = lambda' + Tbar.
 void wave_adj_step(ShotsWave &w, int kk, int it, int j) {
     msfdm_solver *s = w.s;
-    const long long n = (long long)kk * w.p.bf;
-    const size_t bytes = (size_t)n * sizeof(float);
-    float *LAM = w.adj[W_LAM], *LP = w.adj[W_LP], *ACC = w.adj[W_ACC], *TA = w.adj[W_TA], *TBb = w.adj[W_TB];
-    (void)hipMemcpyAsync(LP, LAM, bytes, hipMemcpyDeviceToDevice, s->stream);
-    D::adj_filter<<<dim3((s->d.nx + D::FB - 1) / D::FB, kk), D::FB, 0, s->stream>>>(s->geo, s->coef, LAM, LP, w.p.bf);
-    (void)hipMemcpyAsync(ACC, LP, bytes, hipMemcpyDeviceToDevice, s->stream);
+    float *LAM = w.adj[W_LAM], *TA = w.adj[W_TA], *TBb = w.adj[W_TB];
+    D::iface_rows_copy<<<dim3((s->d.nx + 255) / 256, kk), 256, 0, s->stream>>>(s->geo, LAM, TA, w.p.bf);
+    D::adj_filter<<<dim3((s->d.nx + D::FB - 1) / D::FB, kk), D::FB, 0, s->stream>>>(s->geo, s->coef, TA, LAM, w.p.bf);
+    const float *LP = LAM;
     const float dt = s->d.dt;
     const bool odd = it & 1;
     const float *Un = w.slab(w.seg, 4LL * j), *T2 = w.slab(w.seg, 4LL * j + 1), *T3 = w.slab(w.seg, 4LL * j + 2),
                 *T4 = w.slab(w.seg, 4LL * j + 3);
     // each stage adds its Tbar to ACC (wave_adj_stage)
-    if (odd) wave_adj_stage<0>(w, kk, T4, nullptr, TA, dt / 6, 0.f); else wave_adj_stage<1>(w, kk, T4, nullptr, TA, dt / 6, 0.f);
-    if (odd) wave_adj_stage<1>(w, kk, T3, TA, TBb, dt / 3, dt); else wave_adj_stage<0>(w, kk, T3, TA, TBb, dt / 3, dt);
-    if (odd) wave_adj_stage<0>(w, kk, T2, TBb, TA, dt / 3, dt / 2); else wave_adj_stage<1>(w, kk, T2, TBb, TA, dt / 3, dt / 2);
+    if (odd) wave_adj_stage<0>(w, kk, T4, nullptr, TA, dt / 6, 0.f, LP, true);
+    else wave_adj_stage<1>(w, kk, T4, nullptr, TA, dt / 6, 0.f, LP, true);
+    if (odd) wave_adj_stage<1>(w, kk, T3, TA, TBb, dt / 3, dt, LP); else wave_adj_stage<0>(w, kk, T3, TA, TBb, dt / 3, dt, LP);
+    if (odd) wave_adj_stage<0>(w, kk, T2, TBb, TA, dt / 3, dt / 2, LP); else wave_adj_stage<1>(w, kk, T2, TBb, TA, dt / 3, dt / 2, LP);
     // (the last stage's Tbar only goes to ACC)
-    if (odd) wave_adj_stage<1>(w, kk, Un, TA, nullptr, dt / 6, dt / 2); else wave_adj_stage<0>(w, kk, Un, TA, nullptr, dt / 6, dt / 2);
+    if (odd) wave_adj_stage<1>(w, kk, Un, TA, nullptr, dt / 6, dt / 2, LP);
+    else wave_adj_stage<0>(w, kk, Un, TA, nullptr, dt / 6, dt / 2, LP);
     std::swap(w.adj[W_LAM], w.adj[W_ACC]);
 }
 
