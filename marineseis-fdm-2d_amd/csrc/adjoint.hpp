@@ -3,7 +3,8 @@
 // The forward stage is k = B_dir(L_dir(S) + src) (interface BC B, RHS L of
 // include/pml_fs_kernels.cuh:333-827).  One adjoint stage takes
 // kbar = clam * LP + cprev * TBprev (the RK4 stage weights), applies B^T on the
-// interface rows (adj_iface_q) and forms TB = L^T(qbar) in two launches:
+// interface rows (iface_qbar, inside the products kernel) and forms
+// TB = L^T(qbar) in two launches:
 //   adj_prod_kernel    per source cell (64 x TZ tiles, the forward state S of
 //                      the tile and its 3-cell halo in LDS): every forward
 //                      expression k[i] = c(i) D(S)[i] (D a one-sided stencil
@@ -23,7 +24,8 @@
 //                      reflections.
 // Every output cell has one writer and sums its terms in a fixed order, so
 // repeated runs give the same bits; nothing needs zeroing between stages.
-//   adj_filter        Filter^T on the interface rows (out of place, LDS).
+//   adj_filter        Filter^T on the interface rows (LDS; its input rows are
+//                     staged by iface_rows_copy, so it can write them in place).
 // The per-column 8x8 interface BC and filter maps are transposed by probing
 // the forward maps with unit vectors.
 #pragma once
@@ -81,7 +83,6 @@ struct AdjIn {
     const float *S;      // forward stage state
     const float *LP;     // lambda'
     const float *TBp;    // previous stage's Tbar (LP with cprev = 0 when there is none)
-    const float *qif;    // [8][P] interface qbar = B^T kbar
     float *TB;           // output Tbar (null: the last stage of a step, whose Tbar only goes to ACC)
     float *ACC;          // null, or the step's accumulator: ACC = ACCin + TB wherever the stage writes TB
     const float *ACCin;  // the accumulator's input: ACC itself, or LP for the first stage of a step
@@ -89,7 +90,6 @@ struct AdjIn {
     float *prod;         // adjoint products: [ADJ_NQF][nzf][P], then [ADJ_NQS][nzs][P]
     float clam, cprev;
     long long bstride;   // floats between the state buffers of consecutive shots
-    long long qstride;   // floats between the qif buffers of consecutive shots
     long long pstride;   // floats between the product buffers of consecutive shots
 };
 
@@ -115,7 +115,6 @@ __device__ __forceinline__ AdjIn adj_of_shot(AdjIn a, int shot) {
     a.S += o; a.LP += o; a.TBp += o;
     if (a.TB) a.TB += o;
     if (a.ACC) { a.ACC += o; a.ACCin += o; }
-    a.qif += (long long)shot * a.qstride;
     a.prod += (long long)shot * a.pstride;
     return a;
 }
@@ -143,6 +142,40 @@ __device__ __forceinline__ float kbar(const AdjIn &a, long long i) {
     float v = a.clam * ldc(&a.LP[i]);
     v += a.cprev * ldc(&a.TBp[i]);
     return v;
+}
+
+// 8x8 interface BC matrix of one column (columns = unit inputs).
+template <int DIR>
+__device__ __forceinline__ void bc_matrix(float2 n, float M[8][8]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float k[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) k[i] = i == j ? 1.0f : 0.0f;
+        interface_bc<DIR>(k, n);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) M[i][j] = k[i];
+    }
+}
+
+// qbar = B^T kbar of interface column ix (3 <= ix <= nx - 4): q[0..2] for the
+// fluid row nzf - 1, q[3..7] for solid row 0, from the kbar of all 8
+// interface-row fields.  The products tiles that hold an interface row
+// evaluate it for their own columns.
+template <int DIR>
+__device__ __forceinline__ void iface_qbar(const Geo &g, const Coef &c, const AdjIn &a, int ix, float q[8]) {
+    float kb[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) kb[f] = kbar(a, iface_index(g, f, ix));
+    float M[8][8];
+    bc_matrix<DIR>(c.nif[ix], M);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float s = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s += M[i][j] * kb[i];
+        q[j] = s;
+    }
 }
 
 // Interior source tiles (the products of a cell come from the plain DRP
@@ -209,9 +242,11 @@ __device__ __forceinline__ void fluid_prod_body(const float (*sh)[TZ + 2 * HALO]
             xi_metrics<MET>(c, gi, xx, xz);
             const float gxixi = xx * xx + xz * xz;
             const bool at_row = !INT && dist_flso == 0;
-            const float qvx = at_row ? ldc(&a.qif[0 * (long long)P + ix]) : kbar(a, g.off[MSFDM_FVX] + gi);
-            const float qvz = at_row ? ldc(&a.qif[1 * (long long)P + ix]) : kbar(a, g.off[MSFDM_FVZ] + gi);
-            const float qp = at_row ? ldc(&a.qif[2 * (long long)P + ix]) : kbar(a, g.off[MSFDM_FP] + gi);
+            float qi[8];
+            if (at_row) iface_qbar<DIR>(g, c, a, ix, qi);
+            const float qvx = at_row ? qi[0] : kbar(a, g.off[MSFDM_FVX] + gi);
+            const float qvz = at_row ? qi[1] : kbar(a, g.off[MSFDM_FVZ] + gi);
+            const float qp = at_row ? qi[2] : kbar(a, g.off[MSFDM_FP] + gi);
             const float rinv = ldc(&c.rhof_inv[gi]);
             // ---- velocity equations
             float Dxb = 0.f, Dzb = 0.f;
@@ -371,11 +406,13 @@ __device__ __forceinline__ void solid_prod_body(const float (*sh)[TZ + 2 * HALO]
             const float gxixi = xx * xx + xz * xz, gzeze = zx * zx + zz * zz, gxize = xx * zx + xz * zz;
             const bool side = scol >= 0, bottom = !INT && iz >= zb0;
             const bool at_row = !INT && iz == 0;
-            const float qvx = at_row ? ldc(&a.qif[3 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SVX] + si);
-            const float qvz = at_row ? ldc(&a.qif[4 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SVZ] + si);
-            const float qsx = at_row ? ldc(&a.qif[5 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SSX] + si);
-            const float qsz = at_row ? ldc(&a.qif[6 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SSZ] + si);
-            const float qsxz = at_row ? ldc(&a.qif[7 * (long long)P + ix]) : kbar(a, g.off[MSFDM_SSXZ] + si);
+            float qi[8];
+            if (at_row) iface_qbar<DIR>(g, c, a, ix, qi);
+            const float qvx = at_row ? qi[3] : kbar(a, g.off[MSFDM_SVX] + si);
+            const float qvz = at_row ? qi[4] : kbar(a, g.off[MSFDM_SVZ] + si);
+            const float qsx = at_row ? qi[5] : kbar(a, g.off[MSFDM_SSX] + si);
+            const float qsz = at_row ? qi[6] : kbar(a, g.off[MSFDM_SSZ] + si);
+            const float qsxz = at_row ? qi[7] : kbar(a, g.off[MSFDM_SSXZ] + si);
             const float c11 = ldc(&c.C11[si]), c13 = ldc(&c.C13[si]), c33 = ldc(&c.C33[si]), c55 = ldc(&c.C55[si]);
             const float rinv = ldc(&c.rhos_inv[si]);
             const int dist = INT ? 3 : iz;
@@ -826,41 +863,6 @@ __global__ void kahan_add_kernel(double *__restrict__ total, double *__restrict_
         comp[i] = (t - total[i]) - y;
         total[i] = t;
         part[i] = 0.0;
-    }
-}
-
-// 8x8 interface BC matrix of one column (columns = unit inputs).
-template <int DIR>
-__device__ __forceinline__ void bc_matrix(float2 n, float M[8][8]) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float k[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) k[i] = i == j ? 1.0f : 0.0f;
-        interface_bc<DIR>(k, n);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) M[i][j] = k[i];
-    }
-}
-
-// qif = B^T kbar on the interface rows, kbar = clam * LP + cprev * TBp.
-template <int DIR>
-__global__ void __launch_bounds__(128) adj_iface_q(Geo g, Coef c, AdjIn a0, float *qif) {
-    const AdjIn a = adj_of_shot(a0, blockIdx.y);
-    qif += (long long)blockIdx.y * a0.qstride;
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ix < 3 || ix > g.nx - 4) return;
-    float kb[8];
-#pragma unroll
-    for (int f = 0; f < 8; ++f) kb[f] = kbar(a, iface_index(g, f, ix));
-    float M[8][8];
-    bc_matrix<DIR>(c.nif[ix], M);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float s = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) s += M[i][j] * kb[i];
-        qif[(long long)j * g.P + ix] = s;
     }
 }
 

@@ -72,7 +72,6 @@ struct msfdm_solver {
     double bytes_kernel[3] = {0.0, 0.0, 0.0};
     // adjoint state: lambda, lambda', accumulator, two Tbar buffers, interface qbar
     float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    float *qif = nullptr;
     float *aprod = nullptr;          // adjoint products of one stage (adj_prod_kernel -> adj_gather_kernel)
     int2 *atiles = nullptr;          // adjoint source tiles: interior, fluid boundary, solid boundary
     int atiles_n[3] = {0, 0, 0};
@@ -414,7 +413,6 @@ int copy_field(msfdm_solver *s, float *dbuf, int f, float *host_out, const float
 
 void free_adjoint(msfdm_solver *s) {
     for (auto &p : s->adj) if (p) { (void)hipFree(p); p = nullptr; }
-    if (s->qif) { (void)hipFree(s->qif); s->qif = nullptr; }
     if (s->aprod) { (void)hipFree(s->aprod); s->aprod = nullptr; }
     if (s->atiles) { (void)hipFree(s->atiles); s->atiles = nullptr; }
     if (s->gmem) { (void)hipFree(s->gmem); s->gmem = nullptr; }
@@ -1524,7 +1522,6 @@ int ensure_adjoint(msfdm_solver *s) {
     for (auto &p : s->adj)
         if (!p) { HIPCHK(hipMalloc(&p, bytes)); HIPCHK(hipMemset(p, 0, bytes)); }
     const long long P = s->geo.P;
-    if (!s->qif) { HIPCHK(hipMalloc(&s->qif, 8 * P * sizeof(float))); HIPCHK(hipMemset(s->qif, 0, 8 * P * sizeof(float))); }
     if (!s->aprod) {   // cells no source covers stay zero
         const size_t n = (size_t)adj_prod_floats(s);
         HIPCHK(hipMalloc(&s->aprod, n * sizeof(float)));
@@ -1602,12 +1599,11 @@ template <int DIR>
 int adj_stage(msfdm_solver *s, const float *S, const float *TBp, float *TB, float clam, float cprev,
               float *ACC = nullptr, const float *LP = nullptr, bool first = false) {
     D::AdjIn a;
-    a.S = S; a.LP = LP ? LP : s->adj[A_LP]; a.qif = s->qif; a.TB = TB; a.ACC = ACC; a.prod = s->aprod;
+    a.S = S; a.LP = LP ? LP : s->adj[A_LP]; a.TB = TB; a.ACC = ACC; a.prod = s->aprod;
     a.ACCin = first ? a.LP : ACC;
     a.TBp = TBp ? TBp : a.LP;   // no previous Tbar: LP with weight 0
     a.clam = clam; a.cprev = TBp ? cprev : 0.0f;
-    a.bstride = 0; a.qstride = 0; a.pstride = 0;
-    D::adj_iface_q<DIR><<<(s->d.nx + 127) / 128, 128, 0, s->stream>>>(s->geo, s->coef, a, s->qif);
+    a.bstride = 0; a.pstride = 0;
     launch_adj<DIR>(s, s->stream, a, s->grad, 1);
     HIPCHK(hipGetLastError());
     return MSFDM_OK;
@@ -1864,7 +1860,7 @@ ShotsPlan plan_shots(const msfdm_solver *s, int nsteps, int every) {
     p.nseg = (nsteps + every - 1) / every;
     p.bufs = 6 + p.nseg + 4LL * every;
     p.pfl = adj_prod_floats(s);
-    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl + p.pfl) +
+    p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + D::KIF_ROWS * p.P + p.rfl + p.pfl) +
                  3 * sizeof(double) * (size_t)p.gfl + (size_t)every * sizeof(D::Shot);
     p.shared = sizeof(double) * (size_t)p.gfl + ((size_t)64 << 20);
     return p;
@@ -1879,7 +1875,7 @@ struct ShotsWave {
     float *fmem = nullptr;             // all float slabs
     double *gmem = nullptr;            // kc per-shot gradient sets three times (segment partial sums,
                                        // Kahan totals, compensations), then the sum over shots
-    float *U = nullptr, *ckpt = nullptr, *seg = nullptr, *kif = nullptr, *qif = nullptr, *res = nullptr;
+    float *U = nullptr, *ckpt = nullptr, *seg = nullptr, *kif = nullptr, *res = nullptr;
     float *prod = nullptr;             // adjoint products, p.pfl floats per shot
     float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double *grad = nullptr, *total = nullptr;
@@ -1899,7 +1895,7 @@ int wave_alloc(ShotsWave &w) {
     msfdm_solver *s = w.s;
     const ShotsPlan &p = w.p;
     const long long kc = w.kc;
-    const long long nfl = kc * (p.bufs * p.bf + (D::KIF_ROWS + 8) * p.P + p.rfl + p.pfl);
+    const long long nfl = kc * (p.bufs * p.bf + D::KIF_ROWS * p.P + p.rfl + p.pfl);
     if (hipMalloc(&w.fmem, nfl * sizeof(float)) != hipSuccess)
         return set_error(MSFDM_EDEVICE, "out of device memory for the multi-shot gradient slabs");
     if (hipMalloc(&w.gmem, (3 * kc + 1) * p.gfl * sizeof(double)) != hipSuccess)
@@ -1912,7 +1908,6 @@ int wave_alloc(ShotsWave &w) {
     w.ckpt = f; f += (long long)p.nseg * kc * p.bf;
     w.seg = f; f += 4LL * p.every * kc * p.bf;
     w.kif = f; f += kc * D::KIF_ROWS * p.P;
-    w.qif = f; f += kc * 8 * p.P;
     w.prod = f; f += kc * p.pfl;
     w.res = f;
     w.grad = w.gmem;
@@ -1971,14 +1966,12 @@ void wave_adj_stage(ShotsWave &w, int kk, const float *S, const float *TBp, floa
                     const float *LP, bool first = false) {
     msfdm_solver *s = w.s;
     D::AdjIn a;
-    a.S = S; a.LP = LP; a.qif = w.qif; a.TB = TB; a.ACC = w.adj[W_ACC]; a.prod = w.prod;
+    a.S = S; a.LP = LP; a.TB = TB; a.ACC = w.adj[W_ACC]; a.prod = w.prod;
     a.ACCin = first ? LP : a.ACC;
     a.TBp = TBp ? TBp : a.LP;   // no previous Tbar: LP with weight 0
     a.clam = clam; a.cprev = TBp ? cprev : 0.0f;
     a.bstride = w.p.bf;
-    a.qstride = 8 * w.p.P;
     a.pstride = w.p.pfl;
-    D::adj_iface_q<DIR><<<dim3((s->d.nx + 127) / 128, kk), 128, 0, s->stream>>>(s->geo, s->coef, a, w.qif);
     launch_adj<DIR>(s, s->stream, a, wave_grad(w), kk);
 }
 
