@@ -60,15 +60,17 @@ def cpu_baseline(model, nsteps: int, nsteps_1core: int):
     return out
 
 
-def stage2_kernel_names(solid: bool, tile_rows: int):
+def stage2_kernel_names(solid: bool, tile_rows: int, occupancy: int = 0):
     """rocprofv3 names of the RK stage-2 launch (F direction), vertical-column
     metrics first.  Template arguments: direction, stage, tile rows, metric form
     (1 = vertical grid columns), then the top kind (free surface: 1) of the
     solid-only kernel, or the CPML flag (0) and the FIN flag (0) of the
-    fluid-solid one."""
+    fluid-solid one.  Occupancy policy 1 runs the _w5 copies (vertical columns
+    only)."""
+    w5 = "_w5" if occupancy == 1 else ""
     if solid:
-        return [f"free_stage_kernel<0, 2, {tile_rows}, {m}, 1>" for m in (1, 0)]
-    return [f"stage_kernel<0, 2, {tile_rows}, {m}, 0, 0>" for m in (1, 0)]
+        return [f"free_stage_kernel{w5}<0, 2, {tile_rows}, 1, 1>"] + ([] if w5 else [f"free_stage_kernel<0, 2, {tile_rows}, 0, 1>"])
+    return [f"stage_kernel{w5}<0, 2, {tile_rows}, 1, 0, 0>"] + ([] if w5 else [f"stage_kernel<0, 2, {tile_rows}, 0, 0, 0>"])
 
 
 def measured_traffic(workload: str, kernels):
@@ -84,6 +86,10 @@ def measured_traffic(workload: str, kernels):
     for name in kernels:
         if name in table:
             return name, round(table[name]["traffic_bytes"])
+        # the occupancy copies (_w5) run the same tile code: same accesses
+        base = name.replace("_w5<", "<")
+        if base in table:
+            return name, round(table[base]["traffic_bytes"])
     return None, None
 
 
@@ -293,8 +299,8 @@ def bench_solid(args):
     t_stage4 = s.time_kernel(1, reps)
     b_stage = s.kernel_bytes(0)
     achieved = b_stage / (t_stage * 1e-3) / 1e9
-    kname, traffic = measured_traffic(args.solid_workload, stage2_kernel_names(True, s.tile_rows))
-    cpu = cpu_baseline(model, args.cpu_steps) if args.cpu_steps > 0 else None
+    kname, traffic = measured_traffic(args.solid_workload, stage2_kernel_names(True, s.tile_rows, s.occupancy))
+    cpu = cpu_baseline(model, args.cpu_steps, args.cpu_steps_1core) if args.cpu_steps > 0 else None
     out = {
         "metric": "Gcell-updates/s (plain FP32 solid-only solver, traction-free surface, 1 GPU)",
         "value": round(cells * args.steps / wall / 1e9, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
@@ -461,7 +467,7 @@ def main():
     t_rep2 = s.time_kernel(0, reps)
     t_rep4 = s.time_kernel(1, reps)
 
-    kname, traffic = measured_traffic(args.workload, stage2_kernel_names(False, tile_rows))
+    kname, traffic = measured_traffic(args.workload, stage2_kernel_names(False, tile_rows, occupancy))
     # what a plain streaming kernel reaches on this part: 1 GB streams (HBM),
     # and streams the size of one stage-2 launch's bytes (partly cache resident)
     stream = {"copy_GBps": round(1e3 * msfdm.stream_probe(1000, 1), 1),
