@@ -176,6 +176,9 @@ def test_bench_roofline_kernels_are_in_the_committed_profiles():
         name, traffic = bench.measured_traffic(workload, bench.stage2_kernel_names(solid, 8))
         assert name is not None, workload
         assert traffic > 0
+        # the occupancy copy is named as such and shares its twin's measured traffic
+        name5, traffic5 = bench.measured_traffic(workload, bench.stage2_kernel_names(solid, 8, 1))
+        assert name5 == name.replace("<", "_w5<", 1) and traffic5 == traffic
 
 
 def test_main_debug_usage():
