@@ -248,7 +248,10 @@ int msfdm_grid_metrics(const double *x, const double *z, int nz, int nx, int dev
  * RK4) runs backwards in time; the forward is recomputed per segment from
  * checkpoints taken every `every` steps (0: about sqrt(nsteps)), and the
  * gradients accumulate in FP64.  Any output pointer may be NULL.  Leaves the
- * records of the forward run readable; the wavefield is undefined afterwards. */
+ * records of the forward run readable; the wavefield is undefined afterwards.
+ * The checkpoint and segment buffers stay allocated for the next gradient
+ * (sized for the longest record seen) until msfdm_gradient_shots or
+ * msfdm_solver_destroy releases them. */
 int msfdm_gradient(msfdm_solver *s, int isrc, int nsteps, int every, const float *res_vx, const float *res_vz,
                    const float *res_p, double *gK, double *grho_f, double *gC11, double *gC13, double *gC33,
                    double *gC55, double *grho_s);
@@ -268,7 +271,9 @@ int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, const float *re
  * the free memory) and is returned in *wave_shots (may be NULL).  Per-shot
  * gradients are FP64 and are added in shot order, so the result equals the
  * in-order sum of the single-shot gradients.  The solver's wavefield and step
- * counter are left as they were. */
+ * counter are left as they were.  The checkpoint and segment buffers that
+ * msfdm_gradient keeps between calls are released first (a pending
+ * msfdm_adjoint_forward has to be repeated). */
 int msfdm_gradient_shots(msfdm_solver *s, int nshot, const int *isrc, int nsteps, int every, double mem_bytes,
                          const float *res, double *gK, double *grho_f, double *gC11, double *gC13, double *gC33,
                          double *gC55, double *grho_s, int *wave_shots);

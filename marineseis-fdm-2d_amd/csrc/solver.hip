@@ -2080,6 +2080,14 @@ extern "C" int msfdm_gradient_shots(msfdm_solver *s, int nshot, const int *isrc,
     if (int rc = grow_tables(s, nsteps)) return rc;
     if (int rc = ensure_adj_tiles(s)) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
+    // the single-shot gradient's checkpoint and segment buffers (kept between
+    // msfdm_gradient calls, tens of GB on a large grid) go back to the pool the
+    // waves are sized from; a pending msfdm_adjoint_forward has to be repeated
+    for (float *p : s->ckpt) (void)hipFree(p);
+    for (float *p : s->seg) (void)hipFree(p);
+    s->ckpt.clear();
+    s->seg.clear();
+    s->adj_nseg = 0;
     ShotsWave w;
     w.s = s;
     w.p = plan_shots(s, nsteps, every);

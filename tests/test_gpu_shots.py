@@ -8,6 +8,8 @@
 * the adjoint products as one launch or as three give the same bits, single
   shot and multi-shot;
 * matches the sum of the FP64 oracle's per-shot gradients to 1e-4 relative L2;
+* it releases the single-shot checkpoints (a pending adjoint forward must be
+  repeated; the next single-shot gradient gives the same bits);
 * a budget below one shot is refused with MSFDM_EDEVICE, a bad shot index
   with MSFDM_EINVAL.
 """
@@ -112,3 +114,19 @@ def test_adjoint_launch_modes_agree_bitwise():
             assert np.any(out[0][k][n] != 0), (k, n)
     with pytest.raises(msfdm.MsfdmError):
         s.adjoint_launch = 2
+
+
+def test_shots_release_the_single_shot_buffers():
+    print("[shots] the multi-shot gradient releases the single-shot checkpoints", flush=True)
+    m = _model()
+    s = msfdm.Solver(m)
+    s.reset(0); s.run(NSTEPS); s.sync()
+    res = _residuals(m, s.num_records())
+    g1 = s.gradient(1, NSTEPS, res[1], every=8)
+    s.adjoint_forward(1, NSTEPS, every=8)
+    s.gradient_shots([0], NSTEPS, res[:1], every=8)
+    with pytest.raises(msfdm.MsfdmError):   # the pending forward's checkpoints are gone
+        s.adjoint_backward(res[1])
+    g2 = s.gradient(1, NSTEPS, res[1], every=8)   # allocates them again
+    for n in msfdm._lib.GRADS:
+        assert np.array_equal(g1[n], g2[n]), n
