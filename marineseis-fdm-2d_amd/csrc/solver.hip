@@ -64,6 +64,8 @@ struct msfdm_solver {
     int nrec_done = 0;
     int tz = 16;                 // tile height (rows per block)
     int occ = 0;                 // occupancy policy of the stage kernels (msfdm_set_occupancy)
+    float *u_out = nullptr;      // non-null: stage 4 writes U^{n+1} here instead of over U^n
+                                 // (the adjoint's segment recomputation; never in a graph)
     int adj_launch = -1;         // adjoint products: -1 by grid size, 0 three launches, 1 one launch
     bool use_graph = true;
     hipGraph_t graph = nullptr;
@@ -295,7 +297,7 @@ void enqueue_step_dir(msfdm_solver *s, int j, bool first, bool last) {
     else launch_stage<D1, 1, 1>(s, U, T2, j, dt / 2, 0.f, dt);
     launch_stage<D2, 2>(s, T2, T3, j, dt / 2, 0.f, dt / 2);
     launch_stage<D1, 3>(s, T3, T4, j, dt, 0.f, dt / 2);
-    launch_stage<D2, 4>(s, T4, U, j, dt / 6, 0.f, dt, !last);
+    launch_stage<D2, 4>(s, T4, s->u_out ? s->u_out : U, j, dt / 6, 0.f, dt, !last);
 }
 // n steps from absolute step it0; U, the records and every interface row are
 // complete when the sequence ends (nothing reads them in between).
@@ -1708,22 +1710,26 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
     int rc = MSFDM_OK;
     for (int seg = s->adj_nseg - 1; seg >= 0 && rc == MSFDM_OK; --seg) {
         const int it0 = seg * every, it1 = std::min(nsteps, it0 + every);
-        // recompute the segment, keeping every step's stage states
-        if (hipMemcpyAsync(s->buf[R_U], s->ckpt[seg], bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+        // recompute the segment, keeping every step's stage states: the
+        // checkpoint goes straight into the segment's first U^n slot, and each
+        // step's stage 4 writes U^{n+1} into the next slot (no copy per step)
+        if (hipMemcpyAsync(s->seg[0], s->ckpt[seg], bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
             rc = set_error(MSFDM_EDEVICE, "checkpoint restore failed");
             break;
         }
         D::set_base<<<1, 64, 0, s->stream>>>(s->base, it0);
-        float *keep[3] = {s->buf[R_T2], s->buf[R_T3], s->buf[R_T4]};
+        float *keep[4] = {s->buf[R_U], s->buf[R_T2], s->buf[R_T3], s->buf[R_T4]};
         for (int it = it0; it < it1; ++it) {
             const int j = it - it0;
-            (void)hipMemcpyAsync(s->seg[4 * j], s->buf[R_U], bytes, hipMemcpyDeviceToDevice, s->stream);
+            s->buf[R_U] = s->seg[4 * j];
             s->buf[R_T2] = s->seg[4 * j + 1];
             s->buf[R_T3] = s->seg[4 * j + 2];
             s->buf[R_T4] = s->seg[4 * j + 3];
+            s->u_out = j + 1 < every ? s->seg[4 * (j + 1)] : keep[0];
             enqueue_steps(s, it, 1);
         }
-        s->buf[R_T2] = keep[0]; s->buf[R_T3] = keep[1]; s->buf[R_T4] = keep[2];
+        s->u_out = nullptr;
+        s->buf[R_U] = keep[0]; s->buf[R_T2] = keep[1]; s->buf[R_T3] = keep[2]; s->buf[R_T4] = keep[3];
         // reverse sweep over the segment
         for (int it = it1 - 1; it >= it0 && rc == MSFDM_OK; --it) {
             if (it % gap == 0)
