@@ -33,6 +33,7 @@ namespace {
 constexpr int GRAPH_STEPS = 32;   // steps captured per graph replay (even: F/B parity is baked in)
 
 enum Role { R_U = 0, R_T2 = 1, R_T3 = 2, R_T4 = 3 };
+constexpr int SHOT_UOUT = 4;   // D::Shot::buf entry that stage 4 writes
 
 inline long long align64(long long n) { return (n + 63) / 64 * 64; }
 
@@ -1121,6 +1122,7 @@ int batch_upload(msfdm_batch *b) {
         t.ctl = s->ctl;
         t.ctl.base = b->base;
         for (int r = 0; r < 4; ++r) t.buf[r] = s->buf[r];
+        t.buf[SHOT_UOUT] = s->buf[R_U];   // in place
         t.kif = s->kif;
     }
     if (h.size() == b->host_shots.size() && std::memcmp(h.data(), b->host_shots.data(), h.size() * sizeof(D::Shot)) == 0)
@@ -1185,7 +1187,7 @@ void batch_step_dir(const BatchView &b, int j, bool first = true, bool last = tr
     else batch_launch_stage<D1, 1, 1>(b, R_U, R_T2, j, dt / 2, dt);
     batch_launch_stage<D2, 2>(b, R_T2, R_T3, j, dt / 2, dt / 2);
     batch_launch_stage<D1, 3>(b, R_T3, R_T4, j, dt, dt / 2);
-    batch_launch_stage<D2, 4>(b, R_T4, R_U, j, dt / 6, dt, !last);
+    batch_launch_stage<D2, 4>(b, R_T4, SHOT_UOUT, j, dt / 6, dt, !last);
 }
 void batch_enqueue(const BatchView &b, int it0, int n) {
     for (int j = 0; j < n; ++j) {
@@ -1867,7 +1869,7 @@ ShotsPlan plan_shots(const msfdm_solver *s, int nsteps, int every) {
     p.bufs = 6 + p.nseg + 4LL * every;
     p.pfl = adj_prod_floats(s);
     p.per_shot = sizeof(float) * (size_t)(p.bufs * p.bf + D::KIF_ROWS * p.P + p.rfl + p.pfl) +
-                 3 * sizeof(double) * (size_t)p.gfl + (size_t)every * sizeof(D::Shot);
+                 3 * sizeof(double) * (size_t)p.gfl + (size_t)(every + 1) * sizeof(D::Shot);
     p.shared = sizeof(double) * (size_t)p.gfl + ((size_t)64 << 20);
     return p;
 }
@@ -1885,7 +1887,7 @@ struct ShotsWave {
     float *prod = nullptr;             // adjoint products, p.pfl floats per shot
     float *adj[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double *grad = nullptr, *total = nullptr;
-    D::Shot *tables = nullptr;         // p.every tables of kc shots
+    D::Shot *tables = nullptr;         // p.every + 1 tables of kc shots (recomputation slots, forward)
     int4 *tiles = nullptr;             // tiles of kc shots (tile height s->tz)
     int nt_shot = 0, nbx = 0, nbf = 0;
     // slab idx of a region holding one state buffer per shot
@@ -1918,7 +1920,7 @@ int wave_alloc(ShotsWave &w) {
     w.res = f;
     w.grad = w.gmem;
     w.total = w.gmem + 3 * kc * p.gfl;
-    HIPCHK(hipMalloc(&w.tables, (size_t)p.every * kc * sizeof(D::Shot)));
+    HIPCHK(hipMalloc(&w.tables, (size_t)(p.every + 1) * kc * sizeof(D::Shot)));
     const int tz = s->tz;
     w.nbx = (s->d.nx - 4) / D::TX + 1;
     w.nbf = (s->d.nz_fluid - 3 + tz - 1) / tz;
@@ -2012,8 +2014,11 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
     const ShotsPlan &p = w.p;
     const int every = p.every, gap = s->gap;
     const size_t sbytes = (size_t)kk * p.bf * sizeof(float);
-    std::vector<D::Shot> h((size_t)every * w.kc);
-    for (int j = 0; j < every; ++j)
+    // table j < every: step j of a segment's recomputation, U^n in slot j of the
+    // segment storage and U^{n+1} written into slot j + 1 (no copy per step);
+    // table `every`: the checkpointed forward, in place on U
+    std::vector<D::Shot> h((size_t)(every + 1) * w.kc);
+    for (int j = 0; j <= every; ++j)
         for (int i = 0; i < kk; ++i) {
             D::Shot &t = h[(size_t)j * w.kc + i];
             t.g = s->geo;
@@ -2025,8 +2030,11 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
             t.ctl.gap = 0x7fffffff;
             t.ctl.srcx = s->srcx.empty() ? -100 : s->srcx[isrc[i]];
             t.ctl.srcz = s->srcz.empty() ? -100 : s->srcz[isrc[i]];
-            t.buf[0] = w.U + (long long)i * p.bf;
-            for (int r = 1; r < 4; ++r) t.buf[r] = w.slab(w.seg, 4LL * j + r) + (long long)i * p.bf;
+            const long long o = (long long)i * p.bf;
+            const int js = j < every ? j : 0;   // the forward's stage states go to slot 0
+            t.buf[0] = (j < every ? w.slab(w.seg, 4LL * j) : w.U) + o;
+            for (int r = 1; r < 4; ++r) t.buf[r] = w.slab(w.seg, 4LL * js + r) + o;
+            t.buf[SHOT_UOUT] = (j + 1 < every ? w.slab(w.seg, 4LL * (j + 1)) : w.U) + o;
             t.kif = w.kif + (long long)i * D::KIF_ROWS * p.P;
         }
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2041,19 +2049,15 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
     for (int it = 0; it < nsteps; ++it) {
         if (it % every == 0)
             HIPCHK(hipMemcpyAsync(w.slab(w.ckpt, it / every), w.U, sbytes, hipMemcpyDeviceToDevice, s->stream));
-        wave_step(w, kk, 0, it);
+        wave_step(w, kk, every, it);
     }
     HIPCHK(hipGetLastError());
     // reverse sweep, segment by segment
     for (int sg = p.nseg - 1; sg >= 0; --sg) {
         const int it0 = sg * every, it1 = std::min(nsteps, it0 + every);
-        HIPCHK(hipMemcpyAsync(w.U, w.slab(w.ckpt, sg), sbytes, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(w.slab(w.seg, 0), w.slab(w.ckpt, sg), sbytes, hipMemcpyDeviceToDevice, s->stream));
         D::set_base<<<1, 64, 0, s->stream>>>(s->base, it0);
-        for (int it = it0; it < it1; ++it) {
-            const int j = it - it0;
-            HIPCHK(hipMemcpyAsync(w.slab(w.seg, 4LL * j), w.U, sbytes, hipMemcpyDeviceToDevice, s->stream));
-            wave_step(w, kk, j, it);
-        }
+        for (int it = it0; it < it1; ++it) wave_step(w, kk, it - it0, it);
         for (int it = it1 - 1; it >= it0; --it) {
             if (it % gap == 0)
                 D::adj_inject<<<dim3((s->d.nx + 255) / 256, kk), 256, 0, s->stream>>>(s->geo, w.adj[W_LAM], w.res, p.nrec,

@@ -1353,7 +1353,8 @@ struct Shot {
     Geo g;
     Coef c;
     Ctl ctl;
-    float *buf[4];    // U, T2, T3, T4
+    float *buf[5];    // U, T2, T3, T4, then where stage 4 writes U^{n+1}: U itself, or (the
+                      // gradient's segment recomputation) the next step's U^n slot
     float *kif;
 };
 
@@ -1400,7 +1401,7 @@ batch_iface_final(const Shot *__restrict__ shots, float c12, float c3, float cb,
     __shared__ float pre[8][FB];
     const Shot &s = shots[blockIdx.y];
     if ((int)blockIdx.x * FO > s.g.nx) return;   // whole block beyond this shot's grid
-    iface_final_body<DIR>(pre, s.g, s.c, s.ctl, s.buf[0], s.kif, c12, c3, cb, jstep, blockIdx.x * FO);
+    iface_final_body<DIR>(pre, s.g, s.c, s.ctl, s.buf[4], s.kif, c12, c3, cb, jstep, blockIdx.x * FO);
 }
 
 __global__ void advance_base(int *base, int n) {
