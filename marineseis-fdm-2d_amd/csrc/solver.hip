@@ -1728,7 +1728,9 @@ extern "C" int msfdm_adjoint_backward(msfdm_solver *s, const float *res_vx, cons
             s->buf[R_T3] = s->seg[4 * j + 2];
             s->buf[R_T4] = s->seg[4 * j + 3];
             s->u_out = j + 1 < every ? s->seg[4 * (j + 1)] : keep[0];
-            enqueue_steps(s, it, 1);
+            // one complete step at offset j from the counter set to it0 above
+            if (it & 1) enqueue_step_dir<1>(s, j, true, true);
+            else enqueue_step_dir<0>(s, j, true, true);
         }
         s->u_out = nullptr;
         s->buf[R_U] = keep[0]; s->buf[R_T2] = keep[1]; s->buf[R_T3] = keep[2]; s->buf[R_T4] = keep[3];
@@ -1951,11 +1953,12 @@ BatchView wave_view(const ShotsWave &w, int kk, int slot) {
     return v;
 }
 
-void wave_step(const ShotsWave &w, int kk, int slot, int it) {
+// Step `it` of the wave with the tables of `slot`; jstep: the step's offset
+// from the device step counter (set once per pass, not advanced per step).
+void wave_step(const ShotsWave &w, int kk, int slot, int it, int jstep) {
     const BatchView v = wave_view(w, kk, slot);
-    if (it & 1) batch_step_dir<1>(v, 0);
-    else batch_step_dir<0>(v, 0);
-    D::advance_base<<<1, 64, 0, v.stream>>>(v.base, 1);
+    if (it & 1) batch_step_dir<1>(v, jstep);
+    else batch_step_dir<0>(v, jstep);
 }
 
 D::Grad wave_grad(const ShotsWave &w) {
@@ -2049,7 +2052,7 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
     for (int it = 0; it < nsteps; ++it) {
         if (it % every == 0)
             HIPCHK(hipMemcpyAsync(w.slab(w.ckpt, it / every), w.U, sbytes, hipMemcpyDeviceToDevice, s->stream));
-        wave_step(w, kk, every, it);
+        wave_step(w, kk, every, it, it);
     }
     HIPCHK(hipGetLastError());
     // reverse sweep, segment by segment
@@ -2057,7 +2060,7 @@ int wave_run(ShotsWave &w, int kk, const int *isrc, int nsteps, const float *res
         const int it0 = sg * every, it1 = std::min(nsteps, it0 + every);
         HIPCHK(hipMemcpyAsync(w.slab(w.seg, 0), w.slab(w.ckpt, sg), sbytes, hipMemcpyDeviceToDevice, s->stream));
         D::set_base<<<1, 64, 0, s->stream>>>(s->base, it0);
-        for (int it = it0; it < it1; ++it) wave_step(w, kk, it - it0, it);
+        for (int it = it0; it < it1; ++it) wave_step(w, kk, it - it0, it, it - it0);
         for (int it = it1 - 1; it >= it0; --it) {
             if (it % gap == 0)
                 D::adj_inject<<<dim3((s->d.nx + 255) / 256, kk), 256, 0, s->stream>>>(s->geo, w.adj[W_LAM], w.res, p.nrec,
