@@ -106,6 +106,7 @@ def bench_batch(args, nsteps=None):
     nsteps = nsteps or args.batch_steps
     b = msfdm.Batch(models)
     best = b.autotune(64)
+    occ = b.occupancy
     b.reset()
     b.run(64)
     b.sync()
@@ -116,11 +117,13 @@ def bench_batch(args, nsteps=None):
     b.close()
     probe = msfdm.Solver(models[0])
     best_seq = probe.autotune(64)
+    occ_seq = probe.occupancy
     probe.close()
     t_seq = 0.0
     for m in models:
         s = msfdm.Solver(m)
         s.tile_rows = best_seq
+        s.occupancy = occ_seq
         s.reset(0)
         s.run(64)
         s.sync()
@@ -136,7 +139,8 @@ def bench_batch(args, nsteps=None):
            "speedup_vs_sequential": round(t_seq / t_batch, 3),
            "batched_Gcell_per_s": round(cells * nsteps / t_batch / 1e9, 3),
            "sequential_Gcell_per_s": round(cells * nsteps / t_seq / 1e9, 3),
-           "tile_rows": best, "tile_rows_sequential": best_seq, "steps_timed": nsteps, "shots": len(models), "total_cells": cells,
+           "tile_rows": best, "occupancy_policy": occ, "tile_rows_sequential": best_seq,
+           "occupancy_policy_sequential": occ_seq, "steps_timed": nsteps, "shots": len(models), "total_cells": cells,
            "note": f"timed {nsteps} of 4000 steps per shot after 64 warm-up steps; shots/s scaled x{scale:g}"}
     return out
 

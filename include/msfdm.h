@@ -209,7 +209,7 @@ int msfdm_kernel_bytes(const msfdm_solver *s, int which, double *bytes);
  * three times, the candidates interleaved, keeps the one with the fastest
  * single timing (msfdm_get_tile_rows / msfdm_get_occupancy read it back) and
  * resets the shot (no reference counterpart).  msfdm_batch_autotune does the
- * same over the tile heights for a batch. */
+ * same for a batch (msfdm_batch_get_tile_rows / msfdm_batch_get_occupancy). */
 int msfdm_autotune(msfdm_solver *s, int nsteps, int *best_rows);
 
 /* Replace the medium arrays of an existing solver (each [rows][nx] float32 in
@@ -307,6 +307,11 @@ int msfdm_batch_size(const msfdm_batch *b, int *nshots);
 int msfdm_batch_shot(msfdm_batch *b, int shot, msfdm_solver **out);
 int msfdm_batch_set_tile_rows(msfdm_batch *b, int rows);
 int msfdm_batch_get_tile_rows(const msfdm_batch *b, int *rows);
+/* Occupancy policy of the batched stage kernels, as msfdm_set_occupancy (1
+ * needs vertical grid columns in every shot).  The multi-shot gradient's
+ * waves use the solver's own policy.  No reference counterpart. */
+int msfdm_batch_set_occupancy(msfdm_batch *b, int policy);
+int msfdm_batch_get_occupancy(const msfdm_batch *b, int *policy);
 int msfdm_batch_set_graph(msfdm_batch *b, int enable);
 int msfdm_batch_time_run(msfdm_batch *b, int nsteps, float *ms);
 int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows);

@@ -1094,6 +1094,7 @@ struct msfdm_batch {
     int4 *d_tiles[2] = {nullptr, nullptr};   // tile tables for 8- and 16-row tiles
     int ntiles[2] = {0, 0};
     int tz = 16;
+    int occ = 0;      // occupancy policy of the batched stage kernels (msfdm_batch_set_occupancy)
     int maxnx = 0;
     int *base = nullptr;
     int host_it = 0;
@@ -1145,6 +1146,7 @@ struct BatchView {
     float dt;
     hipStream_t stream;
     bool vert;   // every shot has vertical grid columns: the MET = 1 kernels
+    int occ;     // occupancy policy (1: the _w5 kernels, vertical columns only)
 };
 bool all_vertical(const std::vector<msfdm_solver *> &shots) {
     for (const msfdm_solver *s : shots)
@@ -1153,14 +1155,17 @@ bool all_vertical(const std::vector<msfdm_solver *> &shots) {
 }
 BatchView view_of(const msfdm_batch *b) {
     return BatchView{b->d_shots, {b->d_tiles[0], b->d_tiles[1]}, {b->ntiles[0], b->ntiles[1]}, b->tz,
-                     (int)b->shots.size(), b->maxnx, b->base, b->dt, b->stream, all_vertical(b->shots)};
+                     (int)b->shots.size(), b->maxnx, b->base, b->dt, b->stream, all_vertical(b->shots), b->occ};
 }
 
 template <int DIR, int STAGE, int TZ, int FIN>
 void batch_launch_media(const BatchView &b, int rS, int rO, int j, float ct, float cprev) {
     const int k = TZ == 8 ? 0 : 1;
     const int grid = 8 * ((b.ntiles[k] + 7) / 8);
-    if (b.vert)
+    if (b.vert && b.occ == 1)
+        D::batch_stage_kernel_w5<DIR, STAGE, TZ, FIN><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k],
+                                                                                       rS, rO, j, ct, cprev);
+    else if (b.vert)
         D::batch_stage_kernel<DIR, STAGE, TZ, 1, FIN><<<grid, D::NTHR, 0, b.stream>>>(b.shots, b.tiles[k], b.ntiles[k],
                                                                                        rS, rO, j, ct, cprev);
     else
@@ -1363,6 +1368,26 @@ extern "C" int msfdm_batch_get_tile_rows(const msfdm_batch *b, int *rows) {
     return MSFDM_OK;
 }
 
+extern "C" int msfdm_batch_set_occupancy(msfdm_batch *b, int policy) {
+    clear_error();
+    if (!b) return set_error(MSFDM_EINVAL, "null batch");
+    if (policy != 0 && policy != 1) return set_error(MSFDM_EINVAL, "occupancy policy must be 0 or 1");
+    if (policy == 1 && !all_vertical(b->shots))
+        return set_error(MSFDM_EINVAL, "occupancy policy 1 needs vertical grid columns in every shot");
+    if (policy != b->occ) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        batch_drop_graph(b);
+        b->occ = policy;
+    }
+    return MSFDM_OK;
+}
+
+extern "C" int msfdm_batch_get_occupancy(const msfdm_batch *b, int *policy) {
+    if (!b || !policy) return set_error(MSFDM_EINVAL, "null argument");
+    *policy = b->occ;
+    return MSFDM_OK;
+}
+
 extern "C" int msfdm_batch_set_graph(msfdm_batch *b, int enable) {
     if (!b) return set_error(MSFDM_EINVAL, "null batch");
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1404,20 +1429,27 @@ constexpr int AUTOTUNE_ROWS[2] = {8, 16};
 extern "C" int msfdm_batch_autotune(msfdm_batch *b, int nsteps, int *best_rows) {
     clear_error();
     if (!b || nsteps <= 0) return set_error(MSFDM_EINVAL, "bad argument");
-    float best[2] = {1e30f, 1e30f};
+    const int nocc = all_vertical(b->shots) ? 2 : 1;
+    float best[2][2] = {{1e30f, 1e30f}, {1e30f, 1e30f}};   // [tile height][policy]
     for (int round = 0; round < AUTOTUNE_ROUNDS; ++round)
-        for (int k = 0; k < 2; ++k) {
-            if (int rc = msfdm_batch_set_tile_rows(b, AUTOTUNE_ROWS[k])) return rc;
-            if (int rc = msfdm_batch_reset(b)) return rc;
-            if (int rc = msfdm_batch_run(b, 2 * GRAPH_STEPS)) return rc;
-            float ms = 0.f;
-            if (int rc = msfdm_batch_time_run(b, nsteps, &ms)) return rc;
-            best[k] = std::min(best[k], ms);
-        }
-    const int arg = best[1] < best[0] ? AUTOTUNE_ROWS[1] : AUTOTUNE_ROWS[0];
-    if (int rc = msfdm_batch_set_tile_rows(b, arg)) return rc;
+        for (int k = 0; k < 2; ++k)
+            for (int o = 0; o < nocc; ++o) {
+                if (int rc = msfdm_batch_set_tile_rows(b, AUTOTUNE_ROWS[k])) return rc;
+                if (int rc = msfdm_batch_set_occupancy(b, o)) return rc;
+                if (int rc = msfdm_batch_reset(b)) return rc;
+                if (int rc = msfdm_batch_run(b, 2 * GRAPH_STEPS)) return rc;
+                float ms = 0.f;
+                if (int rc = msfdm_batch_time_run(b, nsteps, &ms)) return rc;
+                best[k][o] = std::min(best[k][o], ms);
+            }
+    int bk = 0, bo = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int o = 0; o < nocc; ++o)
+            if (best[k][o] < best[bk][bo]) { bk = k; bo = o; }
+    if (int rc = msfdm_batch_set_tile_rows(b, AUTOTUNE_ROWS[bk])) return rc;
+    if (int rc = msfdm_batch_set_occupancy(b, bo)) return rc;
     if (int rc = msfdm_batch_reset(b)) return rc;
-    if (best_rows) *best_rows = arg;
+    if (best_rows) *best_rows = AUTOTUNE_ROWS[bk];
     return MSFDM_OK;
 }
 
@@ -1950,6 +1982,7 @@ BatchView wave_view(const ShotsWave &w, int kk, int slot) {
     v.dt = w.s->d.dt;
     v.stream = w.s->stream;
     v.vert = w.s->coef.vert != 0;   // every shot of a wave shares the solver's grid
+    v.occ = w.s->occ;               // and its occupancy policy
     return v;
 }
 

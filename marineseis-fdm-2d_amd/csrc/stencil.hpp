@@ -1361,13 +1361,11 @@ struct Shot {
 // tiles[t] = (shot, band, bx, fluid bands of that shot); FIN as in stage_kernel.
 // MET as in stage_kernel: 1 when every shot of the launch has vertical grid
 // columns (xi_x a constant, xi_z = 0), 2 to decide per shot at run time.
-template <int DIR, int STAGE, int TZ, int MET, int FIN = 0>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(stage_min_waves<STAGE>(), stage_max_waves<STAGE>())))
-batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
-                   int jstep, float ct, float cprev) {
+template <int DIR, int STAGE, int TZ, int MET, int FIN>
+__device__ __forceinline__ void batch_stage_block(float (*sh)[TZ + 2 * HALO][LXP], float (*fin)[FW],
+                                                  const Shot *__restrict__ shots, const int4 *__restrict__ tiles,
+                                                  int ntiles, int rS, int rO, int jstep, float ct, float cprev) {
     static_assert(FIN == 0 || STAGE == 1, "only stage 1 finishes the previous step");
-    __shared__ float sh[5][TZ + 2 * HALO][LXP];
-    __shared__ float fin[FIN ? 8 : 1][FW];
     const int per_xcd = gridDim.x >> 3;
     const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (t >= ntiles) return;
@@ -1392,6 +1390,27 @@ batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tile
             solid_body<DIR, STAGE, TZ, MET, TOP_IFACE, FIN, 0>(sh, s.g, s.c, s.ctl, S, s.buf[0], s.buf[1], s.buf[2], O,
                                                                s.kif, jstep, ct, 0.0f, cprev, tl.z, sb, fin);
     }
+}
+
+
+template <int DIR, int STAGE, int TZ, int MET, int FIN = 0>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(stage_min_waves<STAGE>(), stage_max_waves<STAGE>())))
+batch_stage_kernel(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
+                   int jstep, float ct, float cprev) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    __shared__ float fin[FIN ? 8 : 1][FW];
+    batch_stage_block<DIR, STAGE, TZ, MET, FIN>(sh, fin, shots, tiles, ntiles, rS, rO, jstep, ct, cprev);
+}
+
+// The batch under occupancy policy 1 (at least 5 waves per SIMD), vertical
+// columns only.
+template <int DIR, int STAGE, int TZ, int FIN = 0>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MORE_WAVES_MIN, 8)))
+batch_stage_kernel_w5(const Shot *__restrict__ shots, const int4 *__restrict__ tiles, int ntiles, int rS, int rO,
+                      int jstep, float ct, float cprev) {
+    __shared__ float sh[5][TZ + 2 * HALO][LXP];
+    __shared__ float fin[FIN ? 8 : 1][FW];
+    batch_stage_block<DIR, STAGE, TZ, 1, FIN>(sh, fin, shots, tiles, ntiles, rS, rO, jstep, ct, cprev);
 }
 
 // grid (column blocks of the widest shot, shots)

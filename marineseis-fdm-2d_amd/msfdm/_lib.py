@@ -44,6 +44,7 @@ EXPORTS = [
     "msfdm_run_cli", "msfdm_autotune", "msfdm_batch_create", "msfdm_batch_destroy",
     "msfdm_batch_reset", "msfdm_batch_run", "msfdm_batch_synchronize", "msfdm_batch_size",
     "msfdm_batch_shot", "msfdm_batch_set_tile_rows", "msfdm_batch_get_tile_rows",
+    "msfdm_batch_set_occupancy", "msfdm_batch_get_occupancy",
     "msfdm_batch_set_graph", "msfdm_batch_time_run", "msfdm_batch_autotune",
     "msfdm_gradient", "msfdm_adjoint_forward", "msfdm_adjoint_backward",
     "msfdm_adjoint_set_input", "msfdm_adjoint_apply", "msfdm_adjoint_get_output",
@@ -126,6 +127,8 @@ def lib():
         "msfdm_batch_shot": ([vp, i, C.POINTER(vp)], i),
         "msfdm_batch_set_tile_rows": ([vp, i], i),
         "msfdm_batch_get_tile_rows": ([vp, ip], i),
+        "msfdm_batch_set_occupancy": ([vp, i], i),
+        "msfdm_batch_get_occupancy": ([vp, ip], i),
         "msfdm_batch_set_graph": ([vp, i], i),
         "msfdm_batch_time_run": ([vp, i, fp], i),
         "msfdm_batch_autotune": ([vp, i, ip], i),
@@ -570,6 +573,17 @@ class Batch:
     @tile_rows.setter
     def tile_rows(self, rows: int):
         check(lib().msfdm_batch_set_tile_rows(self.h, int(rows)))
+
+    @property
+    def occupancy(self) -> int:
+        """Occupancy policy of the batched stage kernels (as Solver.occupancy)."""
+        v = C.c_int()
+        check(lib().msfdm_batch_get_occupancy(self.h, C.byref(v)))
+        return v.value
+
+    @occupancy.setter
+    def occupancy(self, policy: int):
+        check(lib().msfdm_batch_set_occupancy(self.h, int(policy)))
 
     def set_graph(self, enable: bool):
         check(lib().msfdm_batch_set_graph(self.h, int(bool(enable))))

@@ -1,6 +1,7 @@
 """GPU: batched shots (one launch per RK stage for all shots) are bit-identical
 to stepping each shot alone, for uneven grid sizes, both tile heights, graph
-replay on and off; the autotuner returns a valid tile height."""
+replay on and off, both occupancy policies; the autotuner returns a valid tile
+height and policy."""
 import numpy as np
 import pytest
 
@@ -15,12 +16,15 @@ def _shots():
                                 water_depth=100.0)
 
 
-@pytest.mark.parametrize("tile_rows,graph", [(8, True), (16, True), (16, False)])
-def test_batch_matches_single_shots_bitwise(tile_rows, graph):
+@pytest.mark.parametrize("tile_rows,graph,occupancy", [(8, True, 0), (16, True, 0), (16, False, 0), (8, True, 1)])
+def test_batch_matches_single_shots_bitwise(tile_rows, graph, occupancy):
     models = _shots()
-    print(f"[batch] sizes {[(m.nx, m.nz) for m in models]} tile_rows={tile_rows} graph={graph}", flush=True)
+    print(f"[batch] sizes {[(m.nx, m.nz) for m in models]} tile_rows={tile_rows} graph={graph} "
+          f"occupancy={occupancy}", flush=True)
     b = msfdm.Batch(models)
     b.tile_rows = tile_rows
+    b.occupancy = occupancy   # the single shots below keep policy 0: same bits either way
+    assert b.occupancy == occupancy
     b.set_graph(graph)
     b.reset()
     b.run(101)
@@ -41,7 +45,9 @@ def test_batch_reset_and_autotune():
     models = _shots()
     b = msfdm.Batch(models)
     best = b.autotune(32)
-    assert best in (8, 16) and b.tile_rows == best
+    assert best in (8, 16) and b.tile_rows == best and b.occupancy in (0, 1)
+    with pytest.raises(msfdm.MsfdmError):
+        b.occupancy = 2
     b.run(40); b.sync()
     r1 = [x.copy() for x in b.shot(2).records()]
     b.reset(); b.run(40); b.sync()
